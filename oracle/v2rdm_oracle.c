@@ -1,0 +1,1369 @@
+/*
+ * v2rdm_oracle.c -- FP64 CPU oracle (TEST INFRASTRUCTURE ONLY; see header).
+ *
+ * Plain-loop restatement of edeprince3/hilbert's v2RDM-CASSCF setup and
+ * DQG+spin constraint maps.  Au and ATu are written as separate loops, as in
+ * the reference, so that the adjointness test checks one against the other.
+ */
+#define _GNU_SOURCE
+#include "v2rdm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- helpers ------------------------------------------------------------ */
+#define NMO      (p->nmo)
+#define SYM(i)   (p->symmetry[(i)])
+#define GAB(h)   (p->gems_ab[(h)])
+#define GAA(h)   (p->gems_aa[(h)])
+#define IAB(h, i, j) (p->ibas_ab[(h)][(i) * NMO + (j)])
+#define IAA(h, i, j) (p->ibas_aa[(h)][(i) * NMO + (j)])
+#define BAB0(h, n) (p->bas_ab[(h)][2 * (n)])
+#define BAB1(h, n) (p->bas_ab[(h)][2 * (n) + 1])
+#define BAA0(h, n) (p->bas_aa[(h)][2 * (n)])
+#define BAA1(h, n) (p->bas_aa[(h)][2 * (n) + 1])
+/* element (i,j) of a one-body block, i and j absolute orbital indices of one irrep */
+#define OB(off, i, j) ((off)[SYM(i)] + (long)((i) - p->pitzer_offset[SYM(i)]) * p->amopi[SYM(i)] \
+                       + ((j) - p->pitzer_offset[SYM(i)]))
+#define SGN(a, b) ((a) < (b) ? 1.0 : -1.0)
+
+static double wall(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return t.tv_sec + 1e-9 * t.tv_nsec;
+}
+
+/* ---- setup -------------------------------------------------------------- */
+
+/* basis.cc:47-76 (product table = XOR of Cotton indices), 104-111 (orbital
+ * irreps), 413-419 (pitzer_offset), 427-533 (geminal tables). */
+static void build_tables(orc_problem *p) {
+    int n = p->nmo;
+    memset(p->amopi, 0, sizeof(p->amopi));
+    for (int i = 0; i < n; i++) p->amopi[p->symmetry[i]]++;
+    int acc = 0;
+    for (int h = 0; h < p->nirrep; h++) { p->pitzer_offset[h] = acc; acc += p->amopi[h]; }
+    for (int h = 0; h < p->nirrep; h++) {
+        p->ibas_ab[h] = (int *)malloc(sizeof(int) * n * n);
+        p->ibas_aa[h] = (int *)malloc(sizeof(int) * n * n);
+        p->bas_ab[h] = (int *)malloc(sizeof(int) * 2 * (n * n + 1));
+        p->bas_aa[h] = (int *)malloc(sizeof(int) * 2 * (n * n + 1));
+        for (int t = 0; t < n * n; t++) { p->ibas_ab[h][t] = -999; p->ibas_aa[h][t] = -999; }
+        for (int t = 0; t < 2 * (n * n + 1); t++) { p->bas_ab[h][t] = -999; p->bas_aa[h][t] = -999; }
+        int nab = 0, naa = 0;
+        /* outer loop over the second orbital, inner over the first: the pair
+         * pushed is (first = inner, second = outer) */
+        for (int outer = 0; outer < n; outer++) {
+            for (int inner = 0; inner < n; inner++) {
+                if ((p->symmetry[outer] ^ p->symmetry[inner]) != h) continue;
+                int a = inner, b = outer;
+                p->ibas_ab[h][a * n + b] = nab;
+                p->bas_ab[h][2 * nab] = a;
+                p->bas_ab[h][2 * nab + 1] = b;
+                nab++;
+                if (a < b) {
+                    p->ibas_aa[h][a * n + b] = naa;
+                    p->ibas_aa[h][b * n + a] = naa;
+                    p->bas_aa[h][2 * naa] = a;
+                    p->bas_aa[h][2 * naa + 1] = b;
+                    naa++;
+                }
+            }
+        }
+        p->gems_ab[h] = nab;
+        p->gems_aa[h] = naa;
+    }
+}
+
+/* v2rdm_solver.cc:3294-3450 (n_primal), 3452-3727 (n_dual), 3729-3846 (offsets). */
+static void build_offsets(orc_problem *p) {
+    int H = p->nirrep;
+    long off = 0;
+    int nb = 0;
+#define PUSH(arr, dim) do { (arr)[h] = off; off += (long)(dim) * (dim); p->dims[nb++] = (dim); } while (0)
+    for (int h = 0; h < H; h++) PUSH(p->d2aboff, GAB(h));
+    for (int h = 0; h < H; h++) PUSH(p->d2aaoff, GAA(h));
+    for (int h = 0; h < H; h++) PUSH(p->d2bboff, GAA(h));
+    if (p->constrain_spin) {
+        int singlet = (p->nalpha == p->nbeta);
+        for (int h = 0; h < H; h++) PUSH(p->d200off, singlet ? GAB(h) : 2 * GAB(h));
+    }
+    for (int h = 0; h < H; h++) PUSH(p->d1aoff, p->amopi[h]);
+    for (int h = 0; h < H; h++) PUSH(p->d1boff, p->amopi[h]);
+    for (int h = 0; h < H; h++) PUSH(p->q1aoff, p->amopi[h]);
+    for (int h = 0; h < H; h++) PUSH(p->q1boff, p->amopi[h]);
+    if (p->constrain_q2) {
+        for (int h = 0; h < H; h++) PUSH(p->q2aboff, GAB(h));
+        for (int h = 0; h < H; h++) PUSH(p->q2aaoff, GAA(h));
+        for (int h = 0; h < H; h++) PUSH(p->q2bboff, GAA(h));
+    }
+    if (p->constrain_g2) {
+        for (int h = 0; h < H; h++) PUSH(p->g2aboff, GAB(h));
+        for (int h = 0; h < H; h++) PUSH(p->g2baoff, GAB(h));
+        for (int h = 0; h < H; h++) PUSH(p->g2aaoff, 2 * GAB(h));
+    }
+#undef PUSH
+    p->n_primal = off;
+    p->nblocks = nb;
+
+    long nd = p->constrain_sz ? 3 : 1;
+    long sab = 0, saa = 0, sm = 0;
+    for (int h = 0; h < H; h++) {
+        sab += (long)GAB(h) * GAB(h);
+        saa += (long)GAA(h) * GAA(h);
+        sm += (long)p->amopi[h] * p->amopi[h];
+    }
+    nd += 2 * saa + sab;                /* hermiticity */
+    nd += 2 * sm;                       /* D1 <-> Q1 */
+    nd += (p->constrain_sz ? 4 : 2) * sm; /* contractions */
+    if (p->constrain_spin) {
+        nd += 1;
+        if (p->nalpha == p->nbeta) nd += sm + 3 * saa + 2 * sab;
+        else nd += 4 * sab;
+        if (p->constrain_g2) nd += 2 * GAB(0);
+    }
+    if (p->constrain_q2) nd += sab + 2 * saa;
+    if (p->constrain_g2) nd += 2 * sab + 4 * sab;
+    p->n_dual = nd;
+}
+
+int orc_problem_init(orc_problem *p, int nmo, int nalpha, int nbeta, int multiplicity,
+                     const int *irreps, int nirrep,
+                     int q2, int g2, int spin, int sz) {
+    memset(p, 0, sizeof(*p));
+    if (nmo <= 0 || nirrep < 1 || nirrep > ORC_MAX_IRREP) return -1;
+    if (nalpha < 0 || nbeta < 0 || nalpha > nmo || nbeta > nmo) return -2;
+    p->nmo = nmo;
+    p->nirrep = nirrep;
+    p->nalpha = nalpha;
+    p->nbeta = nbeta;
+    p->multiplicity = multiplicity;
+    p->constrain_q2 = q2;
+    p->constrain_g2 = g2;
+    p->constrain_spin = spin;
+    p->constrain_sz = sz;
+    p->symmetry = (int *)malloc(sizeof(int) * nmo);
+    for (int i = 0; i < nmo; i++) {
+        if (irreps[i] < 0 || irreps[i] >= nirrep) return -3;
+        if (i > 0 && irreps[i] < irreps[i - 1]) return -4; /* Pitzer order required */
+        p->symmetry[i] = irreps[i];
+    }
+    build_tables(p);
+    build_offsets(p);
+    return 0;
+}
+
+void orc_problem_free(orc_problem *p) {
+    for (int h = 0; h < p->nirrep; h++) {
+        free(p->ibas_ab[h]); free(p->ibas_aa[h]);
+        free(p->bas_ab[h]); free(p->bas_aa[h]);
+    }
+    free(p->symmetry);
+    memset(p, 0, sizeof(*p));
+}
+
+long orc_n_primal(const orc_problem *p) { return p->n_primal; }
+long orc_n_dual(const orc_problem *p) { return p->n_dual; }
+int orc_nblocks(const orc_problem *p) { return p->nblocks; }
+void orc_get_dims(const orc_problem *p, int *dims) { memcpy(dims, p->dims, sizeof(int) * p->nblocks); }
+void orc_get_gems(const orc_problem *p, int *gab, int *gaa) {
+    for (int h = 0; h < p->nirrep; h++) { gab[h] = GAB(h); gaa[h] = GAA(h); }
+}
+void orc_get_tables(const orc_problem *p, int *bab, int *iab, int *baa, int *iaa) {
+    int n = NMO;
+    long ob = 0, oa = 0;
+    for (int h = 0; h < p->nirrep; h++) {
+        memcpy(bab + ob, p->bas_ab[h], sizeof(int) * 2 * GAB(h)); ob += 2 * GAB(h);
+        memcpy(baa + oa, p->bas_aa[h], sizeof(int) * 2 * GAA(h)); oa += 2 * GAA(h);
+        memcpy(iab + (long)h * n * n, p->ibas_ab[h], sizeof(int) * n * n);
+        memcpy(iaa + (long)h * n * n, p->ibas_aa[h], sizeof(int) * n * n);
+    }
+}
+void orc_get_offsets(const orc_problem *p, long *off) {
+    const long *arrs[14] = {p->d2aboff, p->d2aaoff, p->d2bboff, p->d200off, p->d1aoff, p->d1boff,
+                            p->q1aoff, p->q1boff, p->q2aboff, p->q2aaoff, p->q2bboff,
+                            p->g2aboff, p->g2baoff, p->g2aaoff};
+    for (int a = 0; a < 14; a++)
+        for (int h = 0; h < p->nirrep; h++) off[a * p->nirrep + h] = arrs[a][h];
+}
+
+/* ---- b ------------------------------------------------------------------ */
+/* v2rdm_solver.cc:1826-2074 */
+void orc_build_b(const orc_problem *p, double *b) {
+    memset(b, 0, sizeof(double) * p->n_dual);
+    int H = p->nirrep;
+    double na = p->nalpha, nb = p->nbeta;
+    long r = 0;
+    if (p->constrain_sz) {
+        b[r++] = na * nb;
+        b[r++] = na * (na - 1.0);
+        b[r++] = nb * (nb - 1.0);
+    } else {
+        b[r++] = (na + nb) * (na + nb - 1.0);
+    }
+    for (int h = 0; h < H; h++) r += 2L * GAA(h) * GAA(h);
+    for (int h = 0; h < H; h++) r += (long)GAB(h) * GAB(h);
+    for (int spin = 0; spin < 2; spin++) /* D1 + Q1 = I, alpha then beta */
+        for (int h = 0; h < H; h++) {
+            int m = p->amopi[h];
+            for (int i = 0; i < m; i++) b[r + i * m + i] = 1.0;
+            r += (long)m * m;
+        }
+    for (int s = 0; s < (p->constrain_sz ? 4 : 2); s++)
+        for (int h = 0; h < H; h++) r += (long)p->amopi[h] * p->amopi[h];
+    if (p->constrain_spin) {
+        double ms = (p->multiplicity - 1.0) / 2.0;
+        b[r++] = 0.5 * (na + nb) + ms * ms - ms * (ms + 1.0);
+        if (p->nalpha == p->nbeta) {
+            for (int h = 0; h < H; h++) r += (long)p->amopi[h] * p->amopi[h];
+            for (int h = 0; h < H; h++) r += 3L * GAA(h) * GAA(h);
+            for (int h = 0; h < H; h++) r += 2L * GAB(h) * GAB(h);
+        } else {
+            for (int h = 0; h < H; h++) r += 4L * GAB(h) * GAB(h);
+        }
+        if (p->constrain_g2) r += 2L * GAB(0);
+    }
+    if (p->constrain_q2) {
+        for (int h = 0; h < H; h++) {
+            for (int ij = 0; ij < GAB(h); ij++)
+                for (int kl = 0; kl < GAB(h); kl++)
+                    b[r + (long)ij * GAB(h) + kl] =
+                        -(double)((BAB0(h, ij) == BAB0(h, kl)) && (BAB1(h, ij) == BAB1(h, kl)));
+            r += (long)GAB(h) * GAB(h);
+        }
+        for (int spin = 0; spin < 2; spin++)
+            for (int h = 0; h < H; h++) {
+                for (int ij = 0; ij < GAA(h); ij++) {
+                    int i = BAA0(h, ij), j = BAA1(h, ij);
+                    for (int kl = 0; kl < GAA(h); kl++) {
+                        int k = BAA0(h, kl), l = BAA1(h, kl);
+                        b[r + (long)ij * GAA(h) + kl] = -(double)(i == k && j == l) + (double)(i == l && j == k);
+                    }
+                }
+                r += (long)GAA(h) * GAA(h);
+            }
+    }
+    /* G2 rows have b = 0 */
+}
+
+/* ---- c ------------------------------------------------------------------ */
+static inline long g4(int n, int a, int b, int c, int d) {
+    return (((long)a * n + b) * n + c) * n + d;
+}
+
+/* c[D1a] = c[D1b] = h (tei.cc:245-271 with no core orbitals);
+ * c[D2ab](ij,kl) = (ik|jl); c[D2aa/bb](ij,kl) = (ik|jl) - (il|jk)
+ * (v2rdm_solver.cc:164-193, tei.cc:139-199). */
+void orc_pack_c(const orc_problem *p, const double *hm, const double *g, double *c) {
+    int n = NMO, H = p->nirrep;
+    memset(c, 0, sizeof(double) * p->n_primal);
+    for (int h = 0; h < H; h++) {
+        int m = p->amopi[h], o = p->pitzer_offset[h];
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+                double v = hm[(long)(i + o) * n + (j + o)];
+                c[p->d1aoff[h] + (long)i * m + j] = v;
+                c[p->d1boff[h] + (long)i * m + j] = v;
+            }
+        for (int ij = 0; ij < GAB(h); ij++) {
+            int i = BAB0(h, ij), j = BAB1(h, ij);
+            for (int kl = 0; kl < GAB(h); kl++) {
+                int k = BAB0(h, kl), l = BAB1(h, kl);
+                c[p->d2aboff[h] + (long)ij * GAB(h) + kl] = g[g4(n, i, k, j, l)];
+            }
+        }
+        for (int ij = 0; ij < GAA(h); ij++) {
+            int i = BAA0(h, ij), j = BAA1(h, ij);
+            for (int kl = 0; kl < GAA(h); kl++) {
+                int k = BAA0(h, kl), l = BAA1(h, kl);
+                double v = g[g4(n, i, k, j, l)] - g[g4(n, i, l, j, k)];
+                c[p->d2aaoff[h] + (long)ij * GAA(h) + kl] = v;
+                c[p->d2bboff[h] + (long)ij * GAA(h) + kl] = v;
+            }
+        }
+    }
+}
+
+/* Exact transpose of orc_pack_c. */
+void orc_pack_c_adjoint(const orc_problem *p, const double *xb, double *hb, double *gb) {
+    int n = NMO, H = p->nirrep;
+    memset(hb, 0, sizeof(double) * n * n);
+    memset(gb, 0, sizeof(double) * (long)n * n * n * n);
+    for (int h = 0; h < H; h++) {
+        int m = p->amopi[h], o = p->pitzer_offset[h];
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++)
+                hb[(long)(i + o) * n + (j + o)] += xb[p->d1aoff[h] + (long)i * m + j]
+                                                 + xb[p->d1boff[h] + (long)i * m + j];
+        for (int ij = 0; ij < GAB(h); ij++) {
+            int i = BAB0(h, ij), j = BAB1(h, ij);
+            for (int kl = 0; kl < GAB(h); kl++) {
+                int k = BAB0(h, kl), l = BAB1(h, kl);
+                gb[g4(n, i, k, j, l)] += xb[p->d2aboff[h] + (long)ij * GAB(h) + kl];
+            }
+        }
+        for (int ij = 0; ij < GAA(h); ij++) {
+            int i = BAA0(h, ij), j = BAA1(h, ij);
+            for (int kl = 0; kl < GAA(h); kl++) {
+                int k = BAA0(h, kl), l = BAA1(h, kl);
+                double v = xb[p->d2aaoff[h] + (long)ij * GAA(h) + kl] + xb[p->d2bboff[h] + (long)ij * GAA(h) + kl];
+                gb[g4(n, i, k, j, l)] += v;
+                gb[g4(n, i, l, j, k)] -= v;
+            }
+        }
+    }
+}
+
+/* oei.cc:76-95: -t between ring neighbours; tei.cc:285-291: U on site. */
+void orc_hubbard_integrals(int n, double t, double U, double *hm, double *g) {
+    memset(hm, 0, sizeof(double) * n * n);
+    memset(g, 0, sizeof(double) * (long)n * n * n * n);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            int d = abs(i - j);
+            if (d == 1 || d == n - 1) hm[i * n + j] = -t;
+        }
+    for (int i = 0; i < n; i++) g[g4(n, i, i, i, i)] = U;
+}
+
+/* ---- A u ---------------------------------------------------------------- */
+/* v2rdm_solver.cc:2995-3050 (dispatch order D2, spin, Q2, G2). */
+void orc_Au(const orc_problem *p, const double *u, double *A) {
+    const int n = NMO, H = p->nirrep;
+    const double na = p->nalpha, nb = p->nbeta;
+    memset(A, 0, sizeof(double) * p->n_dual);
+    long r = 0;
+
+    /* d2.cc:316-380 traces */
+    if (p->constrain_sz) {
+        double tab = 0.0, taa = 0.0, tbb = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                int h = SYM(i) ^ SYM(j);
+                int ij = IAB(h, i, j);
+                tab += u[p->d2aboff[h] + (long)ij * GAB(h) + ij];
+            }
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                if (i == j) continue;
+                int h = SYM(i) ^ SYM(j);
+                int ij = IAA(h, i, j);
+                taa += u[p->d2aaoff[h] + (long)ij * GAA(h) + ij];
+                tbb += u[p->d2bboff[h] + (long)ij * GAA(h) + ij];
+            }
+        A[r++] = tab; A[r++] = taa; A[r++] = tbb;
+    } else {
+        double t = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                int h = SYM(i) ^ SYM(j);
+                int ij = IAB(h, i, j);
+                t += 2.0 * u[p->d2aboff[h] + (long)ij * GAB(h) + ij];
+                if (i == j) continue;
+                ij = IAA(h, i, j);
+                t += u[p->d2aaoff[h] + (long)ij * GAA(h) + ij];
+                t += u[p->d2bboff[h] + (long)ij * GAA(h) + ij];
+            }
+        A[r++] = t;
+    }
+
+    /* d2.cc:382-414 hermiticity: aa, bb, ab */
+    for (int blk = 0; blk < 3; blk++)
+        for (int h = 0; h < H; h++) {
+            const long *off = blk == 0 ? p->d2aaoff : blk == 1 ? p->d2bboff : p->d2aboff;
+            int g = blk == 2 ? GAB(h) : GAA(h);
+            for (int ij = 0; ij < g; ij++)
+                for (int kl = 0; kl < g; kl++)
+                    A[r++] = u[off[h] + (long)ij * g + kl] - u[off[h] + (long)kl * g + ij];
+        }
+
+    /* d2.cc:416-434 D1(j,i) + Q1(i,j) */
+    for (int s = 0; s < 2; s++)
+        for (int h = 0; h < H; h++) {
+            int m = p->amopi[h];
+            const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+            const long q1 = s == 0 ? p->q1aoff[h] : p->q1boff[h];
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++)
+                    A[r + (long)i * m + j] = u[d1 + (long)j * m + i] + u[q1 + (long)i * m + j];
+            r += (long)m * m;
+        }
+
+    /* d2.cc:442-592 contractions */
+    if (!p->constrain_sz) {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int m = p->amopi[h], o = p->pitzer_offset[h];
+                const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+                const long *d2s = s == 0 ? p->d2aaoff : p->d2bboff;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        int ii = i + o, jj = j + o;
+                        double acc = (na + nb - 1.0) * u[d1 + (long)i * m + j];
+                        for (int k = 0; k < n; k++) {
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = s == 0 ? IAB(h2, ii, k) : IAB(h2, k, ii);
+                            int jk = s == 0 ? IAB(h2, jj, k) : IAB(h2, k, jj);
+                            acc -= u[p->d2aboff[h2] + (long)ik * GAB(h2) + jk];
+                        }
+                        for (int k = 0; k < n; k++) {
+                            if (k == ii || k == jj) continue;
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = IAA(h2, ii, k), jk = IAA(h2, jj, k);
+                            acc -= SGN(ii, k) * SGN(jj, k) * u[d2s[h2] + (long)ik * GAA(h2) + jk];
+                        }
+                        A[r + (long)i * m + j] = acc;
+                    }
+                r += (long)m * m;
+            }
+    } else {
+        /* D2ab -> D1a, D2ab -> D1b */
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int m = p->amopi[h], o = p->pitzer_offset[h];
+                const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+                double pre = s == 0 ? nb : na;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        int ii = i + o, jj = j + o;
+                        double acc = pre * u[d1 + (long)i * m + j];
+                        for (int k = 0; k < n; k++) {
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = s == 0 ? IAB(h2, ii, k) : IAB(h2, k, ii);
+                            int jk = s == 0 ? IAB(h2, jj, k) : IAB(h2, k, jj);
+                            acc -= u[p->d2aboff[h2] + (long)ik * GAB(h2) + jk];
+                        }
+                        A[r + (long)i * m + j] = acc;
+                    }
+                r += (long)m * m;
+            }
+        /* D2aa -> D1a, D2bb -> D1b */
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int m = p->amopi[h], o = p->pitzer_offset[h];
+                const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+                const long *d2s = s == 0 ? p->d2aaoff : p->d2bboff;
+                double pre = s == 0 ? na - 1.0 : nb - 1.0;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        int ii = i + o, jj = j + o;
+                        double acc = pre * u[d1 + (long)i * m + j];
+                        for (int k = 0; k < n; k++) {
+                            if (k == ii || k == jj) continue;
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = IAA(h2, ii, k), jk = IAA(h2, jj, k);
+                            acc -= SGN(ii, k) * SGN(jj, k) * u[d2s[h2] + (long)ik * GAA(h2) + jk];
+                        }
+                        A[r + (long)i * m + j] = acc;
+                    }
+                r += (long)m * m;
+            }
+    }
+
+    /* spin.cc:276-508 */
+    if (p->constrain_spin) {
+        double s2 = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                int h = SYM(i) ^ SYM(j);
+                s2 += u[p->d2aboff[h] + (long)IAB(h, i, j) * GAB(h) + IAB(h, j, i)];
+            }
+        A[r++] = s2;
+        if (p->nalpha == p->nbeta) {
+            for (int h = 0; h < H; h++) {
+                long mm = (long)p->amopi[h] * p->amopi[h];
+                for (long t = 0; t < mm; t++) A[r + t] = u[p->d1aoff[h] + t] - u[p->d1boff[h] + t];
+                r += mm;
+            }
+            for (int h = 0; h < H; h++) {
+                long aa = (long)GAA(h) * GAA(h);
+                for (long t = 0; t < aa; t++) A[r + t] = u[p->d2aaoff[h] + t] - u[p->d2bboff[h] + t];
+                r += aa;
+            }
+            for (int s = 0; s < 2; s++) /* D2aa / D2bb from D2ab */
+                for (int h = 0; h < H; h++) {
+                    const long d2s = s == 0 ? p->d2aaoff[h] : p->d2bboff[h];
+                    const long ab = p->d2aboff[h];
+                    int ga = GAA(h), gb = GAB(h);
+                    for (int ij = 0; ij < ga; ij++) {
+                        int i = BAA0(h, ij), j = BAA1(h, ij);
+                        long ijb = IAB(h, i, j), jib = IAB(h, j, i);
+                        for (int kl = 0; kl < ga; kl++) {
+                            int k = BAA0(h, kl), l = BAA1(h, kl);
+                            long klb = IAB(h, k, l), lkb = IAB(h, l, k);
+                            double v = u[d2s + (long)ij * ga + kl];
+                            v -= 0.5 * u[ab + ijb * gb + klb];
+                            v += 0.5 * u[ab + jib * gb + klb];
+                            v += 0.5 * u[ab + ijb * gb + lkb];
+                            v -= 0.5 * u[ab + jib * gb + lkb];
+                            A[r + (long)ij * ga + kl] = v;
+                        }
+                    }
+                    r += (long)ga * ga;
+                }
+            for (int h = 0; h < H; h++) { /* D200 */
+                int g = GAB(h);
+                const long ab = p->d2aboff[h];
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    long ji = IAB(h, j, i);
+                    double dij = i == j ? sqrt(2.0) : 1.0;
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long lk = IAB(h, l, k);
+                        double dkl = k == l ? sqrt(2.0) : 1.0;
+                        double f = 0.5 / (dij * dkl);
+                        double v = u[p->d200off[h] + (long)ij * g + kl];
+                        v -= f * u[ab + (long)ij * g + kl];
+                        v -= f * u[ab + ji * g + kl];
+                        v -= f * u[ab + (long)ij * g + lk];
+                        v -= f * u[ab + ji * g + lk];
+                        A[r + (long)ij * g + kl] = v;
+                    }
+                }
+                r += (long)g * g;
+            }
+            for (int h = 0; h < H; h++) { /* D2ab(ij,kl) = D2ab(ji,lk) */
+                int g = GAB(h);
+                const long ab = p->d2aboff[h];
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    long ji = IAB(h, j, i);
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long lk = IAB(h, l, k);
+                        A[r + (long)ij * g + kl] = u[ab + (long)ij * g + kl] - u[ab + ji * g + lk];
+                    }
+                }
+                r += (long)g * g;
+            }
+        } else {
+            for (int h = 0; h < H; h++) {
+                int g = GAB(h), g2 = 2 * GAB(h);
+                const long ab = p->d2aboff[h], d0 = p->d200off[h];
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    long ji = IAB(h, j, i);
+                    double dij = i == j ? sqrt(2.0) : 1.0;
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long lk = IAB(h, l, k);
+                        double dkl = k == l ? sqrt(2.0) : 1.0;
+                        double xijkl = u[ab + (long)ij * g + kl], xjikl = u[ab + ji * g + kl];
+                        double xijlk = u[ab + (long)ij * g + lk], xjilk = u[ab + ji * g + lk];
+                        long q00 = (long)ij * g2 + kl, q01 = (long)ij * g2 + kl + g;
+                        long q10 = (long)(ij + g) * g2 + kl, q11 = (long)(ij + g) * g2 + kl + g;
+                        A[r + q00] = u[d0 + q00] - 0.5 / (dij * dkl) * (xijkl + xjikl + xijlk + xjilk);
+                        A[r + q01] = u[d0 + q01] - 0.5 / dij * (xijkl - xijlk + xjikl - xjilk);
+                        A[r + q10] = u[d0 + q10] - 0.5 / dkl * (xijkl + xijlk - xjikl - xjilk);
+                        A[r + q11] = u[d0 + q11] - 0.5 * (xijkl - xjikl - xijlk + xjilk);
+                    }
+                }
+                r += (long)g2 * g2;
+            }
+        }
+        if (p->constrain_g2) {
+            int g = GAB(0);
+            for (int kl = 0; kl < g; kl++) {
+                double acc = 0.0;
+                for (int i = 0; i < n; i++) acc += u[p->g2baoff[0] + (long)kl * g + IAB(0, i, i)];
+                A[r + kl] = acc;
+            }
+            r += g;
+            for (int kl = 0; kl < g; kl++) {
+                double acc = 0.0;
+                for (int i = 0; i < n; i++) acc += u[p->g2baoff[0] + (long)IAB(0, i, i) * g + kl];
+                A[r + kl] = acc;
+            }
+            r += g;
+        }
+    }
+
+    /* q2.cc:170-317 */
+    if (p->constrain_q2) {
+        for (int h = 0; h < H; h++) {
+            int g = GAB(h);
+            for (int ij = 0; ij < g; ij++) {
+                int i = BAB0(h, ij), j = BAB1(h, ij);
+                for (int kl = 0; kl < g; kl++) {
+                    int k = BAB0(h, kl), l = BAB1(h, kl);
+                    long t = (long)ij * g + kl;
+                    double v = u[p->d2aboff[h] + t] - u[p->q2aboff[h] + t];
+                    if (j == l) v -= u[OB(p->d1aoff, k, i)];
+                    if (i == k) v -= u[OB(p->d1boff, l, j)];
+                    A[r + t] = v;
+                }
+            }
+            r += (long)g * g;
+        }
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int g = GAA(h);
+                const long d2 = s == 0 ? p->d2aaoff[h] : p->d2bboff[h];
+                const long q2 = s == 0 ? p->q2aaoff[h] : p->q2bboff[h];
+                const long *d1 = s == 0 ? p->d1aoff : p->d1boff;
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAA0(h, ij), j = BAA1(h, ij);
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAA0(h, kl), l = BAA1(h, kl);
+                        long t = (long)ij * g + kl;
+                        double v = u[d2 + t] - u[q2 + t];
+                        if (j == l) v -= u[OB(d1, k, i)];
+                        if (j == k) v += u[OB(d1, l, i)];
+                        if (i == l) v += u[OB(d1, k, j)];
+                        if (i == k) v -= u[OB(d1, l, j)];
+                        A[r + t] = v;
+                    }
+                }
+                r += (long)g * g;
+            }
+    }
+
+    /* g2.cc:245-441 */
+    if (p->constrain_g2) {
+        for (int s = 0; s < 2; s++) /* G2ab, G2ba */
+            for (int h = 0; h < H; h++) {
+                int g = GAB(h);
+                const long gof = s == 0 ? p->g2aboff[h] : p->g2baoff[h];
+                const long *d1 = s == 0 ? p->d1aoff : p->d1boff;
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long t = (long)ij * g + kl;
+                        double v = -u[gof + t];
+                        if (j == l) v += u[OB(d1, i, k)];
+                        int h2 = SYM(i) ^ SYM(l);
+                        long a, bcol;
+                        if (s == 0) { a = IAB(h2, i, l); bcol = IAB(h2, k, j); }
+                        else        { a = IAB(h2, l, i); bcol = IAB(h2, j, k); }
+                        v -= u[p->d2aboff[h2] + a * GAB(h2) + bcol];
+                        A[r + t] = v;
+                    }
+                }
+                r += (long)g * g;
+            }
+        for (int h = 0; h < H; h++) {
+            int g = GAB(h), G = 2 * GAB(h);
+            const long gof = p->g2aaoff[h];
+            for (int ij = 0; ij < g; ij++) {
+                int i = BAB0(h, ij), j = BAB1(h, ij);
+                for (int kl = 0; kl < g; kl++) {
+                    int k = BAB0(h, kl), l = BAB1(h, kl);
+                    int h2 = SYM(i) ^ SYM(l);
+                    for (int s = 0; s < 2; s++) { /* aaaa, bbbb */
+                        long t = (long)(ij + s * g) * G + (kl + s * g);
+                        double v = -u[gof + t];
+                        if (j == l) v += u[OB(s == 0 ? p->d1aoff : p->d1boff, i, k)];
+                        if (i != l && k != j) {
+                            const long d2 = s == 0 ? p->d2aaoff[h2] : p->d2bboff[h2];
+                            v -= SGN(i, l) * SGN(k, j) * u[d2 + (long)IAA(h2, i, l) * GAA(h2) + IAA(h2, k, j)];
+                        }
+                        A[r + t] = v;
+                    }
+                    long tab = (long)ij * G + (kl + g);      /* aabb */
+                    A[r + tab] = -u[gof + tab] + u[p->d2aboff[h2] + (long)IAB(h2, i, l) * GAB(h2) + IAB(h2, j, k)];
+                    long tba = (long)(ij + g) * G + kl;      /* bbaa */
+                    A[r + tba] = -u[gof + tba] + u[p->d2aboff[h2] + (long)IAB(h2, l, i) * GAB(h2) + IAB(h2, k, j)];
+                }
+            }
+            r += (long)G * G;
+        }
+    }
+}
+
+/* ---- A^T u -------------------------------------------------------------- */
+/* v2rdm_solver.cc:3052-3108; d2.cc:46-311, spin.cc:46-273, q2.cc:320-438, g2.cc:444-634 */
+void orc_ATu(const orc_problem *p, const double *u, double *A) {
+    const int n = NMO, H = p->nirrep;
+    const double na = p->nalpha, nb = p->nbeta;
+    memset(A, 0, sizeof(double) * p->n_primal);
+    long r = 0;
+
+    if (p->constrain_sz) {
+        double yab = u[r++], yaa = u[r++], ybb = u[r++];
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                int h = SYM(i) ^ SYM(j);
+                int ij = IAB(h, i, j);
+                A[p->d2aboff[h] + (long)ij * GAB(h) + ij] += yab;
+            }
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                if (i == j) continue;
+                int h = SYM(i) ^ SYM(j);
+                int ij = IAA(h, i, j);
+                A[p->d2aaoff[h] + (long)ij * GAA(h) + ij] += yaa;
+                A[p->d2bboff[h] + (long)ij * GAA(h) + ij] += ybb;
+            }
+    } else {
+        double y = u[r++];
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                int h = SYM(i) ^ SYM(j);
+                int ij = IAB(h, i, j);
+                A[p->d2aboff[h] + (long)ij * GAB(h) + ij] += 2.0 * y;
+                if (i == j) continue;
+                ij = IAA(h, i, j);
+                A[p->d2aaoff[h] + (long)ij * GAA(h) + ij] += y;
+                A[p->d2bboff[h] + (long)ij * GAA(h) + ij] += y;
+            }
+    }
+
+    for (int blk = 0; blk < 3; blk++)
+        for (int h = 0; h < H; h++) {
+            const long *off = blk == 0 ? p->d2aaoff : blk == 1 ? p->d2bboff : p->d2aboff;
+            int g = blk == 2 ? GAB(h) : GAA(h);
+            for (int ij = 0; ij < g; ij++)
+                for (int kl = 0; kl < g; kl++) {
+                    double y = u[r++];
+                    A[off[h] + (long)ij * g + kl] += y;
+                    A[off[h] + (long)kl * g + ij] -= y;
+                }
+        }
+
+    for (int s = 0; s < 2; s++)
+        for (int h = 0; h < H; h++) {
+            int m = p->amopi[h];
+            const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+            const long q1 = s == 0 ? p->q1aoff[h] : p->q1boff[h];
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++) {
+                    double y = u[r + (long)i * m + j];
+                    A[d1 + (long)j * m + i] += y;
+                    A[q1 + (long)i * m + j] += y;
+                }
+            r += (long)m * m;
+        }
+
+    if (!p->constrain_sz) {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int m = p->amopi[h], o = p->pitzer_offset[h];
+                const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+                const long *d2s = s == 0 ? p->d2aaoff : p->d2bboff;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        int ii = i + o, jj = j + o;
+                        double y = u[r + (long)i * m + j];
+                        A[d1 + (long)i * m + j] += (na + nb - 1.0) * y;
+                        for (int k = 0; k < n; k++) {
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = s == 0 ? IAB(h2, ii, k) : IAB(h2, k, ii);
+                            int jk = s == 0 ? IAB(h2, jj, k) : IAB(h2, k, jj);
+                            A[p->d2aboff[h2] + (long)ik * GAB(h2) + jk] -= y;
+                        }
+                        for (int k = 0; k < n; k++) {
+                            if (k == ii || k == jj) continue;
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = IAA(h2, ii, k), jk = IAA(h2, jj, k);
+                            A[d2s[h2] + (long)ik * GAA(h2) + jk] -= SGN(ii, k) * SGN(jj, k) * y;
+                        }
+                    }
+                r += (long)m * m;
+            }
+    } else {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int m = p->amopi[h], o = p->pitzer_offset[h];
+                const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+                double pre = s == 0 ? nb : na;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        int ii = i + o, jj = j + o;
+                        double y = u[r + (long)i * m + j];
+                        A[d1 + (long)i * m + j] += pre * y;
+                        for (int k = 0; k < n; k++) {
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = s == 0 ? IAB(h2, ii, k) : IAB(h2, k, ii);
+                            int jk = s == 0 ? IAB(h2, jj, k) : IAB(h2, k, jj);
+                            A[p->d2aboff[h2] + (long)ik * GAB(h2) + jk] -= y;
+                        }
+                    }
+                r += (long)m * m;
+            }
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int m = p->amopi[h], o = p->pitzer_offset[h];
+                const long d1 = s == 0 ? p->d1aoff[h] : p->d1boff[h];
+                const long *d2s = s == 0 ? p->d2aaoff : p->d2bboff;
+                double pre = s == 0 ? na - 1.0 : nb - 1.0;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        int ii = i + o, jj = j + o;
+                        double y = u[r + (long)i * m + j];
+                        A[d1 + (long)i * m + j] += pre * y;
+                        for (int k = 0; k < n; k++) {
+                            if (k == ii || k == jj) continue;
+                            int h2 = SYM(ii) ^ SYM(k);
+                            int ik = IAA(h2, ii, k), jk = IAA(h2, jj, k);
+                            A[d2s[h2] + (long)ik * GAA(h2) + jk] -= SGN(ii, k) * SGN(jj, k) * y;
+                        }
+                    }
+                r += (long)m * m;
+            }
+    }
+
+    if (p->constrain_spin) {
+        double y = u[r++];
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                int h = SYM(i) ^ SYM(j);
+                A[p->d2aboff[h] + (long)IAB(h, i, j) * GAB(h) + IAB(h, j, i)] += y;
+            }
+        if (p->nalpha == p->nbeta) {
+            for (int h = 0; h < H; h++) {
+                long mm = (long)p->amopi[h] * p->amopi[h];
+                for (long t = 0; t < mm; t++) {
+                    A[p->d1aoff[h] + t] += u[r + t];
+                    A[p->d1boff[h] + t] -= u[r + t];
+                }
+                r += mm;
+            }
+            for (int h = 0; h < H; h++) {
+                long aa = (long)GAA(h) * GAA(h);
+                for (long t = 0; t < aa; t++) {
+                    A[p->d2aaoff[h] + t] += u[r + t];
+                    A[p->d2bboff[h] + t] -= u[r + t];
+                }
+                r += aa;
+            }
+            for (int s = 0; s < 2; s++)
+                for (int h = 0; h < H; h++) {
+                    const long d2s = s == 0 ? p->d2aaoff[h] : p->d2bboff[h];
+                    const long ab = p->d2aboff[h];
+                    int ga = GAA(h), gb = GAB(h);
+                    for (int ij = 0; ij < ga; ij++) {
+                        int i = BAA0(h, ij), j = BAA1(h, ij);
+                        long ijb = IAB(h, i, j), jib = IAB(h, j, i);
+                        for (int kl = 0; kl < ga; kl++) {
+                            int k = BAA0(h, kl), l = BAA1(h, kl);
+                            long klb = IAB(h, k, l), lkb = IAB(h, l, k);
+                            double y = u[r + (long)ij * ga + kl];
+                            A[d2s + (long)ij * ga + kl] += y;
+                            A[ab + ijb * gb + klb] -= 0.5 * y;
+                            A[ab + jib * gb + klb] += 0.5 * y;
+                            A[ab + ijb * gb + lkb] += 0.5 * y;
+                            A[ab + jib * gb + lkb] -= 0.5 * y;
+                        }
+                    }
+                    r += (long)ga * ga;
+                }
+            for (int h = 0; h < H; h++) {
+                int g = GAB(h);
+                const long ab = p->d2aboff[h];
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    long ji = IAB(h, j, i);
+                    double dij = i == j ? sqrt(2.0) : 1.0;
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long lk = IAB(h, l, k);
+                        double dkl = k == l ? sqrt(2.0) : 1.0;
+                        double y = u[r + (long)ij * g + kl];
+                        double f = 0.5 / (dij * dkl) * y;
+                        A[p->d200off[h] + (long)ij * g + kl] += y;
+                        A[ab + (long)ij * g + kl] -= f;
+                        A[ab + ji * g + kl] -= f;
+                        A[ab + (long)ij * g + lk] -= f;
+                        A[ab + ji * g + lk] -= f;
+                    }
+                }
+                r += (long)g * g;
+            }
+            for (int h = 0; h < H; h++) {
+                int g = GAB(h);
+                const long ab = p->d2aboff[h];
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    long ji = IAB(h, j, i);
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long lk = IAB(h, l, k);
+                        double y = u[r + (long)ij * g + kl];
+                        A[ab + (long)ij * g + kl] += y;
+                        A[ab + ji * g + lk] -= y;
+                    }
+                }
+                r += (long)g * g;
+            }
+        } else {
+            for (int h = 0; h < H; h++) {
+                int g = GAB(h), g2 = 2 * GAB(h);
+                const long ab = p->d2aboff[h], d0 = p->d200off[h];
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    long ji = IAB(h, j, i);
+                    double dij = i == j ? sqrt(2.0) : 1.0;
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long lk = IAB(h, l, k);
+                        double dkl = k == l ? sqrt(2.0) : 1.0;
+                        long q00 = (long)ij * g2 + kl, q01 = (long)ij * g2 + kl + g;
+                        long q10 = (long)(ij + g) * g2 + kl, q11 = (long)(ij + g) * g2 + kl + g;
+                        double y00 = u[r + q00], y01 = u[r + q01], y10 = u[r + q10], y11 = u[r + q11];
+                        A[d0 + q00] += y00; A[d0 + q01] += y01; A[d0 + q10] += y10; A[d0 + q11] += y11;
+                        double f00 = 0.5 / (dij * dkl) * y00, f01 = 0.5 / dij * y01;
+                        double f10 = 0.5 / dkl * y10, f11 = 0.5 * y11;
+                        A[ab + (long)ij * g + kl] += -f00 - f01 - f10 - f11;
+                        A[ab + ji * g + kl]       += -f00 - f01 + f10 + f11;
+                        A[ab + (long)ij * g + lk] += -f00 + f01 - f10 + f11;
+                        A[ab + ji * g + lk]       += -f00 + f01 + f10 - f11;
+                    }
+                }
+                r += (long)g2 * g2;
+            }
+        }
+        if (p->constrain_g2) {
+            int g = GAB(0);
+            for (int kl = 0; kl < g; kl++) {
+                double y = u[r + kl];
+                for (int i = 0; i < n; i++) A[p->g2baoff[0] + (long)kl * g + IAB(0, i, i)] += y;
+            }
+            r += g;
+            for (int kl = 0; kl < g; kl++) {
+                double y = u[r + kl];
+                for (int i = 0; i < n; i++) A[p->g2baoff[0] + (long)IAB(0, i, i) * g + kl] += y;
+            }
+            r += g;
+        }
+    }
+
+    if (p->constrain_q2) {
+        for (int h = 0; h < H; h++) {
+            int g = GAB(h);
+            for (int ij = 0; ij < g; ij++) {
+                int i = BAB0(h, ij), j = BAB1(h, ij);
+                for (int kl = 0; kl < g; kl++) {
+                    int k = BAB0(h, kl), l = BAB1(h, kl);
+                    long t = (long)ij * g + kl;
+                    double y = u[r + t];
+                    A[p->d2aboff[h] + t] += y;
+                    A[p->q2aboff[h] + t] -= y;
+                    if (j == l) A[OB(p->d1aoff, k, i)] -= y;
+                    if (i == k) A[OB(p->d1boff, l, j)] -= y;
+                }
+            }
+            r += (long)g * g;
+        }
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int g = GAA(h);
+                const long d2 = s == 0 ? p->d2aaoff[h] : p->d2bboff[h];
+                const long q2 = s == 0 ? p->q2aaoff[h] : p->q2bboff[h];
+                const long *d1 = s == 0 ? p->d1aoff : p->d1boff;
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAA0(h, ij), j = BAA1(h, ij);
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAA0(h, kl), l = BAA1(h, kl);
+                        long t = (long)ij * g + kl;
+                        double y = u[r + t];
+                        A[d2 + t] += y;
+                        A[q2 + t] -= y;
+                        if (j == l) A[OB(d1, k, i)] -= y;
+                        if (j == k) A[OB(d1, l, i)] += y;
+                        if (i == l) A[OB(d1, k, j)] += y;
+                        if (i == k) A[OB(d1, l, j)] -= y;
+                    }
+                }
+                r += (long)g * g;
+            }
+    }
+
+    if (p->constrain_g2) {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int g = GAB(h);
+                const long gof = s == 0 ? p->g2aboff[h] : p->g2baoff[h];
+                const long *d1 = s == 0 ? p->d1aoff : p->d1boff;
+                for (int ij = 0; ij < g; ij++) {
+                    int i = BAB0(h, ij), j = BAB1(h, ij);
+                    for (int kl = 0; kl < g; kl++) {
+                        int k = BAB0(h, kl), l = BAB1(h, kl);
+                        long t = (long)ij * g + kl;
+                        double y = u[r + t];
+                        A[gof + t] -= y;
+                        if (j == l) A[OB(d1, i, k)] += y;
+                        int h2 = SYM(i) ^ SYM(l);
+                        long a, bcol;
+                        if (s == 0) { a = IAB(h2, i, l); bcol = IAB(h2, k, j); }
+                        else        { a = IAB(h2, l, i); bcol = IAB(h2, j, k); }
+                        A[p->d2aboff[h2] + a * GAB(h2) + bcol] -= y;
+                    }
+                }
+                r += (long)g * g;
+            }
+        for (int h = 0; h < H; h++) {
+            int g = GAB(h), G = 2 * GAB(h);
+            const long gof = p->g2aaoff[h];
+            for (int ij = 0; ij < g; ij++) {
+                int i = BAB0(h, ij), j = BAB1(h, ij);
+                for (int kl = 0; kl < g; kl++) {
+                    int k = BAB0(h, kl), l = BAB1(h, kl);
+                    int h2 = SYM(i) ^ SYM(l);
+                    for (int s = 0; s < 2; s++) {
+                        long t = (long)(ij + s * g) * G + (kl + s * g);
+                        double y = u[r + t];
+                        A[gof + t] -= y;
+                        if (j == l) A[OB(s == 0 ? p->d1aoff : p->d1boff, i, k)] += y;
+                        if (i != l && k != j) {
+                            const long d2 = s == 0 ? p->d2aaoff[h2] : p->d2bboff[h2];
+                            A[d2 + (long)IAA(h2, i, l) * GAA(h2) + IAA(h2, k, j)] -= SGN(i, l) * SGN(k, j) * y;
+                        }
+                    }
+                    long tab = (long)ij * G + (kl + g);
+                    A[gof + tab] -= u[r + tab];
+                    A[p->d2aboff[h2] + (long)IAB(h2, i, l) * GAB(h2) + IAB(h2, j, k)] += u[r + tab];
+                    long tba = (long)(ij + g) * G + kl;
+                    A[gof + tba] -= u[r + tba];
+                    A[p->d2aboff[h2] + (long)IAB(h2, l, i) * GAB(h2) + IAB(h2, k, j)] += u[r + tba];
+                }
+            }
+            r += (long)G * G;
+        }
+    }
+}
+
+void orc_cb_Au(double *out, const double *in, void *data) { orc_Au((const orc_problem *)data, in, out); }
+void orc_cb_ATu(double *out, const double *in, void *data) { orc_ATu((const orc_problem *)data, in, out); }
+
+/* ---- guess / post-processing --------------------------------------------- */
+
+/* v2rdm_solver.cc:1799-1806 */
+void orc_guess_random(long n, double *x) {
+    srand(0);
+    for (long i = 0; i < n; i++) x[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
+}
+
+/* v2rdm_solver.cc:1370-1381: returns sum_ij D2ab(ij,ji); S^2 = N/2 + ms^2 - that */
+double orc_spin_squared(const orc_problem *p, const double *x) {
+    int n = NMO;
+    double s = 0.0;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            int h = SYM(i) ^ SYM(j);
+            s += x[p->d2aboff[h] + (long)IAB(h, i, j) * GAB(h) + IAB(h, j, i)];
+        }
+    double ms = (p->multiplicity - 1.0) / 2.0;
+    return 0.5 * (p->nalpha + p->nbeta) + ms * ms - s;
+}
+
+/* python_api.cc:374-394, as a dense n x n matrix in Pitzer order */
+void orc_opdm(const orc_problem *p, const double *x, double *d) {
+    int n = NMO;
+    memset(d, 0, sizeof(double) * n * n);
+    for (int h = 0; h < p->nirrep; h++) {
+        int m = p->amopi[h], o = p->pitzer_offset[h];
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++)
+                d[(long)(i + o) * n + (j + o)] = x[p->d1aoff[h] + (long)i * m + j] + x[p->d1boff[h] + (long)i * m + j];
+    }
+}
+
+/* python_api.cc:396-456: tpdm[(i*n+k)*n*n + j*n+l] = D2ab(ij,kl)+D2ab(ji,lk)+s(D2aa+D2bb)(ij,kl) */
+int orc_tpdm(const orc_problem *p, const double *x, double *t) {
+    if (p->nirrep > 1) return -1;
+    int n = NMO;
+    long g = GAB(0), ga = GAA(0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++)
+                for (int l = 0; l < n; l++) {
+                    double v = x[p->d2aboff[0] + IAB(0, i, j) * g + IAB(0, k, l)]
+                             + x[p->d2aboff[0] + IAB(0, j, i) * g + IAB(0, l, k)];
+                    if (i != j && k != l) {
+                        double sg = ((i > j) ? -1.0 : 1.0) * ((k > l) ? -1.0 : 1.0);
+                        long e = IAA(0, i, j) * ga + IAA(0, k, l);
+                        v += sg * (x[p->d2aaoff[0] + e] + x[p->d2bboff[0] + e]);
+                    }
+                    t[((long)i * n + k) * n * n + (long)j * n + l] = v;
+                }
+    return 0;
+}
+
+/* ---- projection ---------------------------------------------------------- */
+
+/* Cyclic Jacobi on a symmetric d x d matrix (row-major, overwritten).
+ * Eigenvalues in w, eigenvectors in the columns of v. */
+void orc_jacobi_eig(int d, double *a, double *w, double *v) {
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) v[(long)i * d + j] = (i == j);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < d; i++)
+            for (int j = 0; j < d; j++) {
+                double q = a[(long)i * d + j] * a[(long)i * d + j];
+                tot += q;
+                if (i != j) off += q;
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int pp = 0; pp < d - 1; pp++)
+            for (int q = pp + 1; q < d; q++) {
+                double apq = a[(long)pp * d + q];
+                if (apq == 0.0) continue;
+                double app = a[(long)pp * d + pp], aqq = a[(long)q * d + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < d; k++) {
+                    double akp = a[(long)k * d + pp], akq = a[(long)k * d + q];
+                    a[(long)k * d + pp] = c * akp - s * akq;
+                    a[(long)k * d + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < d; k++) {
+                    double apk = a[(long)pp * d + k], aqk = a[(long)q * d + k];
+                    a[(long)pp * d + k] = c * apk - s * aqk;
+                    a[(long)q * d + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < d; k++) {
+                    double vkp = v[(long)k * d + pp], vkq = v[(long)k * d + q];
+                    v[(long)k * d + pp] = c * vkp - s * vkq;
+                    v[(long)k * d + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < d; i++) w[i] = a[(long)i * d + i];
+}
+
+/* U (d x d) -> x = V max(L,0) V^T / mu, z = -V min(L,0) V^T, after
+ * symmetrising U.  work holds at least 3 d*d doubles (+ LAPACK workspace). */
+int orc_project_block(int d, const double *U, double mu, double *x, double *z,
+                      orc_dsyevd_fn eig, double *work, int lwork, int *iwork, int liwork) {
+    if (d == 0) return 0;
+    long dd = (long)d * d;
+    double *a = work, *w = work + dd, *v = work + dd + d, *lw = work + 2 * dd + d;
+    for (int i = 0; i < d; i++)
+        for (int j = 0; j < d; j++) a[(long)i * d + j] = 0.5 * (U[(long)i * d + j] + U[(long)j * d + i]);
+    if (eig) {
+        int info = 0, n = d, lda = d;
+        int lw_avail = lwork - (int)(2 * dd + d);
+        eig("V", "U", &n, a, &lda, w, lw, &lw_avail, iwork, &liwork, &info, 1, 1);
+        if (info != 0) return info;
+        /* column-major result: eigenvector k is a[k*d + i] */
+        for (int i = 0; i < d; i++)
+            for (int k = 0; k < d; k++) v[(long)i * d + k] = a[(long)k * d + i];
+    } else {
+        orc_jacobi_eig(d, a, w, v);
+    }
+    for (int i = 0; i < d; i++)
+        for (int j = i; j < d; j++) {
+            double sp = 0.0, sn = 0.0;
+            for (int k = 0; k < d; k++) {
+                double t = v[(long)i * d + k] * v[(long)j * d + k] * w[k];
+                if (w[k] > 0.0) sp += t; else sn += t;
+            }
+            x[(long)i * d + j] = x[(long)j * d + i] = sp / mu;
+            z[(long)i * d + j] = z[(long)j * d + i] = -sn;
+        }
+    return 0;
+}
+
+/* ---- BPSDP ----------------------------------------------------------------- */
+
+void orc_sdp_state_init(orc_sdp_state *s, long np, long nd) {
+    memset(s, 0, sizeof(*s));
+    s->n_primal = np;
+    s->n_dual = nd;
+    s->y = (double *)calloc(nd, sizeof(double));
+    s->z = (double *)calloc(np, sizeof(double));
+}
+
+void orc_sdp_state_free(orc_sdp_state *s) {
+    free(s->y);
+    free(s->z);
+    memset(s, 0, sizeof(*s));
+}
+
+static double dot(long n, const double *a, const double *b) {
+    double s = 0.0;
+    for (long i = 0; i < n; i++) s += a[i] * b[i];
+    return s;
+}
+
+/* Relative CG residual tolerance for one outer iteration: tighten with the
+ * current primal/dual errors, floored by CG_CONVERGENCE. */
+double orc_cg_tolerance(const orc_sdp_options *o, long oiter, double eps_p, double eps_d) {
+    double e = eps_p > eps_d ? eps_p : eps_d;
+    if (oiter == 0) e = 1.0;
+    double tol = 0.01 * e;
+    if (tol > 1e-2) tol = 1e-2;
+    if (tol < o->cg_convergence) tol = o->cg_convergence;
+    return tol;
+}
+
+/*
+ * Boundary-point method (Povh, Rendl, Wiegele, Computing 78, 277 (2006);
+ * Malick, Povh, Rendl, Wiegele, SIAM J. Optim. 20, 336 (2009); Mazziotti,
+ * PRL 106, 083001 (2011)), the iteration libsdp's BPSDPSolver::solve runs for
+ * v2rdm_solver.cc:1340.  Per outer iteration:
+ *   1. solve (A A^T) y = A(c - z) + mu (b - A x) by CG, warm-started at y;
+ *   2. U = mu x + A^T y - c; per block U = V L V^T (symmetrised);
+ *      x <- V L+ V^T / mu,  z <- -V L- V^T;
+ *   3. eps_p = ||A x - b|| / ||b||, eps_d = ||A^T y - c + z|| / ||c||,
+ *      E_p = c.x, E_d = b.y;
+ *   4. every MU_UPDATE_FREQUENCY outer iterations mu <- mu * eps_p / eps_d;
+ *   5. converged when |E_p - E_d| < E_CONVERGENCE and eps_p, eps_d < R_CONVERGENCE.
+ */
+int orc_bpsdp_solve(orc_sdp_state *s, const orc_sdp_options *o,
+                    double *x, const double *b, const double *c,
+                    const int *dims, int nblocks, int maxiter,
+                    orc_map_fn Au, orc_map_fn ATu, void *data,
+                    orc_dsyevd_fn eig, orc_monitor_fn mon, int print_level, void *mon_data) {
+    long np = s->n_primal, nd = s->n_dual;
+    if (!s->initialized) {
+        memset(s->y, 0, sizeof(double) * nd);
+        memset(s->z, 0, sizeof(double) * np);
+        s->mu = o->mu0 > 0 ? o->mu0 : 1.0;
+        s->eps_p = s->eps_d = 1.0;
+        s->initialized = 1;
+    }
+    double *Ax = (double *)malloc(sizeof(double) * nd);
+    double *rhs = (double *)malloc(sizeof(double) * nd);
+    double *r = (double *)malloc(sizeof(double) * nd);
+    double *pv = (double *)malloc(sizeof(double) * nd);
+    double *w = (double *)malloc(sizeof(double) * nd);
+    double *tp = (double *)malloc(sizeof(double) * np);
+    double *ATy = (double *)malloc(sizeof(double) * np);
+    double *U = (double *)malloc(sizeof(double) * np);
+
+    double bn = sqrt(dot(nd, b, b)), cn = sqrt(dot(np, c, c));
+    if (bn == 0.0) bn = 1.0;
+    if (cn == 0.0) cn = 1.0;
+
+    long boff[14 * ORC_MAX_IRREP + 1];
+    int maxd = 0;
+    boff[0] = 0;
+    for (int k = 0; k < nblocks; k++) {
+        boff[k + 1] = boff[k] + (long)dims[k] * dims[k];
+        if (dims[k] > maxd) maxd = dims[k];
+    }
+    int lwork = 2 * maxd * maxd + maxd + 1 + 6 * maxd + 2 * maxd * maxd + 64;
+    int liwork = 3 + 5 * maxd + 16;
+
+    s->converged = 0;
+    Au(Ax, x, data);
+    for (int it = 0; it < maxiter; it++) {
+        if (o->maxiter > 0 && s->oiter_total >= o->maxiter) break;
+        double t0 = wall();
+        /* rhs = A(c - z) + mu (b - Ax) */
+        for (long i = 0; i < np; i++) tp[i] = c[i] - s->z[i];
+        Au(rhs, tp, data);
+        for (long i = 0; i < nd; i++) rhs[i] += s->mu * (b[i] - Ax[i]);
+        /* r = rhs - A A^T y */
+        ATu(tp, s->y, data);
+        Au(w, tp, data);
+        for (long i = 0; i < nd; i++) { r[i] = rhs[i] - w[i]; pv[i] = r[i]; }
+        double rr = dot(nd, r, r);
+        double tol = orc_cg_tolerance(o, s->oiter_total, s->eps_p, s->eps_d);
+        double thr = tol * tol * dot(nd, rhs, rhs);
+        int k = 0;
+        while (k < o->cg_maxiter && rr > thr) {
+            ATu(tp, pv, data);
+            Au(w, tp, data);
+            double pw = dot(nd, pv, w);
+            if (pw <= 0.0) break;
+            double alpha = rr / pw;
+            for (long i = 0; i < nd; i++) { s->y[i] += alpha * pv[i]; r[i] -= alpha * w[i]; }
+            double rr1 = dot(nd, r, r);
+            double beta = rr1 / rr;
+            rr = rr1;
+            for (long i = 0; i < nd; i++) pv[i] = r[i] + beta * pv[i];
+            k++;
+        }
+        s->iiter_total += k;
+        double t1 = wall();
+        s->cg_time += t1 - t0;
+
+        /* U = mu x + A^T y - c, project each block */
+        ATu(ATy, s->y, data);
+        for (long i = 0; i < np; i++) U[i] = s->mu * x[i] + ATy[i] - c[i];
+        int fail = 0;
+#pragma omp parallel num_threads(o->nthreads > 0 ? o->nthreads : 1) reduction(+ : fail)
+        {
+            double *work = (double *)malloc(sizeof(double) * (lwork + 3L * maxd * maxd));
+            int *iwork = (int *)malloc(sizeof(int) * liwork);
+#pragma omp for schedule(dynamic, 1)
+            for (int kb = 0; kb < nblocks; kb++) {
+                if (orc_project_block(dims[kb], U + boff[kb], s->mu, x + boff[kb], s->z + boff[kb],
+                                      eig, work, lwork + 3 * maxd * maxd, iwork, liwork) != 0)
+                    fail += 1;
+            }
+            free(work);
+            free(iwork);
+        }
+        double t2 = wall();
+        s->proj_time += t2 - t1;
+        if (fail) break;
+
+        /* errors and energies */
+        double ed = 0.0;
+        for (long i = 0; i < np; i++) { double t = ATy[i] - c[i] + s->z[i]; ed += t * t; }
+        Au(Ax, x, data);
+        double ep = 0.0;
+        for (long i = 0; i < nd; i++) { double t = Ax[i] - b[i]; ep += t * t; }
+        s->eps_p = sqrt(ep) / bn;
+        s->eps_d = sqrt(ed) / cn;
+        s->energy_primal = dot(np, c, x);
+        s->energy_dual = dot(nd, b, s->y);
+        s->oiter_total++;
+        if (o->mu_update_frequency > 0 && s->oiter_total % o->mu_update_frequency == 0 &&
+            s->eps_d > 0.0 && s->eps_p > 0.0)
+            s->mu *= s->eps_p / s->eps_d;
+        s->cg_time += wall() - t2;
+        if (mon) mon(print_level, (int)s->oiter_total, k, s->energy_primal, s->energy_dual, s->mu,
+                     s->eps_p, s->eps_d, mon_data);
+        if (fabs(s->energy_primal - s->energy_dual) < o->e_convergence &&
+            s->eps_p < o->r_convergence && s->eps_d < o->r_convergence) {
+            s->converged = 1;
+            break;
+        }
+    }
+    free(Ax); free(rhs); free(r); free(pv); free(w); free(tp); free(ATy); free(U);
+    return s->converged;
+}
+
+/* ---- heap constructors for ctypes callers -------------------------------- */
+orc_problem *orc_problem_new(int nmo, int nalpha, int nbeta, int multiplicity,
+                             const int *irreps, int nirrep, int q2, int g2, int spin, int sz) {
+    orc_problem *p = (orc_problem *)calloc(1, sizeof(orc_problem));
+    if (orc_problem_init(p, nmo, nalpha, nbeta, multiplicity, irreps, nirrep, q2, g2, spin, sz) != 0) {
+        free(p);
+        return NULL;
+    }
+    return p;
+}
+void orc_problem_delete(orc_problem *p) {
+    if (!p) return;
+    orc_problem_free(p);
+    free(p);
+}
+orc_sdp_state *orc_sdp_state_new(long np, long nd) {
+    orc_sdp_state *s = (orc_sdp_state *)calloc(1, sizeof(orc_sdp_state));
+    orc_sdp_state_init(s, np, nd);
+    return s;
+}
+void orc_sdp_state_delete(orc_sdp_state *s) {
+    if (!s) return;
+    orc_sdp_state_free(s);
+    free(s);
+}
+/* scalars of a state: mu, E_p, E_d, eps_p, eps_d, cg_time, proj_time */
+void orc_sdp_state_scalars(const orc_sdp_state *s, double *out, long *counts) {
+    out[0] = s->mu; out[1] = s->energy_primal; out[2] = s->energy_dual;
+    out[3] = s->eps_p; out[4] = s->eps_d; out[5] = s->cg_time; out[6] = s->proj_time;
+    counts[0] = s->oiter_total; counts[1] = s->iiter_total; counts[2] = s->converged;
+}
+double *orc_sdp_state_y(orc_sdp_state *s) { return s->y; }
+double *orc_sdp_state_z(orc_sdp_state *s) { return s->z; }
+/* Solve with the oracle's own DQG maps. */
+int orc_solve_dqg(orc_sdp_state *s, const orc_sdp_options *o, const orc_problem *p,
+                  double *x, const double *b, const double *c, int maxiter,
+                  orc_dsyevd_fn eig, orc_monitor_fn mon, int print_level) {
+    return orc_bpsdp_solve(s, o, x, b, c, p->dims, p->nblocks, maxiter, orc_cb_Au, orc_cb_ATu,
+                           (void *)p, eig, mon, print_level, NULL);
+}
