@@ -1,0 +1,146 @@
+"""ctypes bindings of libv2rdm_hip.so (include/v2rdm_hip.h).
+
+The library is built in-tree by hilbert_amd.build; importing this module does
+not build it.  There is no CPU fallback: a missing library is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libv2rdm_hip.so")
+
+V2H_OK = 0
+ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "internal error"}
+
+
+class V2hError(RuntimeError):
+    """A non-zero status from the native library."""
+
+
+class SdpOptions(C.Structure):
+    _fields_ = [("sdp_objective_convergence", C.c_double), ("sdp_error_convergence", C.c_double),
+                ("cg_convergence", C.c_double), ("cg_maxiter", C.c_int), ("maxiter", C.c_int),
+                ("mu_update_frequency", C.c_int), ("mu0", C.c_double)]
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [("nmo", C.c_int), ("nalpha", C.c_int), ("nbeta", C.c_int), ("multiplicity", C.c_int),
+                ("nirrep", C.c_int), ("orbital_irreps", C.POINTER(C.c_int)),
+                ("constrain_q2", C.c_int), ("constrain_g2", C.c_int), ("constrain_spin", C.c_int),
+                ("constrain_sz", C.c_int)]
+
+
+class Status(C.Structure):
+    _fields_ = [("mu", C.c_double), ("energy_primal", C.c_double), ("energy_dual", C.c_double),
+                ("primal_error", C.c_double), ("dual_error", C.c_double),
+                ("oiter_total", C.c_int64), ("iiter_total", C.c_int64), ("converged", C.c_int),
+                ("active", C.c_int), ("status_bits", C.c_int), ("last_cg_iterations", C.c_int)]
+
+
+MONITOR = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                      C.c_double, C.c_double, C.c_void_p)
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/v2rdm_hip.h
+SIGNATURES = {
+    "v2h_last_error": (C.c_char_p, []),
+    "v2h_default_options": (C.c_int, [C.POINTER(SdpOptions)]),
+    "v2h_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "v2h_batch_create": (C.c_int, [C.POINTER(ProblemDesc), C.c_int, C.POINTER(SdpOptions), C.c_int,
+                                   C.POINTER(_vp)]),
+    "v2h_batch_destroy": (C.c_int, [_vp]),
+    "v2h_batch_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "v2h_batch_set_options": (C.c_int, [_vp, C.POINTER(SdpOptions)]),
+    "v2h_batch_sizes": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int)]),
+    "v2h_batch_block_dims": (C.c_int, [_vp, C.c_int, _ip]),
+    "v2h_batch_orbital_order": (C.c_int, [_vp, C.c_int, _ip]),
+    "v2h_batch_set_integrals": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "v2h_batch_set_cost": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_set_guess": (C.c_int, [_vp, C.c_int, _vp]),
+    "v2h_batch_reset": (C.c_int, [_vp, C.c_int]),
+    "v2h_batch_solve": (C.c_int, [_vp, C.c_int, _vp, _vp, C.POINTER(C.c_int)]),
+    "v2h_batch_status": (C.c_int, [_vp, C.c_int, C.POINTER(Status)]),
+    "v2h_batch_last_solve_seconds": (C.c_double, [_vp]),
+    "v2h_batch_get_x": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_get_y": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_get_z": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_set_y": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_set_z": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_get_mu": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
+    "v2h_batch_set_mu": (C.c_int, [_vp, C.c_int, C.c_double]),
+    "v2h_batch_get_c": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_get_b": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_Au": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "v2h_batch_ATu": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "v2h_batch_cost_adjoint": (C.c_int, [_vp, C.c_int, _vp, _dp, _dp]),
+    "v2h_structure_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(_vp)]),
+    "v2h_structure_destroy": (C.c_int, [_vp]),
+    "v2h_structure_sizes": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "v2h_structure_block_dims": (C.c_int, [_vp, _ip]),
+    "v2h_structure_tables": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
+    "v2h_structure_constraints": (C.c_int, [_vp, _lp, _ip, _dp, _dp]),
+    "v2h_structure_cost_map": (C.c_int, [_vp, _ip, _ip]),
+    "v2h_structure_gather_stats": (C.c_int, [_vp, _lp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is missing: run `python -m hilbert_amd.build` "
+                              "(the solver has no CPU fallback)")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != V2H_OK:
+        msg = lib().v2h_last_error().decode(errors="replace")
+        raise V2hError(f"{ERRORS.get(rc, rc)}: {msg}")
+
+
+def make_desc(nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=0, q2=True, g2=True, spin=True,
+              sz=True):
+    """ProblemDesc plus the array that keeps its irrep pointer alive."""
+    arr = None
+    ptr = None
+    if irreps is not None:
+        arr = (C.c_int * nmo)(*[int(v) for v in irreps])
+        ptr = C.cast(arr, C.POINTER(C.c_int))
+        if nirrep is None:
+            nirrep = 1
+            while nirrep <= max(int(v) for v in irreps):
+                nirrep *= 2
+    d = ProblemDesc(int(nmo), int(nalpha), int(nbeta), int(multiplicity), int(nirrep or 1), ptr,
+                    int(q2), int(g2), int(spin), int(sz))
+    return d, arr
+
+
+def default_options() -> SdpOptions:
+    o = SdpOptions()
+    check(lib().v2h_default_options(C.byref(o)))
+    return o
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    rc = lib().v2h_device_count(C.byref(c))
+    return c.value if rc == V2H_OK else 0

@@ -1,0 +1,183 @@
+"""A batch of independent DQG v2RDM SDPs solved together on one MI355X
+(v2h_batch_* in include/v2rdm_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+
+@dataclass
+class Problem:
+    """One fixed-orbital DQG problem; irreps are per orbital, caller order."""
+    nmo: int
+    nalpha: int
+    nbeta: int
+    irreps: object = None
+    nirrep: int = None
+    multiplicity: int = 0
+    q2: bool = True
+    g2: bool = True
+    spin: bool = True
+    sz: bool = True
+
+
+def options(e_convergence=1e-6, r_convergence=1e-5, cg_convergence=1e-9, cg_maxiter=10000, maxiter=50,
+            mu_update_frequency=1000, mu0=1.0) -> N.SdpOptions:
+    return N.SdpOptions(e_convergence, r_convergence, cg_convergence, cg_maxiter, maxiter,
+                        mu_update_frequency, mu0)
+
+
+class Batch:
+    def __init__(self, problems, opts: N.SdpOptions = None, device: int = 0):
+        problems = list(problems)
+        descs = (N.ProblemDesc * len(problems))()
+        keep = []
+        for k, p in enumerate(problems):
+            d, arr = N.make_desc(p.nmo, p.nalpha, p.nbeta, p.irreps, p.nirrep, p.multiplicity,
+                                 p.q2, p.g2, p.spin, p.sz)
+            descs[k] = d
+            keep.append(arr)
+        self.problems = problems
+        self.opts = opts or options()
+        h = C.c_void_p()
+        N.check(N.lib().v2h_batch_create(descs, len(problems), C.byref(self.opts), int(device), C.byref(h)))
+        self._h = h
+        self.count = len(problems)
+        self.n_primal, self.n_dual, self.n_blocks = [], [], []
+        for i in range(self.count):
+            a, b, c = C.c_int64(), C.c_int64(), C.c_int()
+            N.check(N.lib().v2h_batch_sizes(self._h, i, C.byref(a), C.byref(b), C.byref(c)))
+            self.n_primal.append(a.value)
+            self.n_dual.append(b.value)
+            self.n_blocks.append(c.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().v2h_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---- setup ---------------------------------------------------------------
+    def set_options(self, opts: N.SdpOptions):
+        self.opts = opts
+        N.check(N.lib().v2h_batch_set_options(self._h, C.byref(opts)))
+
+    def set_integrals(self, i, h, g):
+        n = self.problems[i].nmo
+        h = np.ascontiguousarray(h, dtype=np.float64).reshape(n * n)
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(n ** 4)
+        N.check(N.lib().v2h_batch_set_integrals(self._h, i, h, g))
+
+    def set_cost(self, i, c):
+        N.check(N.lib().v2h_batch_set_cost(self._h, i, np.ascontiguousarray(c, dtype=np.float64)))
+
+    def set_guess(self, i, x=None):
+        if x is None:
+            N.check(N.lib().v2h_batch_set_guess(self._h, i, None))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            N.check(N.lib().v2h_batch_set_guess(self._h, i, x.ctypes.data_as(C.c_void_p)))
+
+    def reset(self, i):
+        N.check(N.lib().v2h_batch_reset(self._h, i))
+
+    def block_dims(self, i):
+        d = np.zeros(self.n_blocks[i], dtype=np.int32)
+        N.check(N.lib().v2h_batch_block_dims(self._h, i, d))
+        return d
+
+    def orbital_order(self, i):
+        o = np.zeros(self.problems[i].nmo, dtype=np.int32)
+        N.check(N.lib().v2h_batch_orbital_order(self._h, i, o))
+        return o
+
+    # ---- solve ---------------------------------------------------------------
+    def solve(self, maxiter, monitor=None) -> int:
+        cb = N.MONITOR(lambda *a: monitor(*a[:-1])) if monitor else None
+        nc = C.c_int(0)
+        N.check(N.lib().v2h_batch_solve(self._h, int(maxiter), C.cast(cb, C.c_void_p) if cb else None, None,
+                                        C.byref(nc)))
+        return nc.value
+
+    def solve_to_convergence(self, chunk=None, monitor=None) -> int:
+        """Run until every instance converged or hit MAXITER (the
+        do { solve } while (!converged) loop of v2rdm_solver.cc:1330-1361)."""
+        chunk = chunk or max(1, self.opts.mu_update_frequency)
+        while True:
+            self.solve(chunk, monitor)
+            st = [self.status(i) for i in range(self.count)]
+            if all(s["converged"] or s["oiter"] >= self.opts.maxiter or s["status_bits"] for s in st):
+                return sum(s["converged"] for s in st)
+
+    def last_solve_seconds(self) -> float:
+        return N.lib().v2h_batch_last_solve_seconds(self._h)
+
+    def status(self, i) -> dict:
+        s = N.Status()
+        N.check(N.lib().v2h_batch_status(self._h, i, C.byref(s)))
+        return {"mu": s.mu, "energy_primal": s.energy_primal, "energy_dual": s.energy_dual,
+                "eps_p": s.primal_error, "eps_d": s.dual_error, "oiter": s.oiter_total,
+                "iiter": s.iiter_total, "converged": bool(s.converged), "active": bool(s.active),
+                "status_bits": s.status_bits, "cg_last": s.last_cg_iterations}
+
+    # ---- vectors ---------------------------------------------------------------
+    def _vec(self, fn, i, n):
+        out = np.zeros(n)
+        N.check(fn(self._h, i, out))
+        return out
+
+    def x(self, i):
+        return self._vec(N.lib().v2h_batch_get_x, i, self.n_primal[i])
+
+    def y(self, i):
+        return self._vec(N.lib().v2h_batch_get_y, i, self.n_dual[i])
+
+    def z(self, i):
+        return self._vec(N.lib().v2h_batch_get_z, i, self.n_primal[i])
+
+    def c(self, i):
+        return self._vec(N.lib().v2h_batch_get_c, i, self.n_primal[i])
+
+    def b(self, i):
+        return self._vec(N.lib().v2h_batch_get_b, i, self.n_dual[i])
+
+    def set_y(self, i, y):
+        N.check(N.lib().v2h_batch_set_y(self._h, i, np.ascontiguousarray(y, dtype=np.float64)))
+
+    def set_z(self, i, z):
+        N.check(N.lib().v2h_batch_set_z(self._h, i, np.ascontiguousarray(z, dtype=np.float64)))
+
+    def mu(self, i):
+        m = C.c_double()
+        N.check(N.lib().v2h_batch_get_mu(self._h, i, C.byref(m)))
+        return m.value
+
+    def set_mu(self, i, mu):
+        N.check(N.lib().v2h_batch_set_mu(self._h, i, float(mu)))
+
+    def Au(self, i, u):
+        out = np.zeros(self.n_dual[i])
+        N.check(N.lib().v2h_batch_Au(self._h, i, np.ascontiguousarray(u, dtype=np.float64), out))
+        return out
+
+    def ATu(self, i, y):
+        out = np.zeros(self.n_primal[i])
+        N.check(N.lib().v2h_batch_ATu(self._h, i, np.ascontiguousarray(y, dtype=np.float64), out))
+        return out
+
+    def cost_adjoint(self, i, xbar=None):
+        n = self.problems[i].nmo
+        dh = np.zeros(n * n)
+        dg = np.zeros(n ** 4)
+        ptr = None
+        if xbar is not None:
+            xbar = np.ascontiguousarray(xbar, dtype=np.float64)
+            ptr = xbar.ctypes.data_as(C.c_void_p)
+        N.check(N.lib().v2h_batch_cost_adjoint(self._h, i, ptr, dh, dg))
+        return dh.reshape(n, n), dg.reshape(n, n, n, n)

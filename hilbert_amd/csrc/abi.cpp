@@ -1,0 +1,432 @@
+// abi.cpp -- extern "C" entry points (include/v2rdm_hip.h).  Exceptions are
+// caught here and turned into status codes plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/v2rdm_hip.h"
+#include "solver.hpp"
+
+struct v2h_batch {
+    std::unique_ptr<v2h::Batch> impl;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return V2H_OK;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return V2H_ERR_INVALID_ARGUMENT;
+    } catch (const v2h::HipError& e) {
+        g_err = e.what();
+        return V2H_ERR_HIP;
+    } catch (const std::bad_alloc& e) {
+        g_err = "out of host memory";
+        return V2H_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return V2H_ERR_INTERNAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return V2H_ERR_INTERNAL;
+    }
+}
+
+v2h::SolverOptions to_opts(const v2h_sdp_options* o) {
+    v2h::SolverOptions s{};
+    s.e_convergence = o->sdp_objective_convergence;
+    s.r_convergence = o->sdp_error_convergence;
+    s.cg_convergence = o->cg_convergence;
+    s.cg_maxiter = o->cg_maxiter;
+    s.maxiter = o->maxiter;
+    s.mu_update_frequency = o->mu_update_frequency;
+    s.mu0 = o->mu0;
+    return s;
+}
+
+v2h::Batch& B(v2h_batch* b) {
+    if (!b || !b->impl) throw std::invalid_argument("null batch handle");
+    return *b->impl;
+}
+const v2h::Batch& B(const v2h_batch* b) {
+    if (!b || !b->impl) throw std::invalid_argument("null batch handle");
+    return *b->impl;
+}
+void check_inst(const v2h::Batch& b, int i) {
+    if (i < 0 || i >= b.count()) throw std::invalid_argument("instance index out of range");
+}
+void check_ptr(const void* p) {
+    if (!p) throw std::invalid_argument("null pointer argument");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* v2h_last_error(void) { return g_err.c_str(); }
+
+int v2h_default_options(v2h_sdp_options* o) {
+    return guarded([&] {
+        check_ptr(o);
+        o->sdp_objective_convergence = 1e-6;
+        o->sdp_error_convergence = 1e-5;
+        o->cg_convergence = 1e-9;
+        o->cg_maxiter = 10000;
+        o->maxiter = 50;
+        o->mu_update_frequency = 1000;
+        o->mu0 = 1.0;
+    });
+}
+
+int v2h_device_count(int* count) {
+    return guarded([&] {
+        check_ptr(count);
+        V2H_HIP(hipGetDeviceCount(count));
+    });
+}
+
+int v2h_batch_create(const v2h_problem_desc* descs, int count, const v2h_sdp_options* opt, int device,
+                     v2h_batch** out) {
+    return guarded([&] {
+        check_ptr(descs);
+        check_ptr(opt);
+        check_ptr(out);
+        *out = nullptr;
+        if (count <= 0) throw std::invalid_argument("count must be positive");
+        std::vector<v2h::ProblemDesc> ds(count);
+        for (int i = 0; i < count; i++) {
+            const v2h_problem_desc& s = descs[i];
+            v2h::ProblemDesc& d = ds[i];
+            d.nmo = s.nmo;
+            d.nalpha = s.nalpha;
+            d.nbeta = s.nbeta;
+            d.multiplicity = s.multiplicity;
+            d.nirrep = s.nirrep;
+            if (s.orbital_irreps) d.irreps.assign(s.orbital_irreps, s.orbital_irreps + s.nmo);
+            d.q2 = s.constrain_q2 != 0;
+            d.g2 = s.constrain_g2 != 0;
+            d.spin = s.constrain_spin != 0;
+            d.sz = s.constrain_sz != 0;
+        }
+        auto h = std::make_unique<v2h_batch>();
+        h->impl = std::make_unique<v2h::Batch>(ds, to_opts(opt), device);
+        *out = h.release();
+    });
+}
+
+int v2h_batch_destroy(v2h_batch* b) {
+    return guarded([&] { delete b; });
+}
+
+int v2h_batch_count(const v2h_batch* b, int* count) {
+    return guarded([&] {
+        check_ptr(count);
+        *count = B(b).count();
+    });
+}
+
+int v2h_batch_set_options(v2h_batch* b, const v2h_sdp_options* opt) {
+    return guarded([&] {
+        check_ptr(opt);
+        B(b).set_options(to_opts(opt));
+    });
+}
+
+int v2h_batch_sizes(const v2h_batch* b, int inst, int64_t* np, int64_t* nd, int* nb) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        if (np) *np = B(b).n_primal(inst);
+        if (nd) *nd = B(b).n_dual(inst);
+        if (nb) *nb = (int)B(b).layout(inst).blocks.size();
+    });
+}
+
+int v2h_batch_block_dims(const v2h_batch* b, int inst, int* dims) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(dims);
+        const auto& bl = B(b).layout(inst).blocks;
+        for (size_t k = 0; k < bl.size(); k++) dims[k] = bl[k].dim;
+    });
+}
+
+int v2h_batch_orbital_order(const v2h_batch* b, int inst, int* to_user) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(to_user);
+        const auto& v = B(b).layout(inst).to_user;
+        for (size_t k = 0; k < v.size(); k++) to_user[k] = v[k];
+    });
+}
+
+int v2h_batch_set_integrals(v2h_batch* b, int inst, const double* h, const double* g) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(h);
+        check_ptr(g);
+        B(b).set_integrals(inst, h, g);
+    });
+}
+
+int v2h_batch_set_cost(v2h_batch* b, int inst, const double* c) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(c);
+        B(b).set_cost(inst, c);
+    });
+}
+
+int v2h_batch_set_guess(v2h_batch* b, int inst, const double* x) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        B(b).set_guess(inst, x);
+    });
+}
+
+int v2h_batch_reset(v2h_batch* b, int inst) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        B(b).reset_state(inst);
+    });
+}
+
+int v2h_batch_solve(v2h_batch* b, int maxiter, v2h_monitor_fn mon, void* data, int* n_converged) {
+    return guarded([&] {
+        if (maxiter < 0) throw std::invalid_argument("maxiter must be non-negative");
+        v2h::Monitor m;
+        if (mon)
+            m = [&](int i, const v2h::InstStatus& s) {
+                mon(i, (int)s.oiter, s.cg_last, s.energy_primal, s.energy_dual, s.mu, s.eps_p, s.eps_d, data);
+            };
+        int nc = B(b).solve(maxiter, m);
+        if (n_converged) *n_converged = nc;
+    });
+}
+
+int v2h_batch_status(v2h_batch* b, int inst, v2h_status* st) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(st);
+        v2h::InstStatus s = B(b).status(inst);
+        st->mu = s.mu;
+        st->energy_primal = s.energy_primal;
+        st->energy_dual = s.energy_dual;
+        st->primal_error = s.eps_p;
+        st->dual_error = s.eps_d;
+        st->oiter_total = s.oiter;
+        st->iiter_total = s.iiter;
+        st->converged = s.converged;
+        st->active = s.active;
+        st->status_bits = s.status;
+        st->last_cg_iterations = s.cg_last;
+    });
+}
+
+double v2h_batch_last_solve_seconds(const v2h_batch* b) {
+    return (b && b->impl) ? b->impl->last_solve_seconds() : -1.0;
+}
+
+#define V2H_VEC_GETTER(name, method)                              \
+    int name(v2h_batch* b, int inst, double* out) {                \
+        return guarded([&] {                                        \
+            check_inst(B(b), inst);                                 \
+            check_ptr(out);                                         \
+            B(b).method(inst, out);                                 \
+        });                                                         \
+    }
+V2H_VEC_GETTER(v2h_batch_get_x, get_primal)
+V2H_VEC_GETTER(v2h_batch_get_y, get_dual)
+V2H_VEC_GETTER(v2h_batch_get_z, get_slack)
+V2H_VEC_GETTER(v2h_batch_get_c, get_cost)
+V2H_VEC_GETTER(v2h_batch_get_b, get_b)
+#undef V2H_VEC_GETTER
+
+int v2h_batch_set_y(v2h_batch* b, int inst, const double* y) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(y);
+        B(b).set_dual(inst, y);
+    });
+}
+
+int v2h_batch_set_z(v2h_batch* b, int inst, const double* z) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(z);
+        B(b).set_slack(inst, z);
+    });
+}
+
+int v2h_batch_get_mu(v2h_batch* b, int inst, double* mu) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(mu);
+        *mu = B(b).get_mu(inst);
+    });
+}
+
+int v2h_batch_set_mu(v2h_batch* b, int inst, double mu) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        if (!(mu > 0.0)) throw std::invalid_argument("mu must be positive");
+        B(b).set_mu(inst, mu);
+    });
+}
+
+int v2h_batch_Au(v2h_batch* b, int inst, const double* u, double* out) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(u);
+        check_ptr(out);
+        B(b).apply_A(inst, u, out);
+    });
+}
+
+int v2h_batch_ATu(v2h_batch* b, int inst, const double* y, double* out) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(y);
+        check_ptr(out);
+        B(b).apply_AT(inst, y, out);
+    });
+}
+
+int v2h_batch_cost_adjoint(v2h_batch* b, int inst, const double* xbar, double* dh, double* dg) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(dh);
+        check_ptr(dg);
+        B(b).cost_adjoint(inst, xbar, dh, dg);
+    });
+}
+
+}  // extern "C"
+
+// ---- host-only structure inspection -------------------------------------
+
+struct v2h_structure {
+    v2h::Layout L;
+    v2h::Csr A;
+    v2h::CostMap cost;
+    v2h::SellHost sA, sAT;
+    explicit v2h_structure(const v2h::ProblemDesc& d) : L(d) {
+        A = v2h::build_constraints(L);
+        cost = v2h::build_cost_map(L);
+        sA = v2h::build_sell(A, 32);
+        sAT = v2h::build_sell(A.transpose(), 32);
+    }
+};
+
+namespace {
+const v2h_structure& Sref(const v2h_structure* s) {
+    if (!s) throw std::invalid_argument("null structure handle");
+    return *s;
+}
+}  // namespace
+
+extern "C" {
+
+int v2h_structure_create(const v2h_problem_desc* s, v2h_structure** out) {
+    return guarded([&] {
+        check_ptr(s);
+        check_ptr(out);
+        *out = nullptr;
+        v2h::ProblemDesc d;
+        d.nmo = s->nmo;
+        d.nalpha = s->nalpha;
+        d.nbeta = s->nbeta;
+        d.multiplicity = s->multiplicity;
+        d.nirrep = s->nirrep;
+        if (s->orbital_irreps) d.irreps.assign(s->orbital_irreps, s->orbital_irreps + s->nmo);
+        d.q2 = s->constrain_q2 != 0;
+        d.g2 = s->constrain_g2 != 0;
+        d.spin = s->constrain_spin != 0;
+        d.sz = s->constrain_sz != 0;
+        *out = new v2h_structure(d);
+    });
+}
+
+int v2h_structure_destroy(v2h_structure* s) {
+    return guarded([&] { delete s; });
+}
+
+int v2h_structure_sizes(const v2h_structure* s, int64_t* np, int64_t* nd, int64_t* nnz, int* nb) {
+    return guarded([&] {
+        const auto& S = Sref(s);
+        if (np) *np = S.L.n_primal;
+        if (nd) *nd = S.A.nrows;
+        if (nnz) *nnz = S.A.nnz();
+        if (nb) *nb = (int)S.L.blocks.size();
+    });
+}
+
+int v2h_structure_block_dims(const v2h_structure* s, int* dims) {
+    return guarded([&] {
+        check_ptr(dims);
+        const auto& bl = Sref(s).L.blocks;
+        for (size_t k = 0; k < bl.size(); k++) dims[k] = bl[k].dim;
+    });
+}
+
+int v2h_structure_tables(const v2h_structure* s, int* gab, int* gaa, int* bab, int* baa) {
+    return guarded([&] {
+        const auto& L = Sref(s).L;
+        int kab = 0, kaa = 0;
+        for (int h = 0; h < L.H; h++) {
+            if (gab) gab[h] = L.gab[h];
+            if (gaa) gaa[h] = L.gaa[h];
+            for (const auto& p : L.ab_pairs[h]) {
+                if (bab) { bab[kab] = p[0]; bab[kab + 1] = p[1]; }
+                kab += 2;
+            }
+            for (const auto& p : L.aa_pairs[h]) {
+                if (baa) { baa[kaa] = p[0]; baa[kaa + 1] = p[1]; }
+                kaa += 2;
+            }
+        }
+    });
+}
+
+int v2h_structure_constraints(const v2h_structure* s, int64_t* ptr, int32_t* col, double* val, double* b) {
+    return guarded([&] {
+        const auto& A = Sref(s).A;
+        if (ptr) std::copy(A.ptr.begin(), A.ptr.end(), ptr);
+        if (col) std::copy(A.col.begin(), A.col.end(), col);
+        if (val) std::copy(A.val.begin(), A.val.end(), val);
+        if (b) std::copy(A.rhs.begin(), A.rhs.end(), b);
+    });
+}
+
+int v2h_structure_cost_map(const v2h_structure* s, int32_t* i1, int32_t* i2) {
+    return guarded([&] {
+        const auto& c = Sref(s).cost;
+        if (i1) std::copy(c.i1.begin(), c.i1.end(), i1);
+        if (i2) std::copy(c.i2.begin(), c.i2.end(), i2);
+    });
+}
+
+int v2h_structure_gather_stats(const v2h_structure* s, int64_t* st) {
+    return guarded([&] {
+        check_ptr(st);
+        const auto& S = Sref(s);
+        st[0] = S.sA.nslices;
+        st[1] = (int64_t)S.sA.ent.size();
+        st[2] = S.sA.nlong();
+        st[3] = (int64_t)S.sA.long_ent.size();
+        st[4] = S.sAT.nslices;
+        st[5] = (int64_t)S.sAT.ent.size();
+        st[6] = S.sAT.nlong();
+        st[7] = (int64_t)S.sAT.long_ent.size();
+    });
+}
+
+}  // extern "C"

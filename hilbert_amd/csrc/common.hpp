@@ -1,0 +1,108 @@
+// common.hpp -- device-resident data structures shared by the host driver and
+// the HIP kernels.  One "structure" holds the gather tables of one problem
+// shape (shared by every instance with the same shape); one "instance" is one
+// SDP of the batch, whose vectors live at fixed offsets inside batch-wide
+// primal and dual arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#define V2H_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw v2h::HipError(std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+namespace v2h {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+constexpr int kLongFlag = 1 << 30;   // work-unit code: long row (else: slice)
+
+// SELL-64 operator tables on the device (see sell.hpp).
+struct SellDev {
+    const uint32_t* ent;
+    const uint32_t* slice_start;
+    const uint32_t* long_start;
+    const uint32_t* long_ent;
+    const int32_t* long_row;
+    const double* coef;     // [256]
+    int64_t nrows;
+    int nslices, nlong;
+};
+
+struct StructDev {
+    SellDev A, AT;
+    const double* b;        // [n_dual]
+    const int32_t* cost1;   // [n_primal] cost-vector gather map
+    const int32_t* cost2;
+    const int32_t* adj_ptr; // [nsrc + 1] transposed cost map (adjoint gather)
+    const int32_t* adj_ent; // primal index, sign in bit 31 (set = minus)
+    int64_t n_primal, n_dual, nsrc;
+};
+
+struct InstDev {
+    int64_t poff, doff;     // offsets into batch primal / dual arrays
+    int64_t soff;           // offset into the batch integral array [h; g]
+    int sid;                // structure id
+    int block_first, nblocks;
+};
+
+// Per-instance solver state, written only by the reducing workgroup of a
+// kernel (the last one to finish) or by the host between launches.
+struct InstState {
+    double mu, bnorm, cnorm;
+    double e_primal, e_dual, eps_p, eps_d;
+    double cg_thr;              // tau^2 ||rhs||^2 of the current CG solve
+    double rr, pw, rw, ww;      // CG scalars of the last matvec
+    int64_t oiter, iiter;
+    int cg_iter, cg_done, cg_last;
+    int active, converged, pending, status;  // pending: a projection awaits its bookkeeping
+    int maxiter_left;           // outer iterations left in the current solve() call
+};
+
+enum StatusBits : int { kStatusJacobiNotConverged = 1, kStatusNonFinite = 2 };
+
+struct SolverOptions {
+    double e_convergence, r_convergence, cg_convergence, mu0;
+    int cg_maxiter, maxiter, mu_update_frequency;
+};
+
+// Batch-wide device arrays.
+struct BatchDev {
+    const StructDev* structs;
+    const InstDev* inst;
+    InstState* st;
+    // primal-length arrays
+    double *x, *z, *c, *v1, *u, *aty, *q;
+    // dual-length arrays
+    double *y, *r, *p, *w, *av1, *ax, *aaty;
+    // reductions
+    double* partials;       // per-workgroup partial dot products
+    int* counters;          // per-instance arrival counters (per kernel kind)
+    double* proj_part;      // per-block projection partials (c.x, dual residual)
+    SolverOptions opt;
+};
+
+// One work unit list per operator direction.
+struct UnitsDev {
+    const int4* units;      // 4 wave units per workgroup (-1 = none)
+    const int* wg_inst;     // instance of each workgroup
+    const int* wg_begin;    // [ninst + 1] first workgroup of each instance
+    int nwg;
+};
+
+struct ProjItem {
+    int inst, d, gblock, pad;
+    int64_t off;            // primal offset inside the instance
+    int64_t rec_off;        // offset of the rotation record (doubles)
+};
+
+}  // namespace v2h

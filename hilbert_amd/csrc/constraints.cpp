@@ -1,0 +1,406 @@
+// constraints.cpp -- DQG + spin rows of A as (column, coefficient) lists.
+//
+// Row families and their order follow hilbert's bpsdp_Au
+// (v2rdm_solver.cc:2995-3050): D2 (traces, hermiticity, D1/Q1, contractions;
+// d2.cc:314-594), spin (spin.cc:276-508), Q2 (q2.cc:170-317), G2
+// (g2.cc:245-441).  Each row is written here as the linear functional it
+// imposes on the primal vector, with its b value.
+#include "constraints.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace v2h {
+
+namespace {
+
+class RowWriter {
+  public:
+    explicit RowWriter(Csr& out) : out_(out) { out_.ptr.assign(1, 0); }
+    void add(int64_t col, double v) { cur_.push_back({col, v}); }
+    void end(double b) {
+        std::sort(cur_.begin(), cur_.end(),
+                  [](const Term& a, const Term& c) { return a.col < c.col; });
+        size_t k = 0;
+        while (k < cur_.size()) {
+            int64_t c = cur_[k].col;
+            double v = 0.0;
+            for (; k < cur_.size() && cur_[k].col == c; k++) v += cur_[k].v;
+            if (v != 0.0) {
+                out_.col.push_back((int32_t)c);
+                out_.val.push_back(v);
+            }
+        }
+        cur_.clear();
+        out_.rhs.push_back(b);
+        out_.ptr.push_back((int64_t)out_.col.size());
+    }
+
+  private:
+    struct Term { int64_t col; double v; };
+    Csr& out_;
+    std::vector<Term> cur_;
+};
+
+inline double sgn(int a, int b) { return a < b ? 1.0 : -1.0; }
+
+}  // namespace
+
+Csr build_constraints(const Layout& L) {
+    Csr A;
+    A.ncols = L.n_primal;
+    RowWriter R(A);
+    const int n = L.n, H = L.H;
+    const double na = L.na, nb = L.nb;
+    const auto& d = L.desc;
+
+    // element helpers (orbital indices in Pitzer order)
+    auto d2ab = [&](int i, int j, int k, int l) -> int64_t {
+        int h = L.pair_irrep(i, j);
+        int a = L.ab(h, i, j), b = L.ab(h, k, l);
+        if (a < 0 || b < 0) throw std::logic_error("D2ab element outside its irrep block");
+        return L.off[D2AB][h] + (int64_t)a * L.gab[h] + b;
+    };
+    auto gem = [&](int f, int h, int a, int b, int dim) -> int64_t {
+        return L.off[f][h] + (int64_t)a * dim + b;
+    };
+    // D2aa / D2bb element for ordered pairs (i,j),(k,l): index and sign
+    auto d2same = [&](int f, int i, int j, int k, int l, double& s) -> int64_t {
+        int h = L.pair_irrep(i, j);
+        s = sgn(i, j) * sgn(k, l);
+        return L.off[f][h] + (int64_t)L.aa(h, i, j) * L.gaa[h] + L.aa(h, k, l);
+    };
+
+    // ---- traces
+    auto trace_ab = [&](double w) {
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) R.add(d2ab(i, j, i, j), w);
+    };
+    auto trace_same = [&](int f) {
+        double s;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++)
+                if (i != j) R.add(d2same(f, i, j, i, j, s), 1.0);
+    };
+    if (d.sz) {
+        trace_ab(1.0); R.end(na * nb);
+        trace_same(D2AA); R.end(na * (na - 1.0));
+        trace_same(D2BB); R.end(nb * (nb - 1.0));
+    } else {
+        trace_ab(2.0); trace_same(D2AA); trace_same(D2BB);
+        R.end((na + nb) * (na + nb - 1.0));
+    }
+
+    // ---- hermiticity: X(ij,kl) - X(kl,ij) = 0 for D2aa, D2bb, D2ab
+    for (int f : {D2AA, D2BB, D2AB})
+        for (int h = 0; h < H; h++) {
+            int g = f == D2AB ? L.gab[h] : L.gaa[h];
+            for (int a = 0; a < g; a++)
+                for (int b = 0; b < g; b++) {
+                    R.add(gem(f, h, a, b, g), 1.0);
+                    R.add(gem(f, h, b, a, g), -1.0);
+                    R.end(0.0);
+                }
+        }
+
+    // ---- D1(j,i) + Q1(i,j) = delta_ij
+    for (int s = 0; s < 2; s++)
+        for (int h = 0; h < H; h++) {
+            int m = L.amopi[h];
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++) {
+                    R.add(L.off[s ? D1B : D1A][h] + (int64_t)j * m + i, 1.0);
+                    R.add(L.off[s ? Q1B : Q1A][h] + (int64_t)i * m + j, 1.0);
+                    R.end(i == j ? 1.0 : 0.0);
+                }
+        }
+
+    // ---- contractions to D1
+    // ab part: alpha -> sum_k D2ab(ik,jk); beta -> sum_k D2ab(ki,kj)
+    auto contract_ab = [&](int s, int ii, int jj) {
+        for (int k = 0; k < n; k++)
+            R.add(s == 0 ? d2ab(ii, k, jj, k) : d2ab(k, ii, k, jj), -1.0);
+    };
+    auto contract_same = [&](int s, int ii, int jj) {
+        double sg;
+        for (int k = 0; k < n; k++) {
+            if (k == ii || k == jj) continue;
+            int64_t e = d2same(s ? D2BB : D2AA, ii, k, jj, k, sg);
+            R.add(e, -sg);
+        }
+    };
+    auto contraction_block = [&](int s, double pre, bool with_ab, bool with_same) {
+        for (int h = 0; h < H; h++) {
+            int m = L.amopi[h], o = L.poff[h];
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++) {
+                    R.add(L.off[s ? D1B : D1A][h] + (int64_t)i * m + j, pre);
+                    if (with_ab) contract_ab(s, i + o, j + o);
+                    if (with_same) contract_same(s, i + o, j + o);
+                    R.end(0.0);
+                }
+        }
+    };
+    if (d.sz) {
+        contraction_block(0, nb, true, false);
+        contraction_block(1, na, true, false);
+        contraction_block(0, na - 1.0, false, true);
+        contraction_block(1, nb - 1.0, false, true);
+    } else {
+        contraction_block(0, na + nb - 1.0, true, true);
+        contraction_block(1, na + nb - 1.0, true, true);
+    }
+
+    // ---- spin
+    if (d.spin) {
+        double ms = (L.mult - 1.0) / 2.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) R.add(d2ab(i, j, j, i), 1.0);
+        R.end(0.5 * (na + nb) + ms * ms - ms * (ms + 1.0));
+
+        if (L.singlet()) {
+            for (int h = 0; h < H; h++) {  // D1a = D1b
+                int64_t mm = (int64_t)L.amopi[h] * L.amopi[h];
+                for (int64_t t = 0; t < mm; t++) {
+                    R.add(L.off[D1A][h] + t, 1.0);
+                    R.add(L.off[D1B][h] + t, -1.0);
+                    R.end(0.0);
+                }
+            }
+            for (int h = 0; h < H; h++) {  // D2aa = D2bb
+                int64_t gg = (int64_t)L.gaa[h] * L.gaa[h];
+                for (int64_t t = 0; t < gg; t++) {
+                    R.add(L.off[D2AA][h] + t, 1.0);
+                    R.add(L.off[D2BB][h] + t, -1.0);
+                    R.end(0.0);
+                }
+            }
+            for (int f : {D2AA, D2BB})  // D2ss(ij,kl) = 1/2 [ab(ij,kl) - ab(ji,kl) - ab(ij,lk) + ab(ji,lk)]
+                for (int h = 0; h < H; h++) {
+                    int g = L.gaa[h];
+                    for (int a = 0; a < g; a++) {
+                        auto [i, j] = L.aa_pairs[h][a];
+                        for (int b = 0; b < g; b++) {
+                            auto [k, l] = L.aa_pairs[h][b];
+                            R.add(gem(f, h, a, b, g), 1.0);
+                            R.add(d2ab(i, j, k, l), -0.5);
+                            R.add(d2ab(j, i, k, l), 0.5);
+                            R.add(d2ab(i, j, l, k), 0.5);
+                            R.add(d2ab(j, i, l, k), -0.5);
+                            R.end(0.0);
+                        }
+                    }
+                }
+            for (int h = 0; h < H; h++) {  // D200 from D2ab
+                int g = L.gab[h];
+                for (int a = 0; a < g; a++) {
+                    auto [i, j] = L.ab_pairs[h][a];
+                    double dij = i == j ? std::sqrt(2.0) : 1.0;
+                    for (int b = 0; b < g; b++) {
+                        auto [k, l] = L.ab_pairs[h][b];
+                        double dkl = k == l ? std::sqrt(2.0) : 1.0;
+                        double f = -0.5 / (dij * dkl);
+                        R.add(gem(D200, h, a, b, g), 1.0);
+                        R.add(d2ab(i, j, k, l), f);
+                        R.add(d2ab(j, i, k, l), f);
+                        R.add(d2ab(i, j, l, k), f);
+                        R.add(d2ab(j, i, l, k), f);
+                        R.end(0.0);
+                    }
+                }
+            }
+            for (int h = 0; h < H; h++) {  // D2ab(ij,kl) = D2ab(ji,lk)
+                int g = L.gab[h];
+                for (int a = 0; a < g; a++) {
+                    auto [i, j] = L.ab_pairs[h][a];
+                    for (int b = 0; b < g; b++) {
+                        auto [k, l] = L.ab_pairs[h][b];
+                        R.add(d2ab(i, j, k, l), 1.0);
+                        R.add(d2ab(j, i, l, k), -1.0);
+                        R.end(0.0);
+                    }
+                }
+            }
+        } else {
+            // (2g x 2g) D200 block, quadrants 00, 01, 10, 11 (row-major over the block)
+            for (int h = 0; h < H; h++) {
+                int g = L.gab[h], G = 2 * g;
+                for (int I = 0; I < G; I++)
+                    for (int K = 0; K < G; K++) {
+                        int a = I % g, b = K % g;
+                        auto [i, j] = L.ab_pairs[h][a];
+                        auto [k, l] = L.ab_pairs[h][b];
+                        double dij = i == j ? std::sqrt(2.0) : 1.0;
+                        double dkl = k == l ? std::sqrt(2.0) : 1.0;
+                        // signs of ab(ij,kl), ab(ji,kl), ab(ij,lk), ab(ji,lk) and the prefactor
+                        double f, s1, s2, s3, s4;
+                        if (I < g && K < g)       { f = 0.5 / (dij * dkl); s1 = s2 = s3 = s4 = 1.0; }
+                        else if (I < g)           { f = 0.5 / dij; s1 = 1; s2 = 1; s3 = -1; s4 = -1; }
+                        else if (K < g)           { f = 0.5 / dkl; s1 = 1; s2 = -1; s3 = 1; s4 = -1; }
+                        else                      { f = 0.5; s1 = 1; s2 = -1; s3 = -1; s4 = 1; }
+                        R.add(gem(D200, h, I, K, G), 1.0);
+                        R.add(d2ab(i, j, k, l), -f * s1);
+                        R.add(d2ab(j, i, k, l), -f * s2);
+                        R.add(d2ab(i, j, l, k), -f * s3);
+                        R.add(d2ab(j, i, l, k), -f * s4);
+                        R.end(0.0);
+                    }
+            }
+        }
+        if (d.g2) {  // maximal spin: sum_i G2ba(kl,ii) = 0 and sum_i G2ba(ii,kl) = 0, irrep 0 only
+            int g = L.gab[0];
+            for (int t = 0; t < 2; t++)
+                for (int a = 0; a < g; a++) {
+                    for (int i = 0; i < n; i++) {
+                        int ii = L.ab(0, i, i);
+                        R.add(t == 0 ? gem(G2BA, 0, a, ii, g) : gem(G2BA, 0, ii, a, g), 1.0);
+                    }
+                    R.end(0.0);
+                }
+        }
+    }
+
+    // ---- Q2
+    if (d.q2) {
+        for (int h = 0; h < H; h++) {
+            int g = L.gab[h];
+            for (int a = 0; a < g; a++) {
+                auto [i, j] = L.ab_pairs[h][a];
+                for (int b = 0; b < g; b++) {
+                    auto [k, l] = L.ab_pairs[h][b];
+                    R.add(gem(D2AB, h, a, b, g), 1.0);
+                    R.add(gem(Q2AB, h, a, b, g), -1.0);
+                    if (j == l) R.add(L.one_body(D1A, k, i), -1.0);
+                    if (i == k) R.add(L.one_body(D1B, l, j), -1.0);
+                    R.end(-(double)(i == k && j == l));
+                }
+            }
+        }
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < H; h++) {
+                int g = L.gaa[h];
+                int fd = s ? D2BB : D2AA, fq = s ? Q2BB : Q2AA, f1 = s ? D1B : D1A;
+                for (int a = 0; a < g; a++) {
+                    auto [i, j] = L.aa_pairs[h][a];
+                    for (int b = 0; b < g; b++) {
+                        auto [k, l] = L.aa_pairs[h][b];
+                        R.add(gem(fd, h, a, b, g), 1.0);
+                        R.add(gem(fq, h, a, b, g), -1.0);
+                        if (j == l) R.add(L.one_body(f1, k, i), -1.0);
+                        if (j == k) R.add(L.one_body(f1, l, i), 1.0);
+                        if (i == l) R.add(L.one_body(f1, k, j), 1.0);
+                        if (i == k) R.add(L.one_body(f1, l, j), -1.0);
+                        R.end(-(double)(i == k && j == l) + (double)(i == l && j == k));
+                    }
+                }
+            }
+    }
+
+    // ---- G2
+    if (d.g2) {
+        for (int s = 0; s < 2; s++)  // G2ab, G2ba
+            for (int h = 0; h < H; h++) {
+                int g = L.gab[h];
+                for (int a = 0; a < g; a++) {
+                    auto [i, j] = L.ab_pairs[h][a];
+                    for (int b = 0; b < g; b++) {
+                        auto [k, l] = L.ab_pairs[h][b];
+                        R.add(gem(s ? G2BA : G2AB, h, a, b, g), -1.0);
+                        if (j == l) R.add(L.one_body(s ? D1B : D1A, i, k), 1.0);
+                        R.add(s ? d2ab(l, i, j, k) : d2ab(i, l, k, j), -1.0);
+                        R.end(0.0);
+                    }
+                }
+            }
+        for (int h = 0; h < H; h++) {
+            int g = L.gab[h], G = 2 * g;
+            for (int I = 0; I < G; I++)
+                for (int K = 0; K < G; K++) {
+                    auto [i, j] = L.ab_pairs[h][I % g];
+                    auto [k, l] = L.ab_pairs[h][K % g];
+                    R.add(gem(G2AA, h, I, K, G), -1.0);
+                    if ((I < g) == (K < g)) {  // aaaa or bbbb
+                        int beta = I >= g;
+                        if (j == l) R.add(L.one_body(beta ? D1B : D1A, i, k), 1.0);
+                        if (i != l && k != j) {
+                            double sg;
+                            int64_t e = d2same(beta ? D2BB : D2AA, i, l, k, j, sg);
+                            R.add(e, -sg);
+                        }
+                    } else if (I < g) {      // aabb
+                        R.add(d2ab(i, l, j, k), 1.0);
+                    } else {                 // bbaa
+                        R.add(d2ab(l, i, k, j), 1.0);
+                    }
+                    R.end(0.0);
+                }
+        }
+    }
+
+    A.nrows = (int64_t)A.rhs.size();
+    return A;
+}
+
+Csr Csr::transpose() const {
+    Csr T;
+    T.nrows = ncols;
+    T.ncols = nrows;
+    T.ptr.assign(ncols + 1, 0);
+    for (int32_t c : col) T.ptr[c + 1]++;
+    for (int64_t c = 0; c < ncols; c++) T.ptr[c + 1] += T.ptr[c];
+    T.col.resize(col.size());
+    T.val.resize(val.size());
+    std::vector<int64_t> cur(T.ptr.begin(), T.ptr.end() - 1);
+    for (int64_t r = 0; r < nrows; r++)
+        for (int64_t e = ptr[r]; e < ptr[r + 1]; e++) {
+            int64_t dst = cur[col[e]]++;
+            T.col[dst] = (int32_t)r;
+            T.val[dst] = val[e];
+        }
+    return T;
+}
+
+CostMap build_cost_map(const Layout& L) {
+    CostMap M;
+    M.i1.assign(L.n_primal, -1);
+    M.i2.assign(L.n_primal, -1);
+    const int n = L.n;
+    const int64_t nn = (int64_t)n * n;
+    auto u = [&](int p) { return L.to_user[p]; };
+    auto hidx = [&](int p, int q) { return (int64_t)u(p) * n + u(q); };
+    auto gidx = [&](int p, int q, int r, int s) {
+        return nn + (((int64_t)u(p) * n + u(q)) * n + u(r)) * n + u(s);
+    };
+    if (nn + nn * nn >= ((int64_t)1 << 31)) throw std::invalid_argument("integral arrays too large");
+    for (int h = 0; h < L.H; h++) {
+        int m = L.amopi[h], o = L.poff[h];
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+                M.i1[L.off[D1A][h] + (int64_t)i * m + j] = (int32_t)hidx(i + o, j + o);
+                M.i1[L.off[D1B][h] + (int64_t)i * m + j] = (int32_t)hidx(i + o, j + o);
+            }
+        int g = L.gab[h];
+        for (int a = 0; a < g; a++) {
+            auto [i, j] = L.ab_pairs[h][a];
+            for (int b = 0; b < g; b++) {
+                auto [k, l] = L.ab_pairs[h][b];
+                M.i1[L.off[D2AB][h] + (int64_t)a * g + b] = (int32_t)gidx(i, k, j, l);
+            }
+        }
+        int ga = L.gaa[h];
+        for (int a = 0; a < ga; a++) {
+            auto [i, j] = L.aa_pairs[h][a];
+            for (int b = 0; b < ga; b++) {
+                auto [k, l] = L.aa_pairs[h][b];
+                for (int f : {D2AA, D2BB}) {
+                    M.i1[L.off[f][h] + (int64_t)a * ga + b] = (int32_t)gidx(i, k, j, l);
+                    M.i2[L.off[f][h] + (int64_t)a * ga + b] = (int32_t)gidx(i, l, j, k);
+                }
+            }
+        }
+    }
+    return M;
+}
+
+}  // namespace v2h

@@ -1,0 +1,33 @@
+// kernels.hpp -- host-side launchers of the HIP kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace v2h {
+
+void launch_cg_at(const BatchDev& B, const UnitsDev& U, hipStream_t s);
+void launch_cg_a(const BatchDev& B, const UnitsDev& U, hipStream_t s);
+void launch_pre(const BatchDev& B, const UnitsDev& U, hipStream_t s);
+void launch_post(const BatchDev& B, const UnitsDev& U, hipStream_t s);
+void launch_spmv_plain(const SellDev& S, const int4* units, int wg0, int nwg, const double* in, double* out,
+                       hipStream_t s);
+
+// LDS bytes of the Jacobi projection for block size d and nt threads.
+size_t project_lds_bytes(int d, int nt);
+void launch_project(int nt, const BatchDev& B, const ProjItem* items, int nitems, int max_d, double* rec,
+                    int max_sweeps, hipStream_t s);
+void set_project_lds_limit(size_t bytes);
+
+void launch_fill(double* p, int64_t n, double v, hipStream_t s);
+void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+void launch_pack_cost(const int32_t* i1, const int32_t* i2, const double* src, double* c, int64_t n,
+                      hipStream_t s);
+void launch_cost_adjoint(const int32_t* ptr, const int32_t* ent, const double* xbar, double* dsrc, int64_t nsrc,
+                         hipStream_t s);
+void launch_dot1(const double* a, const double* b, int64_t n, double* out, hipStream_t s);
+
+}  // namespace v2h

@@ -1,0 +1,103 @@
+// layout.cpp -- tables and primal layout of a DQG v2RDM problem.
+//
+// Follows edeprince3/hilbert:
+//   basis.cc:47-76      irrep product = XOR of Cotton indices
+//   basis.cc:104-111    orbitals of an irrep are contiguous (Pitzer order)
+//   basis.cc:427-533    geminal enumeration: outer loop over the second
+//                       orbital, inner over the first, pair = (inner, outer);
+//                       aa pairs keep first < second in the same order
+//   v2rdm_solver.cc:3729-3846  primal blocks D2ab, D2aa, D2bb, D200, D1a, D1b,
+//                       Q1a, Q1b, Q2ab, Q2aa, Q2bb, G2ab, G2ba, G2aa(2g)
+#include "layout.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+namespace v2h {
+
+std::string ProblemDesc::key() const {
+    std::ostringstream k;
+    k << nmo << ':' << nalpha << ':' << nbeta << ':' << (multiplicity ? multiplicity : nalpha - nbeta + 1)
+      << ':' << nirrep << ':' << q2 << g2 << spin << sz << ':';
+    std::vector<int> s = irreps.empty() ? std::vector<int>(nmo, 0) : irreps;
+    std::sort(s.begin(), s.end());  // the tables depend only on the irrep counts
+    for (int v : s) k << v << ',';
+    return k.str();
+}
+
+Layout::Layout(const ProblemDesc& d) : desc(d) {
+    n = d.nmo;
+    H = d.nirrep;
+    na = d.nalpha;
+    nb = d.nbeta;
+    mult = d.multiplicity ? d.multiplicity : na - nb + 1;
+    if (n <= 0) throw std::invalid_argument("nmo must be positive");
+    if (H != 1 && H != 2 && H != 4 && H != 8) throw std::invalid_argument("nirrep must be 1, 2, 4 or 8");
+    if (na < 0 || nb < 0 || na > n || nb > n) throw std::invalid_argument("electron counts out of range");
+    std::vector<int> irr = d.irreps.empty() ? std::vector<int>(n, 0) : d.irreps;
+    if ((int)irr.size() != n) throw std::invalid_argument("orbital_irreps has the wrong length");
+    for (int v : irr)
+        if (v < 0 || v >= H) throw std::invalid_argument("orbital irrep out of range");
+
+    // stable sort of the user orbitals by irrep gives the Pitzer order
+    to_user.resize(n);
+    std::iota(to_user.begin(), to_user.end(), 0);
+    std::stable_sort(to_user.begin(), to_user.end(), [&](int a, int b) { return irr[a] < irr[b]; });
+    sym.resize(n);
+    for (int p = 0; p < n; p++) sym[p] = irr[to_user[p]];
+    amopi.fill(0);
+    for (int p = 0; p < n; p++) amopi[sym[p]]++;
+    for (int h = 0, acc = 0; h < H; h++) { poff[h] = acc; acc += amopi[h]; }
+
+    for (int h = 0; h < H; h++) {
+        ab_index[h].assign((size_t)n * n, -1);
+        aa_index[h].assign((size_t)n * n, -1);
+        ab_pairs[h].clear();
+        aa_pairs[h].clear();
+        for (int second = 0; second < n; second++)
+            for (int first = 0; first < n; first++) {
+                if ((sym[first] ^ sym[second]) != h) continue;
+                ab_index[h][first * n + second] = (int)ab_pairs[h].size();
+                ab_pairs[h].push_back({first, second});
+                if (first < second) {
+                    int k = (int)aa_pairs[h].size();
+                    aa_index[h][first * n + second] = k;
+                    aa_index[h][second * n + first] = k;
+                    aa_pairs[h].push_back({first, second});
+                }
+            }
+        gab[h] = (int)ab_pairs[h].size();
+        gaa[h] = (int)aa_pairs[h].size();
+    }
+
+    int64_t o = 0;
+    auto push = [&](int f, int h, int dim) {
+        off[f][h] = o;
+        blocks.push_back({f, h, dim, o});
+        o += (int64_t)dim * dim;
+    };
+    for (int h = 0; h < H; h++) push(D2AB, h, gab[h]);
+    for (int h = 0; h < H; h++) push(D2AA, h, gaa[h]);
+    for (int h = 0; h < H; h++) push(D2BB, h, gaa[h]);
+    if (d.spin)
+        for (int h = 0; h < H; h++) push(D200, h, singlet() ? gab[h] : 2 * gab[h]);
+    for (int f : {D1A, D1B, Q1A, Q1B})
+        for (int h = 0; h < H; h++) push(f, h, amopi[h]);
+    if (d.q2) {
+        for (int h = 0; h < H; h++) push(Q2AB, h, gab[h]);
+        for (int h = 0; h < H; h++) push(Q2AA, h, gaa[h]);
+        for (int h = 0; h < H; h++) push(Q2BB, h, gaa[h]);
+    }
+    if (d.g2) {
+        for (int h = 0; h < H; h++) push(G2AB, h, gab[h]);
+        for (int h = 0; h < H; h++) push(G2BA, h, gab[h]);
+        for (int h = 0; h < H; h++) push(G2AA, h, 2 * gab[h]);
+    }
+    n_primal = o;
+    if (n_primal >= (int64_t)1 << 24)
+        throw std::invalid_argument("problem too large: more than 2^24 primal variables per instance");
+}
+
+}  // namespace v2h

@@ -1,0 +1,508 @@
+// solver.cpp -- host driver of the batched boundary-point solve.
+//
+// Per outer iteration (all instances of the batch at once, converged ones
+// skipped inside the kernels):
+//   k_pre      A [c - z, x, A^T y]; finish the previous iteration
+//   CG steps   (k_cg_at, k_cg_a) until every instance's CG has stopped
+//   k_post     A^T y, U = mu x + A^T y - c
+//   k_project  one workgroup per PSD block (four size classes)
+// The host only reads the small per-instance state array between CG chunks.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
+#include "kernels.hpp"
+
+namespace v2h {
+
+namespace {
+constexpr int kLongThreshold = 32;
+constexpr size_t kLdsLimit = 160 * 1024;
+
+std::vector<int4> unit_list(const SellHost& S) {
+    std::vector<int> u;
+    for (int s = 0; s < S.nslices; s++) u.push_back(s);
+    for (int l = 0; l < S.nlong(); l++) u.push_back(kLongFlag | l);
+    while (u.size() % 4) u.push_back(-1);
+    std::vector<int4> out(u.size() / 4);
+    for (size_t g = 0; g < out.size(); g++) out[g] = make_int4(u[4 * g], u[4 * g + 1], u[4 * g + 2], u[4 * g + 3]);
+    return out;
+}
+
+int proj_class(int d) {
+    if (d <= 16) return 0;
+    if (d <= 64) return 1;
+    if (d <= 140) return 2;
+    return 3;
+}
+constexpr int kClassThreads[4] = {64, 256, 512, 1024};
+}  // namespace
+
+// ------------------------------------------------------------------ Structure
+
+Structure::Structure(const ProblemDesc& d) : L(d) {
+    Csr A = build_constraints(L);
+    n_primal = L.n_primal;
+    n_dual = A.nrows;
+    if (n_dual >= ((int64_t)1 << 24)) throw std::invalid_argument("problem too large: more than 2^24 constraints");
+    b = A.rhs;
+    Csr AT = A.transpose();
+    hA = build_sell(A, kLongThreshold);
+    hAT = build_sell(AT, kLongThreshold);
+    unitsA = unit_list(hA);
+    unitsAT = unit_list(hAT);
+
+    CostMap cm = build_cost_map(L);
+    const int64_t n = L.n;
+    nsrc = n * n + n * n * n * n;
+    std::vector<int32_t> ptr(nsrc + 1, 0), ent;
+    for (int64_t e = 0; e < n_primal; e++) {
+        if (cm.i1[e] >= 0) ptr[cm.i1[e] + 1]++;
+        if (cm.i2[e] >= 0) ptr[cm.i2[e] + 1]++;
+    }
+    for (int64_t t = 0; t < nsrc; t++) ptr[t + 1] += ptr[t];
+    ent.resize(ptr[nsrc]);
+    std::vector<int32_t> cur(ptr.begin(), ptr.end() - 1);
+    for (int64_t e = 0; e < n_primal; e++) {
+        if (cm.i1[e] >= 0) ent[cur[cm.i1[e]]++] = (int32_t)e;
+        if (cm.i2[e] >= 0) ent[cur[cm.i2[e]]++] = (int32_t)((uint32_t)e | 0x80000000u);
+    }
+
+    a_ent.upload(hA.ent); a_slice.upload(hA.slice_start); a_lstart.upload(hA.long_start);
+    a_lent.upload(hA.long_ent); a_lrow.upload(hA.long_row); a_coef.upload(hA.coef);
+    at_ent.upload(hAT.ent); at_slice.upload(hAT.slice_start); at_lstart.upload(hAT.long_start);
+    at_lent.upload(hAT.long_ent); at_lrow.upload(hAT.long_row); at_coef.upload(hAT.coef);
+    b_dev.upload(b);
+    cost1.upload(cm.i1);
+    cost2.upload(cm.i2);
+    adj_ptr.upload(ptr);
+    adj_ent.upload(ent);
+}
+
+StructDev Structure::dev() const {
+    StructDev s{};
+    s.A = SellDev{a_ent.p, a_slice.p, a_lstart.p, a_lent.p, a_lrow.p, a_coef.p, hA.nrows, hA.nslices, hA.nlong()};
+    s.AT = SellDev{at_ent.p, at_slice.p, at_lstart.p, at_lent.p, at_lrow.p, at_coef.p, hAT.nrows, hAT.nslices,
+                   hAT.nlong()};
+    s.b = b_dev.p;
+    s.cost1 = cost1.p;
+    s.cost2 = cost2.p;
+    s.adj_ptr = adj_ptr.p;
+    s.adj_ent = adj_ent.p;
+    s.n_primal = n_primal;
+    s.n_dual = n_dual;
+    s.nsrc = nsrc;
+    return s;
+}
+
+// ---------------------------------------------------------------------- Batch
+
+Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, int device) : opt_(opt), device_(device) {
+    if (descs.empty()) throw std::invalid_argument("empty batch");
+    V2H_HIP(hipSetDevice(device_));
+    V2H_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+    std::map<std::string, int> sid_of;
+    for (const auto& d : descs) {
+        std::string k = d.key();
+        auto it = sid_of.find(k);
+        int sid;
+        if (it == sid_of.end()) {
+            sid = (int)structs_.size();
+            structs_.push_back(std::make_shared<Structure>(d));
+            sid_of[k] = sid;
+        } else {
+            sid = it->second;
+        }
+        // identical shape but a different orbital order: own structure (the
+        // cost map follows the user's order)
+        if (structs_[sid]->L.to_user != Layout(d).to_user) {
+            sid = (int)structs_.size();
+            structs_.push_back(std::make_shared<Structure>(d));
+        }
+        inst_sid_.push_back(sid);
+    }
+    const int N = count();
+    inst_.resize(N);
+    for (int i = 0; i < N; i++) {
+        const Structure& S = *structs_[inst_sid_[i]];
+        inst_[i].poff = np_total_;
+        inst_[i].doff = nd_total_;
+        inst_[i].soff = ns_total_;
+        inst_[i].sid = inst_sid_[i];
+        np_total_ += S.n_primal;
+        nd_total_ += S.n_dual;
+        ns_total_ += S.nsrc;
+    }
+    build_units();
+    build_projection();
+
+    for (auto* b : {&x_, &z_, &c_, &v1_, &u_, &aty_, &q_}) b->alloc(np_total_);
+    for (auto* b : {&y_, &r_, &p_, &w_, &av1_, &ax_, &aaty_}) b->alloc(nd_total_);
+    src_.alloc(ns_total_);
+    partials_.alloc(4 * (size_t)std::max(nwgA_, nwgAT_) + 4);
+    counters_.alloc(N);
+    V2H_HIP(hipMemsetAsync(counters_.p, 0, sizeof(int) * N, stream_));
+    int64_t smax = 0;
+    for (const auto& s : structs_) smax = std::max({smax, s->n_primal, s->n_dual, s->nsrc});
+    scratch_.alloc(2 * smax);
+    for (auto* b : {&x_, &z_, &c_, &v1_, &u_, &aty_, &q_}) launch_fill(b->p, np_total_, 0.0, stream_);
+    for (auto* b : {&y_, &r_, &p_, &w_, &av1_, &ax_, &aaty_}) launch_fill(b->p, nd_total_, 0.0, stream_);
+    launch_fill(src_.p, ns_total_, 0.0, stream_);
+
+    std::vector<StructDev> sd;
+    for (const auto& s : structs_) sd.push_back(s->dev());
+    d_structs_.upload(sd);
+    d_inst_.upload(inst_);
+    state_.assign(N, InstState{});
+    for (int i = 0; i < N; i++) {
+        const auto& b = structs_[inst_sid_[i]]->b;
+        double s = 0.0;
+        for (double v : b) s += v * v;
+        state_[i].bnorm = s > 0 ? std::sqrt(s) : 1.0;
+        state_[i].cnorm = 1.0;
+    }
+    d_state_.alloc(N);
+    V2H_HIP(hipHostMalloc((void**)&pinned_, sizeof(InstState) * N, hipHostMallocDefault));
+    for (int i = 0; i < N; i++) {
+        set_guess(i, nullptr);
+        reset_state(i);
+    }
+    sync_state_to_device();
+    V2H_HIP(hipStreamSynchronize(stream_));
+    constructed_ = true;
+}
+
+Batch::~Batch() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (pinned_) (void)hipHostFree(pinned_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::build_units() {
+    std::vector<int4> uA, uAT;
+    std::vector<int> iA, iAT;
+    hwgA_begin_.assign(1, 0);
+    hwgAT_begin_.assign(1, 0);
+    for (int i = 0; i < count(); i++) {
+        const Structure& S = *structs_[inst_sid_[i]];
+        uA.insert(uA.end(), S.unitsA.begin(), S.unitsA.end());
+        uAT.insert(uAT.end(), S.unitsAT.begin(), S.unitsAT.end());
+        iA.insert(iA.end(), S.unitsA.size(), i);
+        iAT.insert(iAT.end(), S.unitsAT.size(), i);
+        hwgA_begin_.push_back((int)uA.size());
+        hwgAT_begin_.push_back((int)uAT.size());
+    }
+    nwgA_ = (int)uA.size();
+    nwgAT_ = (int)uAT.size();
+    unitsA_.upload(uA);
+    unitsAT_.upload(uAT);
+    wgA_inst_.upload(iA);
+    wgAT_inst_.upload(iAT);
+    wgA_begin_.upload(hwgA_begin_);
+    wgAT_begin_.upload(hwgAT_begin_);
+}
+
+void Batch::build_projection() {
+    int gblock = 0;
+    int64_t rec = 0;
+    for (auto& c : pclass_) c.items.clear();
+    for (int i = 0; i < count(); i++) {
+        const Layout& L = structs_[inst_sid_[i]]->L;
+        inst_[i].block_first = gblock;
+        int nb = 0;
+        for (const Block& b : L.blocks) {
+            if (b.dim == 0) continue;
+            if (b.dim > 200) throw std::invalid_argument("PSD blocks larger than 200 are not supported");
+            ProjItem it{};
+            it.inst = i;
+            it.d = b.dim;
+            it.gblock = gblock++;
+            it.off = b.off;
+            it.rec_off = rec;
+            int dp = b.dim + (b.dim & 1);
+            rec += (int64_t)max_sweeps_ * (dp - 1) * (dp / 2) * 2;
+            pclass_[proj_class(b.dim)].items.push_back(it);
+            nb++;
+        }
+        inst_[i].nblocks = nb;
+    }
+    size_t lds_max = 0;
+    for (int k = 0; k < 4; k++) {
+        auto& c = pclass_[k];
+        c.nt = kClassThreads[k];
+        std::stable_sort(c.items.begin(), c.items.end(), [](const ProjItem& a, const ProjItem& b) { return a.d > b.d; });
+        c.max_d = c.items.empty() ? 0 : c.items.front().d;
+        if (!c.items.empty()) {
+            size_t lds = project_lds_bytes(c.max_d, c.nt);
+            if (lds > kLdsLimit) throw std::invalid_argument("PSD block does not fit in LDS");
+            lds_max = std::max(lds_max, lds);
+        }
+        c.d_items.upload(c.items);
+    }
+    if (lds_max > 64 * 1024) set_project_lds_limit(lds_max);
+    proj_part_.alloc(2 * (size_t)std::max(gblock, 1));
+    rec_.alloc(std::max<int64_t>(rec, 1));
+}
+
+BatchDev Batch::dev() const {
+    BatchDev B{};
+    B.structs = d_structs_.p;
+    B.inst = d_inst_.p;
+    B.st = d_state_.p;
+    B.x = x_.p; B.z = z_.p; B.c = c_.p; B.v1 = v1_.p; B.u = u_.p; B.aty = aty_.p; B.q = q_.p;
+    B.y = y_.p; B.r = r_.p; B.p = p_.p; B.w = w_.p; B.av1 = av1_.p; B.ax = ax_.p; B.aaty = aaty_.p;
+    B.partials = partials_.p;
+    B.counters = counters_.p;
+    B.proj_part = proj_part_.p;
+    B.opt = opt_;
+    return B;
+}
+
+void Batch::sync_state_to_host() {
+    V2H_HIP(hipMemcpyAsync(pinned_, d_state_.p, sizeof(InstState) * count(), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(state_.data(), pinned_, sizeof(InstState) * count());
+}
+
+void Batch::sync_state_to_device() {
+    std::memcpy(pinned_, state_.data(), sizeof(InstState) * count());
+    V2H_HIP(hipMemcpyAsync(d_state_.p, pinned_, sizeof(InstState) * count(), hipMemcpyHostToDevice, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
+}
+
+void Batch::refresh_norms(int i) {
+    const InstDev& I = inst_[i];
+    const Structure& S = *structs_[inst_sid_[i]];
+    launch_dot1(c_.p + I.poff, c_.p + I.poff, S.n_primal, scratch_.p, stream_);
+    double cc = 0.0;
+    V2H_HIP(hipMemcpyAsync(&cc, scratch_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
+    sync_state_to_host();
+    state_[i].cnorm = cc > 0 ? std::sqrt(cc) : 1.0;
+    sync_state_to_device();
+}
+
+void Batch::set_integrals(int i, const double* h, const double* g) {
+    const InstDev& I = inst_[i];
+    const Structure& S = *structs_[inst_sid_[i]];
+    const int64_t n = S.L.n;
+    V2H_HIP(hipMemcpyAsync(src_.p + I.soff, h, sizeof(double) * n * n, hipMemcpyHostToDevice, stream_));
+    V2H_HIP(hipMemcpyAsync(src_.p + I.soff + n * n, g, sizeof(double) * n * n * n * n, hipMemcpyHostToDevice, stream_));
+    launch_pack_cost(S.cost1.p, S.cost2.p, src_.p + I.soff, c_.p + I.poff, S.n_primal, stream_);
+    // v1 = c - z
+    launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);
+    refresh_norms(i);
+}
+
+void Batch::set_cost(int i, const double* c) {
+    const InstDev& I = inst_[i];
+    const Structure& S = *structs_[inst_sid_[i]];
+    V2H_HIP(hipMemcpyAsync(c_.p + I.poff, c, sizeof(double) * S.n_primal, hipMemcpyHostToDevice, stream_));
+    launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);
+    refresh_norms(i);
+}
+
+void Batch::set_guess(int i, const double* x) {
+    const InstDev& I = inst_[i];
+    const int64_t np = structs_[inst_sid_[i]]->n_primal;
+    std::vector<double> g;
+    if (!x) {  // v2rdm_solver.cc:1799-1806
+        g.resize(np);
+        srand(0);
+        for (int64_t k = 0; k < np; k++) g[k] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
+        x = g.data();
+    }
+    V2H_HIP(hipMemcpyAsync(x_.p + I.poff, x, sizeof(double) * np, hipMemcpyHostToDevice, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
+}
+
+void Batch::reset_state(int i) {
+    const InstDev& I = inst_[i];
+    const Structure& S = *structs_[inst_sid_[i]];
+    launch_fill(y_.p + I.doff, S.n_dual, 0.0, stream_);
+    launch_fill(z_.p + I.poff, S.n_primal, 0.0, stream_);
+    launch_fill(aty_.p + I.poff, S.n_primal, 0.0, stream_);
+    launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);
+    if (constructed_) sync_state_to_host();
+    InstState& st = state_[i];
+    double bn = st.bnorm, cn = st.cnorm;
+    st = InstState{};
+    st.bnorm = bn;
+    st.cnorm = cn;
+    st.mu = opt_.mu0 > 0 ? opt_.mu0 : 1.0;
+    st.eps_p = st.eps_d = 1.0;
+    st.active = 1;
+    if (constructed_) sync_state_to_device();
+    V2H_HIP(hipStreamSynchronize(stream_));
+}
+
+int Batch::solve(int maxiter, const Monitor& mon) {
+    auto t0 = std::chrono::steady_clock::now();
+    V2H_HIP(hipSetDevice(device_));
+    sync_state_to_host();
+    for (auto& st : state_) {
+        bool runnable = !st.converged && st.status == 0 && st.oiter < opt_.maxiter;
+        st.active = runnable ? 1 : 0;
+        st.maxiter_left = maxiter;
+    }
+    sync_state_to_device();
+    const BatchDev B = dev();
+    const UnitsDev UA{unitsA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_};
+    const UnitsDev UAT{unitsAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_};
+    std::vector<int64_t> seen(count());
+    for (int i = 0; i < count(); i++) seen[i] = state_[i].oiter;
+
+    int cg_est = 4;
+    for (const auto& st : state_) cg_est = std::max(cg_est, st.cg_last + 2);
+    for (int it = 0; it < maxiter; it++) {
+        launch_pre(B, UA, stream_);
+        launches_++;
+        if (mon) {
+            sync_state_to_host();
+            for (int i = 0; i < count(); i++)
+                if (state_[i].oiter != seen[i]) {
+                    seen[i] = state_[i].oiter;
+                    mon(i, status(i));
+                }
+        }
+        int steps = 0, chunk = cg_est;
+        bool any_active = true;
+        for (;;) {
+            for (int k = 0; k < chunk; k++) {
+                launch_cg_at(B, UAT, stream_);
+                launch_cg_a(B, UA, stream_);
+            }
+            launches_ += 2 * chunk;
+            steps += chunk;
+            sync_state_to_host();
+            bool pending = false;
+            any_active = false;
+            for (const auto& st : state_) {
+                any_active |= st.active != 0;
+                pending |= st.active && !st.cg_done;
+            }
+            if (!pending || steps > opt_.cg_maxiter) break;
+            chunk = 4;
+        }
+        if (!any_active) break;
+        cg_est = 4;
+        for (const auto& st : state_)
+            if (st.active) cg_est = std::max(cg_est, st.cg_last + 2);
+        launch_post(B, UAT, stream_);
+        launches_++;
+        for (auto& c : pclass_) {
+            if (c.items.empty()) continue;
+            launch_project(c.nt, B, c.d_items.p, (int)c.items.size(), c.max_d, rec_.p, max_sweeps_, stream_);
+            launches_++;
+        }
+    }
+    launch_pre(B, UA, stream_);  // book-keeping of the last iteration
+    launches_++;
+    sync_state_to_host();
+    if (mon)
+        for (int i = 0; i < count(); i++)
+            if (state_[i].oiter != seen[i]) mon(i, status(i));
+    last_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int nconv = 0;
+    for (const auto& st : state_) nconv += st.converged;
+    return nconv;
+}
+
+InstStatus Batch::status(int i) {
+    const InstState& st = state_[i];
+    InstStatus s{};
+    s.mu = st.mu;
+    s.energy_primal = st.e_primal;
+    s.energy_dual = st.e_dual;
+    s.eps_p = st.eps_p;
+    s.eps_d = st.eps_d;
+    s.oiter = st.oiter;
+    s.iiter = st.iiter;
+    s.converged = st.converged;
+    s.active = st.active;
+    s.status = st.status;
+    s.cg_last = st.cg_last;
+    return s;
+}
+
+static void d2h(double* dst, const double* src, int64_t n, hipStream_t s) {
+    V2H_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    V2H_HIP(hipStreamSynchronize(s));
+}
+static void h2d(double* dst, const double* src, int64_t n, hipStream_t s) {
+    V2H_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    V2H_HIP(hipStreamSynchronize(s));
+}
+
+void Batch::get_primal(int i, double* x) { d2h(x, x_.p + inst_[i].poff, n_primal(i), stream_); }
+void Batch::get_dual(int i, double* y) { d2h(y, y_.p + inst_[i].doff, n_dual(i), stream_); }
+void Batch::get_slack(int i, double* z) { d2h(z, z_.p + inst_[i].poff, n_primal(i), stream_); }
+void Batch::get_cost(int i, double* c) { d2h(c, c_.p + inst_[i].poff, n_primal(i), stream_); }
+void Batch::get_b(int i, double* b) const {
+    const auto& v = structs_[inst_sid_[i]]->b;
+    std::copy(v.begin(), v.end(), b);
+}
+void Batch::set_dual(int i, const double* y) { h2d(y_.p + inst_[i].doff, y, n_dual(i), stream_); }
+void Batch::set_slack(int i, const double* z) {
+    const InstDev& I = inst_[i];
+    h2d(z_.p + I.poff, z, n_primal(i), stream_);
+    // v1 = c - z
+    std::vector<double> c(n_primal(i));
+    get_cost(i, c.data());
+    for (int64_t k = 0; k < n_primal(i); k++) c[k] -= z[k];
+    h2d(v1_.p + I.poff, c.data(), n_primal(i), stream_);
+}
+double Batch::get_mu(int i) {
+    sync_state_to_host();
+    return state_[i].mu;
+}
+void Batch::set_mu(int i, double mu) {
+    sync_state_to_host();
+    state_[i].mu = mu;
+    sync_state_to_device();
+}
+
+void Batch::apply_A(int i, const double* u, double* out) {
+    const Structure& S = *structs_[inst_sid_[i]];
+    double* in = scratch_.p;
+    double* o = scratch_.p + std::max(S.n_primal, S.n_dual);
+    h2d(in, u, S.n_primal, stream_);
+    launch_fill(o, S.n_dual, 0.0, stream_);
+    launch_spmv_plain(S.dev().A, unitsA_.p, hwgA_begin_[i], hwgA_begin_[i + 1] - hwgA_begin_[i], in, o, stream_);
+    d2h(out, o, S.n_dual, stream_);
+}
+
+void Batch::apply_AT(int i, const double* y, double* out) {
+    const Structure& S = *structs_[inst_sid_[i]];
+    double* in = scratch_.p;
+    double* o = scratch_.p + std::max(S.n_primal, S.n_dual);
+    h2d(in, y, S.n_dual, stream_);
+    launch_fill(o, S.n_primal, 0.0, stream_);
+    launch_spmv_plain(S.dev().AT, unitsAT_.p, hwgAT_begin_[i], hwgAT_begin_[i + 1] - hwgAT_begin_[i], in, o,
+                      stream_);
+    d2h(out, o, S.n_primal, stream_);
+}
+
+void Batch::cost_adjoint(int i, const double* xbar, double* dh, double* dg) {
+    const Structure& S = *structs_[inst_sid_[i]];
+    const InstDev& I = inst_[i];
+    const double* xb = x_.p + I.poff;
+    if (xbar) {
+        h2d(scratch_.p, xbar, S.n_primal, stream_);
+        xb = scratch_.p;
+    }
+    double* out = scratch_.p + S.n_primal;
+    if (scratch_.n < (size_t)(S.n_primal + S.nsrc)) throw std::runtime_error("adjoint scratch too small");
+    launch_cost_adjoint(S.adj_ptr.p, S.adj_ent.p, xb, out, S.nsrc, stream_);
+    const int64_t n = S.L.n;
+    d2h(dh, out, n * n, stream_);
+    d2h(dg, out + n * n, n * n * n * n, stream_);
+}
+
+}  // namespace v2h

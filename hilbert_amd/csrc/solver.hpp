@@ -1,0 +1,148 @@
+// solver.hpp -- a batch of independent DQG v2RDM SDPs solved together on one
+// GPU by the boundary-point method.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "constraints.hpp"
+#include "layout.hpp"
+#include "sell.hpp"
+
+namespace v2h {
+
+// Device buffer owned by the solver.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    void alloc(size_t count) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = count;
+        if (count) V2H_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    void upload(const std::vector<T>& v) {
+        alloc(v.size());
+        if (!v.empty()) V2H_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    }
+};
+
+// Gather tables and maps of one problem shape, resident in HBM.
+struct Structure {
+    explicit Structure(const ProblemDesc& d);
+    Layout L;
+    int64_t n_primal, n_dual, nsrc;
+    SellHost hA, hAT;
+    std::vector<double> b;
+    std::vector<int4> unitsA, unitsAT;   // per-instance workgroup unit lists
+    DBuf<uint32_t> a_ent, a_slice, a_lstart, a_lent, at_ent, at_slice, at_lstart, at_lent;
+    DBuf<int32_t> a_lrow, at_lrow, cost1, cost2, adj_ptr, adj_ent;
+    DBuf<double> a_coef, at_coef, b_dev;
+    StructDev dev() const;
+};
+
+struct InstStatus {
+    double mu, energy_primal, energy_dual, eps_p, eps_d;
+    int64_t oiter, iiter;
+    int converged, active, status, cg_last;
+};
+
+using Monitor = std::function<void(int inst, const InstStatus&)>;
+
+class Batch {
+  public:
+    Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, int device);
+    ~Batch();
+
+    int count() const { return (int)inst_sid_.size(); }
+    const Layout& layout(int i) const { return structs_[inst_sid_[i]]->L; }
+    int64_t n_primal(int i) const { return structs_[inst_sid_[i]]->n_primal; }
+    int64_t n_dual(int i) const { return structs_[inst_sid_[i]]->n_dual; }
+    const SolverOptions& options() const { return opt_; }
+    void set_options(const SolverOptions& o) { opt_ = o; }
+
+    // Hamiltonian in the user's orbital order: h[n*n], g[n^4] chemists' (pq|rs).
+    void set_integrals(int i, const double* h, const double* g);
+    void set_cost(int i, const double* c);        // primal-layout cost vector
+    void set_guess(int i, const double* x);       // nullptr: hilbert's random guess
+    void reset_state(int i);                      // y = z = 0, mu = mu0, counters cleared
+
+    // Run up to maxiter outer iterations for every unconverged instance.
+    // Returns the number of converged instances.
+    int solve(int maxiter, const Monitor& mon = nullptr);
+
+    InstStatus status(int i);
+    void get_primal(int i, double* x);
+    void get_dual(int i, double* y);
+    void get_slack(int i, double* z);
+    void get_cost(int i, double* c);
+    void get_b(int i, double* b) const;
+    void set_dual(int i, const double* y);
+    void set_slack(int i, const double* z);
+    double get_mu(int i);
+    void set_mu(int i, double mu);
+
+    // Maps on host vectors (copy in, apply on the device, copy out).
+    void apply_A(int i, const double* u, double* out);
+    void apply_AT(int i, const double* y, double* out);
+    // dE/dsrc for src = [h; g] (user order) from a primal vector (default x).
+    void cost_adjoint(int i, const double* xbar, double* dh, double* dg);
+
+    hipStream_t stream() const { return stream_; }
+    double last_solve_seconds() const { return last_seconds_; }
+    int64_t kernel_launches() const { return launches_; }
+
+  private:
+    void build_units();
+    void build_projection();
+    void sync_state_to_host();
+    void sync_state_to_device();
+    void refresh_norms(int i);
+    BatchDev dev() const;
+
+    SolverOptions opt_;
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    std::vector<std::shared_ptr<Structure>> structs_;
+    std::vector<int> inst_sid_;
+    std::vector<InstDev> inst_;
+    std::vector<InstState> state_;      // host mirror
+    int64_t np_total_ = 0, nd_total_ = 0, ns_total_ = 0;
+
+    DBuf<StructDev> d_structs_;
+    DBuf<InstDev> d_inst_;
+    DBuf<InstState> d_state_;
+    DBuf<double> x_, z_, c_, v1_, u_, aty_, q_, y_, r_, p_, w_, av1_, ax_, aaty_;
+    DBuf<double> src_, partials_, proj_part_, rec_, scratch_;
+    DBuf<int> counters_;
+    DBuf<int4> unitsA_, unitsAT_;
+    DBuf<int> wgA_inst_, wgAT_inst_, wgA_begin_, wgAT_begin_;
+    int nwgA_ = 0, nwgAT_ = 0;
+    std::vector<int> hwgA_begin_, hwgAT_begin_;
+
+    struct ProjClass {
+        int nt = 0, max_d = 0;
+        std::vector<ProjItem> items;
+        DBuf<ProjItem> d_items;
+    };
+    ProjClass pclass_[4];
+    int max_sweeps_ = 24;
+
+    InstState* pinned_ = nullptr;
+    bool constructed_ = false;
+    double last_seconds_ = 0.0;
+    int64_t launches_ = 0;
+};
+
+}  // namespace v2h

@@ -1,0 +1,154 @@
+/*
+ * v2rdm_hip.h -- C ABI of the MI355X-native fixed-orbital v2RDM (DQG) solver.
+ *
+ * The boundary replaces, in edeprince3/hilbert:
+ *   - libsdp::BPSDPSolver (constructed at src/v2rdm_casscf/v2rdm_solver.cc:424-433,
+ *     driven by sdp_->solve(...) at v2rdm_solver.cc:1340 and the
+ *     do { solve } while (!is_converged()) loop at v2rdm_solver.cc:1330-1361),
+ *   - the callbacks it calls back into, v2RDMSolver::bpsdp_Au / bpsdp_ATu
+ *     (v2rdm_solver.cc:2995-3108, trampolines evaluate_Au / evaluate_ATu at
+ *     v2rdm_solver.cc:97-112),
+ *   - the setup they rely on: build_mapping_arrays (basis.cc:44-535),
+ *     determine_n_primal/dual and set_primal_offsets
+ *     (v2rdm_solver.cc:3294-3846), BuildConstraints (v2rdm_solver.cc:1826-2074)
+ *     and the cost-vector packing of the external-Hamiltonian constructor
+ *     (v2rdm_solver.cc:164-193, tei.cc:139-199).
+ *
+ * All functions return V2H_OK (0) or a positive error code; no exception
+ * crosses the boundary.  v2h_last_error() returns the message of the last
+ * failure on the calling thread.  Host pointers are copied in and out; all
+ * solver state lives in device memory owned by the handle.  A handle is not
+ * re-entrant: calls on one handle must not overlap.
+ */
+#ifndef V2RDM_HIP_H
+#define V2RDM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define V2H_OK 0
+#define V2H_ERR_INVALID_ARGUMENT 1
+#define V2H_ERR_HIP 2
+#define V2H_ERR_OUT_OF_MEMORY 3
+#define V2H_ERR_INTERNAL 4
+
+/* Mirrors libsdp::SDPOptions as filled at v2rdm_solver.cc:426-431 from the
+ * plugin options of plugin.cc:94-100, 217, 238-241. */
+typedef struct v2h_sdp_options {
+    double sdp_objective_convergence; /* E_CONVERGENCE, default 1e-6 */
+    double sdp_error_convergence;     /* R_CONVERGENCE, default 1e-5 */
+    double cg_convergence;            /* CG_CONVERGENCE, default 1e-9 */
+    int cg_maxiter;                   /* CG_MAXITER, default 10000 */
+    int maxiter;                      /* MAXITER: total outer iterations, default 50 */
+    int mu_update_frequency;          /* MU_UPDATE_FREQUENCY, default 1000 */
+    double mu0;                       /* initial penalty parameter, default 1 */
+} v2h_sdp_options;
+
+/* One fixed-orbital DQG problem (the v2RDMHelper external-Hamiltonian
+ * constructor, python_helpers.cc:106 / v2rdm_solver.cc:147-195, extended with
+ * per-orbital irrep labels; NULL irreps = C1 as in v2rdm_solver.cc:988). */
+typedef struct v2h_problem_desc {
+    int nmo;            /* active orbitals */
+    int nalpha, nbeta;  /* active electrons */
+    int multiplicity;   /* 2S+1; 0 = nalpha - nbeta + 1 (v2rdm_solver.cc:985-986) */
+    int nirrep;         /* 1, 2, 4 or 8 */
+    const int *orbital_irreps; /* [nmo] in the caller's orbital order, or NULL */
+    int constrain_q2, constrain_g2;   /* POSITIVITY (plugin.cc:220, v2rdm_solver.cc:4081-4104) */
+    int constrain_spin;               /* CONSTRAIN_SPIN (plugin.cc:306) */
+    int constrain_sz;                 /* CONSTRAIN_SZ (plugin.cc:309) */
+} v2h_problem_desc;
+
+typedef struct v2h_status {
+    double mu, energy_primal, energy_dual, primal_error, dual_error;
+    int64_t oiter_total, iiter_total;  /* libsdp oiter_total() / iiter_total() (v2rdm_solver.cc:1546-1547) */
+    int converged;                     /* libsdp is_converged() (v2rdm_solver.cc:1361) */
+    int active, status_bits, last_cg_iterations;
+} v2h_status;
+
+/* Progress monitor, called once per outer iteration and instance; the fields
+ * of libsdp's SDPProgressMonitorFunction (v2rdm_solver.cc:115-124) plus the
+ * instance index. */
+typedef void (*v2h_monitor_fn)(int instance, int oiter, int iiter, double energy_primal, double energy_dual,
+                               double mu, double primal_error, double dual_error, void *data);
+
+const char *v2h_last_error(void);
+int v2h_default_options(v2h_sdp_options *opt);
+int v2h_device_count(int *count);
+
+typedef struct v2h_batch v2h_batch;
+
+/* A batch of independent SDPs on one GPU; count = 1 is the single-instance
+ * replacement of new libsdp::BPSDPSolver(n_primal, n_dual, options). */
+int v2h_batch_create(const v2h_problem_desc *descs, int count, const v2h_sdp_options *opt, int device,
+                     v2h_batch **out);
+int v2h_batch_destroy(v2h_batch *b);
+int v2h_batch_count(const v2h_batch *b, int *count);
+int v2h_batch_set_options(v2h_batch *b, const v2h_sdp_options *opt);
+
+/* Sizes (determine_n_primal / determine_n_dual) and the PSD block list
+ * (dimensions_, v2rdm_solver.cc:3729-3846) of one instance. */
+int v2h_batch_sizes(const v2h_batch *b, int inst, int64_t *n_primal, int64_t *n_dual, int *n_blocks);
+int v2h_batch_block_dims(const v2h_batch *b, int inst, int *dims);
+/* Pitzer position -> caller orbital index (orbitals of one irrep contiguous). */
+int v2h_batch_orbital_order(const v2h_batch *b, int inst, int *to_user);
+
+/* Hamiltonian: h[nmo*nmo], g[nmo^4] chemists' (pq|rs) at ((p*n+q)*n+r)*n+s, in
+ * the caller's orbital order; packed on the device into c. */
+int v2h_batch_set_integrals(v2h_batch *b, int inst, const double *h, const double *g);
+int v2h_batch_set_cost(v2h_batch *b, int inst, const double *c);
+/* Primal guess; NULL = hilbert's TPDM_GUESS RANDOM (v2rdm_solver.cc:1799-1806). */
+int v2h_batch_set_guess(v2h_batch *b, int inst, const double *x);
+/* y = z = 0, mu = mu0, iteration counters cleared. */
+int v2h_batch_reset(v2h_batch *b, int inst);
+
+/* Up to maxiter outer iterations for every unconverged instance (the
+ * maxiter argument of SDPSolver::solve).  y, z and mu persist across calls. */
+int v2h_batch_solve(v2h_batch *b, int maxiter, v2h_monitor_fn mon, void *data, int *n_converged);
+int v2h_batch_status(v2h_batch *b, int inst, v2h_status *st);
+double v2h_batch_last_solve_seconds(const v2h_batch *b);
+
+/* Vectors of one instance, primal layout (v2rdm_solver.cc:3729-3846). */
+int v2h_batch_get_x(v2h_batch *b, int inst, double *x);
+int v2h_batch_get_y(v2h_batch *b, int inst, double *y);   /* libsdp get_y (checkpoint.cc:58-64) */
+int v2h_batch_get_z(v2h_batch *b, int inst, double *z);   /* libsdp get_z */
+int v2h_batch_set_y(v2h_batch *b, int inst, const double *y);  /* libsdp set_y (checkpoint.cc:106-120) */
+int v2h_batch_set_z(v2h_batch *b, int inst, const double *z);  /* libsdp set_z */
+int v2h_batch_get_mu(v2h_batch *b, int inst, double *mu);      /* libsdp get_mu */
+int v2h_batch_set_mu(v2h_batch *b, int inst, double mu);       /* libsdp set_mu */
+int v2h_batch_get_c(v2h_batch *b, int inst, double *c);
+int v2h_batch_get_b(v2h_batch *b, int inst, double *bvec);
+
+/* The constraint maps (bpsdp_Au / bpsdp_ATu, v2rdm_solver.cc:2995-3108). */
+int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
+int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
+
+/* Adjoint of the cost packing: dE/dh[nmo*nmo], dE/dg[nmo^4] (caller order)
+ * for E = c . xbar; xbar = NULL uses the current primal solution x. */
+int v2h_batch_cost_adjoint(v2h_batch *b, int inst, const double *xbar, double *dh, double *dg);
+
+/* ---- host-only inspection of one problem structure (no GPU needed) ----
+ * The tables the device path is built from: geminal tables
+ * (build_mapping_arrays, basis.cc:427-533), the constraint matrix A with b in
+ * CSR (the rows of bpsdp_Au, v2rdm_solver.cc:2995-3050, and BuildConstraints,
+ * v2rdm_solver.cc:1826-2074), and the cost-vector gather map
+ * (v2rdm_solver.cc:164-193).  Orbital indices are Pitzer positions. */
+typedef struct v2h_structure v2h_structure;
+int v2h_structure_create(const v2h_problem_desc *desc, v2h_structure **out);
+int v2h_structure_destroy(v2h_structure *s);
+int v2h_structure_sizes(const v2h_structure *s, int64_t *n_primal, int64_t *n_dual, int64_t *nnz, int *n_blocks);
+int v2h_structure_block_dims(const v2h_structure *s, int *dims);
+/* gems_ab[nirrep], gems_aa[nirrep], bas_ab[2 sum gems_ab], bas_aa[2 sum gems_aa] */
+int v2h_structure_tables(const v2h_structure *s, int *gems_ab, int *gems_aa, int *bas_ab, int *bas_aa);
+int v2h_structure_constraints(const v2h_structure *s, int64_t *row_ptr, int32_t *col, double *val, double *b);
+int v2h_structure_cost_map(const v2h_structure *s, int32_t *i1, int32_t *i2);
+/* SELL-64 statistics: [A slices, A entries, A long rows, A long entries,
+ * A^T slices, A^T entries, A^T long rows, A^T long entries] */
+int v2h_structure_gather_stats(const v2h_structure *s, int64_t *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
