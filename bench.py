@@ -1,0 +1,204 @@
+"""Benchmark: converged v2RDM SDPs per second at (16e,16o), batched on one MI355X.
+
+A step is one batched solve, from hilbert's random guess to convergence
+(E_CONVERGENCE 1e-6, R_CONVERGENCE 1e-5), of B independent (16e,16o) DQG SDPs
+with C2v irrep labels drawn at random and synthetic molecular-like integrals
+(naux = 48), every instance with its own seed.  Integrals and tables are
+resident in HBM before the timed region; the timed region is reset + solve.
+
+Printed (one JSON line): value = converged SDPs/s of the batch; the single
+instance rate and the CPU oracle rate are reported beside it; `roofline` is
+the dominant kernel's achieved rate over its launches that processed the whole
+batch, timed with HIP events on the library's stream inside the timed region.
+
+Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
+E_CONV, R_CONV = 1e-6, 1e-5
+
+
+def bytes_per_instance(kind, n_p, n_d):
+    """Algorithmic HBM bytes one launch moves for one instance."""
+    return 8 * {"cg_at": n_p + 3 * n_d,        # write q; read r, w, p
+                "cg_a": n_p + 8 * n_d,         # read q; read+write y, r, p, w
+                "pre": 3 * n_p + 5 * n_d,      # read c-z, x, A^T y; write 3; read b, y
+                "post": 4 * n_p + n_d}[kind]   # read y, x, c; write A^T y, U
+
+
+def project_flops(dims):
+    """Nominal FP64 flops of one PSD projection sweep: (16/3) d^3 per block."""
+    return sum(16.0 / 3.0 * float(d) ** 3 for d in dims if d > 0)
+
+
+def make_batch(items, maxiter):
+    from hilbert_amd.batch import Batch, options
+    B = Batch([p for p, _, _ in items],
+              options(e_convergence=E_CONV, r_convergence=R_CONV, maxiter=maxiter))
+    for i, (_, h, g) in enumerate(items):
+        B.set_integrals(i, h, g)
+    return B
+
+
+def reset_all(B):
+    for i in range(B.count):
+        B.set_guess(i, None)
+        B.reset(i)
+
+
+def roofline(B, kt_full):
+    """Dominant kernel (largest time over whole-batch launches) -> roofline dict."""
+    n_p, n_d = B.n_primal, B.n_dual
+    proj_kinds = [k for k in kt_full if k.startswith("project")]
+    totals = {k: kt_full[k][0] for k in ("pre", "cg_at", "cg_a", "post")}
+    totals["project"] = sum(kt_full[k][0] for k in proj_kinds)
+    dom = max(totals, key=totals.get)
+    if dom == "project":
+        # one launch per class per iteration; an iteration projects every block once
+        iters = max(kt_full[k][1] for k in proj_kinds)
+        per_iter_ms = totals["project"] / max(iters, 1)
+        flops = sum(project_flops(B.block_dims(i)) for i in range(B.count))
+        achieved = flops / (per_iter_ms * 1e-3) / 1e12
+        return dom, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_project (all size classes, one outer iteration)",
+                     "per_launch_ms": per_iter_ms, "work_per_launch": flops}
+    ms, cnt = kt_full[dom]
+    per_launch_ms = ms / max(cnt, 1)
+    nbytes = sum(bytes_per_instance(dom, n_p[i], n_d[i]) for i in range(B.count))
+    achieved = nbytes / (per_launch_ms * 1e-3) / 1e9
+    return dom, {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_" + dom,
+                 "per_launch_ms": per_launch_ms, "work_per_launch": nbytes}
+
+
+def cpu_baseline(item, mean_oiter, seconds):
+    """Oracle BPSDP (C + OpenMP + LAPACK) on one instance for a bounded number
+    of outer iterations; converted to SDPs/s with the instance's iteration count."""
+    from oracle.pyoracle import OracleProblem
+    prob, h, g = item
+    P = OracleProblem(prob.nmo, prob.nalpha, prob.nbeta, prob.irreps, prob.nirrep)
+    c = P.pack_c(h, g)
+    cores = min(os.cpu_count() or 1, 16)
+    # first chunk sizes the sample to about `seconds` of CPU work
+    t0 = time.time()
+    P.solve(c, maxiter=5, nthreads=cores, e_convergence=E_CONV, r_convergence=R_CONV)
+    per_it = (time.time() - t0) / 5
+    n = max(5, int(seconds / max(per_it, 1e-6)))
+    t0 = time.time()
+    r = P.solve(c, maxiter=n, nthreads=cores, e_convergence=E_CONV, r_convergence=R_CONV)
+    dt = time.time() - t0
+    per_it = dt / max(r["oiter"], 1)
+    return {"value": 1.0 / (per_it * mean_oiter), "unit": "SDPs/s", "cores": cores, "kind": "port",
+            "sample": f"oracle BPSDP on instance 0 for its first {r['oiter']} outer iterations "
+                      f"({r['iiter']} CG steps, {dt:.1f} s, {cores} OpenMP threads); SDPs/s = 1 / "
+                      f"(s per outer iteration x {mean_oiter:.0f} outer iterations per SDP)",
+            "seconds_per_outer_iteration": per_it}
+
+
+def host_info():
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        model = "unknown"
+    return {"nproc": os.cpu_count(), "cpu": model}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--maxiter", type=int, default=20000)
+    ap.add_argument("--single", type=int, default=1, help="also time one instance alone (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU oracle sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
+              "communication, and multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    from hilbert_amd import _native, configs
+    if _native.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    items = configs.c2v_16(args.batch)
+    B = make_batch(items, args.maxiter)
+    B.set_profiling(True)
+    for _ in range(args.warmup):
+        reset_all(B)
+        B.solve_to_convergence()
+    B.reset_kernel_times()
+
+    step_times, conv, oiters = [], [], []
+    for _ in range(args.steps):
+        reset_all(B)
+        t0 = time.perf_counter()
+        nconv = B.solve_to_convergence()
+        step_times.append(time.perf_counter() - t0)
+        conv.append(nconv)
+        oiters.append([B.status(i)["oiter"] for i in range(B.count)])
+    kt = B.kernel_times()
+    kt_full = B.kernel_times(full=True)
+    total = sum(step_times)
+    value = sum(conv) / total
+    mean_oiter = sum(sum(o) for o in oiters) / max(1, sum(len(o) for o in oiters))
+    st = [B.status(i) for i in range(B.count)]
+    dom, roof = roofline(B, kt_full)
+    B.close()
+
+    single = None
+    if args.single:
+        S = make_batch(items[:1], args.maxiter)
+        reset_all(S)
+        t0 = time.perf_counter()
+        S.solve_to_convergence()
+        dt = time.perf_counter() - t0
+        single = {"value": 1.0 / dt, "unit": "SDPs/s", "seconds": dt, "oiter": S.status(0)["oiter"]}
+        S.close()
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(items[0], mean_oiter, args.cpu_seconds)
+
+    out = {
+        "metric": "converged v2RDM SDPs/s (ΔE≤1e-6 Eh), batched vs single GPU vs CPU, (16e,16o)",
+        "value": value, "unit": "SDPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.batch} x (16e,16o) DQG SDPs, C2v labels at random, naux 48, "
+                               "one batched solve to convergence per step",
+                   "batch": args.batch, "nmo": 16, "electrons": 16, "nirrep": 4,
+                   "e_convergence": E_CONV, "r_convergence": R_CONV, "positivity": "DQG",
+                   "constrain_spin": True, "constrain_sz": True},
+        "converged": sum(conv), "attempted": args.batch * args.steps,
+        "mean_outer_iterations": mean_oiter,
+        "mean_cg_per_outer": sum(s["iiter"] for s in st) / max(1, sum(s["oiter"] for s in st)),
+        "single_gpu": single,
+        "batched_over_single": (value / single["value"]) if single else None,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        "roofline": roof,
+        "kernel_ms": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
+        "kernel_launches": {k: v[1] for k, v in kt.items() if v[1]},
+        "host": host_info(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

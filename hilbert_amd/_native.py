@@ -81,6 +81,10 @@ SIGNATURES = {
     "v2h_batch_Au": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_ATu": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_cost_adjoint": (C.c_int, [_vp, C.c_int, _vp, _dp, _dp]),
+    "v2h_batch_set_profiling": (C.c_int, [_vp, C.c_int]),
+    "v2h_batch_kernel_times": (C.c_int, [_vp, _dp, _lp, _dp, _lp]),
+    "v2h_batch_reset_kernel_times": (C.c_int, [_vp]),
+    "v2h_batch_launch_stats": (C.c_int, [_vp, _lp]),
     "v2h_structure_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(_vp)]),
     "v2h_structure_destroy": (C.c_int, [_vp]),
     "v2h_structure_sizes": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),

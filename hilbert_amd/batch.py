@@ -118,6 +118,31 @@ class Batch:
     def last_solve_seconds(self) -> float:
         return N.lib().v2h_batch_last_solve_seconds(self._h)
 
+    KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project64", "project256", "project512", "project1024")
+
+    def set_profiling(self, on=True):
+        N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))
+
+    def kernel_times(self, full=False) -> dict:
+        """{kind: (total ms, launches)}; full=True: only launches that
+        processed the whole batch (every instance running)."""
+        ms = np.zeros(8)
+        cnt = np.zeros(8, dtype=np.int64)
+        fms = np.zeros(8)
+        fcnt = np.zeros(8, dtype=np.int64)
+        N.check(N.lib().v2h_batch_kernel_times(self._h, ms, cnt, fms, fcnt))
+        if full:
+            ms, cnt = fms, fcnt
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.KERNEL_KINDS)}
+
+    def reset_kernel_times(self):
+        N.check(N.lib().v2h_batch_reset_kernel_times(self._h))
+
+    def launch_stats(self) -> dict:
+        st = np.zeros(4, dtype=np.int64)
+        N.check(N.lib().v2h_batch_launch_stats(self._h, st))
+        return dict(zip(("launches", "a_workgroups", "at_workgroups", "blocks"), st.tolist()))
+
     def status(self, i) -> dict:
         s = N.Status()
         N.check(N.lib().v2h_batch_status(self._h, i, C.byref(s)))

@@ -300,6 +300,25 @@ int v2h_batch_ATu(v2h_batch* b, int inst, const double* y, double* out) {
     });
 }
 
+int v2h_batch_set_profiling(v2h_batch* b, int on) {
+    return guarded([&] { B(b).set_profiling(on != 0); });
+}
+
+int v2h_batch_kernel_times(v2h_batch* b, double* ms, int64_t* counts, double* full_ms, int64_t* full_counts) {
+    return guarded([&] { B(b).kernel_times(ms, counts, full_ms, full_counts); });
+}
+
+int v2h_batch_reset_kernel_times(v2h_batch* b) {
+    return guarded([&] { B(b).reset_kernel_times(); });
+}
+
+int v2h_batch_launch_stats(v2h_batch* b, int64_t* st) {
+    return guarded([&] {
+        check_ptr(st);
+        B(b).launch_stats(st);
+    });
+}
+
 int v2h_batch_cost_adjoint(v2h_batch* b, int inst, const double* xbar, double* dh, double* dg) {
     return guarded([&] {
         check_inst(B(b), inst);

@@ -86,7 +86,6 @@ struct BatchDev {
     double *y, *r, *p, *w, *av1, *ax, *aaty;
     // reductions
     double* partials;       // per-workgroup partial dot products
-    int* counters;          // per-instance arrival counters (per kernel kind)
     double* proj_part;      // per-block projection partials (c.x, dual residual)
     SolverOptions opt;
 };
@@ -97,12 +96,14 @@ struct UnitsDev {
     const int* wg_inst;     // instance of each workgroup
     const int* wg_begin;    // [ninst + 1] first workgroup of each instance
     int nwg;
+    int ninst;
 };
 
 struct ProjItem {
     int inst, d, gblock, pad;
     int64_t off;            // primal offset inside the instance
     int64_t rec_off;        // offset of the rotation record (doubles)
+    int64_t mat_off;        // offset of the packed matrix when it lives in global memory
 };
 
 }  // namespace v2h

@@ -1,19 +1,20 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the batched BPSDP solve.
 //
 // * Gather kernels over SELL-64 tables (one wavefront per 64 rows, or per
-//   long row).  Every CG step is two launches:
-//     k_cg_at : q = A^T p, with p = (r - alpha w) + beta p formed on the fly
-//     k_cg_a  : w = A q, own-row updates of y, r, p, and the CG dot products
-//   The dots are reduced deterministically: every workgroup writes its
-//   partial sums, the last workgroup of an instance to arrive (atomic
-//   counter) reduces them in a fixed order and advances the CG state.
-// * k_pre  : A [v1, x, A^T y] in one pass (rhs pieces, A x) and the end-of-
-//   iteration bookkeeping (errors, energies, mu, convergence, CG threshold).
+//   long row).  Every CG step is three launches:
+//     k_cg_at     : q = A^T p, with p = (r - alpha w) + beta p formed on the fly
+//     k_cg_a      : w = A q, own-row updates of y, r, p, per-workgroup dot partials
+//     k_cg_reduce : one workgroup per instance sums the partials in a fixed
+//                   order (deterministic) and advances the CG state
+// * k_pre (+ k_pre_reduce): A [v1, x, A^T y] in one pass (rhs pieces, A x),
+//   then the end-of-iteration bookkeeping (errors, energies, mu, convergence,
+//   CG threshold).
 // * k_post : A^T y and U = mu x + A^T y - c.
-// * k_project<NT> : PSD projection of one block per workgroup by parallel
-//   cyclic Jacobi on the packed upper triangle in LDS; the rotations are
-//   recorded and replayed in reverse on diag(max(L,0)) to form U+ without an
-//   explicit eigenvector matrix.
+// * k_project<NT, GLOBAL> : PSD projection of one block per workgroup by
+//   parallel cyclic Jacobi on the packed upper triangle (in LDS, or in global
+//   memory for blocks too large for LDS); the rotations are recorded and
+//   replayed in reverse on diag(max(L,0)) to form U+ without an explicit
+//   eigenvector matrix.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -121,30 +122,22 @@ __device__ __forceinline__ void block_partials(const double (&v)[N], double* red
     }
 }
 
-// Last-arriving workgroup of an instance reduces all partials of that
-// instance (fixed order) into tot[N].  Returns true in that workgroup.
+// Sum the nwg partial records (N doubles each) of one instance in a fixed
+// order; every thread of the workgroup receives tot[N].  Runs in its own
+// launch after the producing kernel: the kernel boundary makes every XCD's
+// writes visible without device-scope fences inside the producer.
 template <int N>
-__device__ bool last_block_reduce(int* counter, int nwg, const double* parts, double* red, double (&tot)[N]) {
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        int prev = atomicAdd(counter, 1);
-        last = (prev == nwg - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
+__device__ void reduce_partials(const double* parts, int nwg, double* red, double (&tot)[N]) {
     double s[N];
 #pragma unroll
     for (int i = 0; i < N; i++) s[i] = 0.0;
     for (int k = threadIdx.x; k < nwg; k += blockDim.x) {
 #pragma unroll
-        for (int i = 0; i < N; i++) s[i] += __builtin_nontemporal_load(parts + (size_t)k * N + i);
+        for (int i = 0; i < N; i++) s[i] += parts[(size_t)k * N + i];
     }
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < N; i++) s[i] = wave_sum(s[i]);
-    __syncthreads();
     if (lane == 0) {
 #pragma unroll
         for (int i = 0; i < N; i++) red[wv * N + i] = s[i];
@@ -157,8 +150,6 @@ __device__ bool last_block_reduce(int* counter, int nwg, const double* parts, do
         for (int q = 0; q < nw; q++) t += red[q * N + i];
         tot[i] = t;
     }
-    if (threadIdx.x == 0) *counter = 0;
-    return true;
 }
 
 __device__ __forceinline__ void load_coef(double* dst, const double* src) {
@@ -248,12 +239,19 @@ __global__ __launch_bounds__(kWG) void k_cg_a(BatchDev B, UnitsDev U) {
         d4[2] = rn * acc;
         d4[3] = acc * acc;
     }
-    double* parts = B.partials + (size_t)wg * 4;
-    block_partials<4>(d4, red, parts);
-    const int nwg = U.wg_begin[inst + 1] - U.wg_begin[inst];
+    block_partials<4>(d4, red, B.partials + (size_t)wg * 4);
+}
+
+// CG step, part 3 (one workgroup per instance): reduce the dots of k_cg_a and
+// advance the CG state.
+__global__ __launch_bounds__(kWG) void k_cg_reduce(BatchDev B, UnitsDev U) {
+    const int inst = blockIdx.x;
+    InstState& st = B.st[inst];
+    if (!st.active || st.cg_done) return;
+    __shared__ double red[4 * 4];
+    const int s = st.cg_iter;
     double tot[4];
-    if (!last_block_reduce<4>(B.counters + inst, nwg, B.partials + (size_t)U.wg_begin[inst] * 4, red, tot))
-        return;
+    reduce_partials<4>(B.partials + (size_t)U.wg_begin[inst] * 4, U.wg_begin[inst + 1] - U.wg_begin[inst], red, tot);
     if (threadIdx.x == 0) {
         st.rr = tot[0];
         st.pw = tot[1];
@@ -318,12 +316,19 @@ __global__ __launch_bounds__(kWG) void k_pre(BatchDev B, UnitsDev U) {
         d4[2] = e * e;
         d4[3] = bv * B.y[g];
     }
-    double* parts = B.partials + (size_t)wg * 4;
-    block_partials<4>(d4, red, parts);
-    const int nwg = U.wg_begin[inst + 1] - U.wg_begin[inst];
+    block_partials<4>(d4, red, B.partials + (size_t)wg * 4);
+}
+
+// Iteration start, part 2 (one workgroup per instance): book-keeping of the
+// finished iteration (errors, energies, mu, convergence) and CG set-up.
+__global__ __launch_bounds__(kWG) void k_pre_reduce(BatchDev B, UnitsDev U) {
+    const int inst = blockIdx.x;
+    InstState& st = B.st[inst];
+    if (!st.active) return;
+    const InstDev I = B.inst[inst];
+    __shared__ double red[4 * 4];
     double tot[4];
-    if (!last_block_reduce<4>(B.counters + inst, nwg, B.partials + (size_t)U.wg_begin[inst] * 4, red, tot))
-        return;
+    reduce_partials<4>(B.partials + (size_t)U.wg_begin[inst] * 4, U.wg_begin[inst + 1] - U.wg_begin[inst], red, tot);
     if (threadIdx.x == 0) {
         const SolverOptions& o = B.opt;
         if (st.pending) {
@@ -503,8 +508,10 @@ __device__ void apply_round(double* A, const int* pp, const int* qq, const doubl
     }
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void k_project(BatchDev B, const ProjItem* items, double* rec_base, int max_sweeps) {
+// GLOBAL: the packed matrix lives in global memory (blocks too large for LDS).
+template <int NT, bool GLOBAL>
+__global__ __launch_bounds__(NT) void k_project(BatchDev B, const ProjItem* items, double* rec_base, int max_sweeps,
+                                                double* gmat) {
     extern __shared__ double smem[];
     const ProjItem it = items[blockIdx.x];
     InstState& st = B.st[it.inst];
@@ -512,8 +519,8 @@ __global__ __launch_bounds__(NT) void k_project(BatchDev B, const ProjItem* item
     const InstDev I = B.inst[it.inst];
     const int d = it.d, dp = d + (d & 1), m = dp >> 1;
     const int np = dp * (dp + 1) / 2;
-    double* A = smem;
-    double* rc = A + np;
+    double* A = GLOBAL ? gmat + it.mat_off : smem;
+    double* rc = GLOBAL ? smem : A + np;
     double* rs = rc + m;
     double* red = rs + m;
     int* pp = (int*)(red + NT / 64);
@@ -694,10 +701,14 @@ void launch_cg_at(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
     if (U.nwg > 0) hipLaunchKernelGGL(k_cg_at, dim3(U.nwg), dim3(kWG), 0, s, B, U);
 }
 void launch_cg_a(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
-    if (U.nwg > 0) hipLaunchKernelGGL(k_cg_a, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    if (U.nwg <= 0) return;
+    hipLaunchKernelGGL(k_cg_a, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(U.ninst), dim3(kWG), 0, s, B, U);
 }
 void launch_pre(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
-    if (U.nwg > 0) hipLaunchKernelGGL(k_pre, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    if (U.nwg <= 0) return;
+    hipLaunchKernelGGL(k_pre, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    hipLaunchKernelGGL(k_pre_reduce, dim3(U.ninst), dim3(kWG), 0, s, B, U);
 }
 void launch_post(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
     if (U.nwg > 0) hipLaunchKernelGGL(k_post, dim3(U.nwg), dim3(kWG), 0, s, B, U);
@@ -707,27 +718,33 @@ void launch_spmv_plain(const SellDev& S, const int4* units, int wg0, int nwg, co
     if (nwg > 0) hipLaunchKernelGGL(k_spmv_plain, dim3(nwg), dim3(kWG), 0, s, S, units, wg0, in, out);
 }
 
-size_t project_lds_bytes(int d, int nt) {
+size_t project_lds_bytes(int d, int nt, bool global_matrix) {
     int dp = d + (d & 1), m = dp / 2;
-    return sizeof(double) * ((size_t)dp * (dp + 1) / 2 + 2 * (size_t)m + nt / 64) + sizeof(int) * 2 * (size_t)m;
+    size_t mat = global_matrix ? 0 : (size_t)dp * (dp + 1) / 2;
+    return sizeof(double) * (mat + 2 * (size_t)m + nt / 64) + sizeof(int) * 2 * (size_t)m;
 }
 
-void launch_project(int nt, const BatchDev& B, const ProjItem* items, int nitems, int max_d, double* rec,
-                    int max_sweeps, hipStream_t s) {
+void launch_project(int nt, bool global_matrix, const BatchDev& B, const ProjItem* items, int nitems, int max_d,
+                    double* rec, int max_sweeps, double* gmat, hipStream_t s) {
     if (nitems <= 0) return;
-    size_t lds = project_lds_bytes(max_d, nt);
+    size_t lds = project_lds_bytes(max_d, nt, global_matrix);
+    if (global_matrix) {
+        hipLaunchKernelGGL((k_project<1024, true>), dim3(nitems), dim3(1024), lds, s, B, items, rec, max_sweeps, gmat);
+        return;
+    }
     switch (nt) {
         case 64:
-            hipLaunchKernelGGL(k_project<64>, dim3(nitems), dim3(64), lds, s, B, items, rec, max_sweeps);
+            hipLaunchKernelGGL((k_project<64, false>), dim3(nitems), dim3(64), lds, s, B, items, rec, max_sweeps, gmat);
             break;
         case 256:
-            hipLaunchKernelGGL(k_project<256>, dim3(nitems), dim3(256), lds, s, B, items, rec, max_sweeps);
+            hipLaunchKernelGGL((k_project<256, false>), dim3(nitems), dim3(256), lds, s, B, items, rec, max_sweeps, gmat);
             break;
         case 512:
-            hipLaunchKernelGGL(k_project<512>, dim3(nitems), dim3(512), lds, s, B, items, rec, max_sweeps);
+            hipLaunchKernelGGL((k_project<512, false>), dim3(nitems), dim3(512), lds, s, B, items, rec, max_sweeps, gmat);
             break;
         default:
-            hipLaunchKernelGGL(k_project<1024>, dim3(nitems), dim3(1024), lds, s, B, items, rec, max_sweeps);
+            hipLaunchKernelGGL((k_project<1024, false>), dim3(nitems), dim3(1024), lds, s, B, items, rec, max_sweeps,
+                               gmat);
             break;
     }
 }
@@ -758,9 +775,13 @@ void launch_dot1(const double* a, const double* b, int64_t n, double* out, hipSt
 
 namespace v2h {
 void set_project_lds_limit(size_t bytes) {
-    V2H_HIP(hipFuncSetAttribute((const void*)k_project<64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    V2H_HIP(hipFuncSetAttribute((const void*)k_project<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    V2H_HIP(hipFuncSetAttribute((const void*)k_project<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    V2H_HIP(hipFuncSetAttribute((const void*)k_project<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    V2H_HIP(hipFuncSetAttribute((const void*)k_project<64, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+    V2H_HIP(hipFuncSetAttribute((const void*)k_project<256, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+    V2H_HIP(hipFuncSetAttribute((const void*)k_project<512, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+    V2H_HIP(hipFuncSetAttribute((const void*)k_project<1024, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
 }
 }  // namespace v2h

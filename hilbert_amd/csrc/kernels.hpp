@@ -16,10 +16,11 @@ void launch_post(const BatchDev& B, const UnitsDev& U, hipStream_t s);
 void launch_spmv_plain(const SellDev& S, const int4* units, int wg0, int nwg, const double* in, double* out,
                        hipStream_t s);
 
-// LDS bytes of the Jacobi projection for block size d and nt threads.
-size_t project_lds_bytes(int d, int nt);
-void launch_project(int nt, const BatchDev& B, const ProjItem* items, int nitems, int max_d, double* rec,
-                    int max_sweeps, hipStream_t s);
+// LDS bytes of the Jacobi projection for block size d and nt threads (the
+// packed matrix is in LDS unless global_matrix).
+size_t project_lds_bytes(int d, int nt, bool global_matrix);
+void launch_project(int nt, bool global_matrix, const BatchDev& B, const ProjItem* items, int nitems, int max_d,
+                    double* rec, int max_sweeps, double* gmat, hipStream_t s);
 void set_project_lds_limit(size_t bytes);
 
 void launch_fill(double* p, int64_t n, double v, hipStream_t s);

@@ -34,13 +34,17 @@ std::vector<int4> unit_list(const SellHost& S) {
     return out;
 }
 
+// Projection classes by block size: threads per workgroup, and whether the
+// packed matrix fits in the 160 KB of LDS (d <= 200) or lives in global memory.
+constexpr int kProjClasses = 5, kGlobalClass = 4;
+constexpr int kClassThreads[kProjClasses] = {64, 256, 512, 1024, 1024};
 int proj_class(int d) {
     if (d <= 16) return 0;
     if (d <= 64) return 1;
     if (d <= 140) return 2;
-    return 3;
+    if (d <= 200) return 3;
+    return kGlobalClass;
 }
-constexpr int kClassThreads[4] = {64, 256, 512, 1024};
 }  // namespace
 
 // ------------------------------------------------------------------ Structure
@@ -146,8 +150,6 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     for (auto* b : {&y_, &r_, &p_, &w_, &av1_, &ax_, &aaty_}) b->alloc(nd_total_);
     src_.alloc(ns_total_);
     partials_.alloc(4 * (size_t)std::max(nwgA_, nwgAT_) + 4);
-    counters_.alloc(N);
-    V2H_HIP(hipMemsetAsync(counters_.p, 0, sizeof(int) * N, stream_));
     int64_t smax = 0;
     for (const auto& s : structs_) smax = std::max({smax, s->n_primal, s->n_dual, s->nsrc});
     scratch_.alloc(2 * smax);
@@ -180,6 +182,11 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
 
 Batch::~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto& t : timers_)
+        for (auto& e : t.pool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
     if (pinned_) (void)hipHostFree(pinned_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -210,7 +217,7 @@ void Batch::build_units() {
 
 void Batch::build_projection() {
     int gblock = 0;
-    int64_t rec = 0;
+    int64_t rec = 0, mat = 0;
     for (auto& c : pclass_) c.items.clear();
     for (int i = 0; i < count(); i++) {
         const Layout& L = structs_[inst_sid_[i]]->L;
@@ -218,7 +225,6 @@ void Batch::build_projection() {
         int nb = 0;
         for (const Block& b : L.blocks) {
             if (b.dim == 0) continue;
-            if (b.dim > 200) throw std::invalid_argument("PSD blocks larger than 200 are not supported");
             ProjItem it{};
             it.inst = i;
             it.d = b.dim;
@@ -227,19 +233,25 @@ void Batch::build_projection() {
             it.rec_off = rec;
             int dp = b.dim + (b.dim & 1);
             rec += (int64_t)max_sweeps_ * (dp - 1) * (dp / 2) * 2;
-            pclass_[proj_class(b.dim)].items.push_back(it);
+            int k = proj_class(b.dim);
+            if (k == kGlobalClass) {
+                it.mat_off = mat;
+                mat += (int64_t)dp * (dp + 1) / 2;
+            }
+            pclass_[k].items.push_back(it);
             nb++;
         }
         inst_[i].nblocks = nb;
     }
     size_t lds_max = 0;
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < kProjClasses; k++) {
         auto& c = pclass_[k];
         c.nt = kClassThreads[k];
+        c.global = k == kGlobalClass;
         std::stable_sort(c.items.begin(), c.items.end(), [](const ProjItem& a, const ProjItem& b) { return a.d > b.d; });
         c.max_d = c.items.empty() ? 0 : c.items.front().d;
         if (!c.items.empty()) {
-            size_t lds = project_lds_bytes(c.max_d, c.nt);
+            size_t lds = project_lds_bytes(c.max_d, c.nt, c.global);
             if (lds > kLdsLimit) throw std::invalid_argument("PSD block does not fit in LDS");
             lds_max = std::max(lds_max, lds);
         }
@@ -248,6 +260,7 @@ void Batch::build_projection() {
     if (lds_max > 64 * 1024) set_project_lds_limit(lds_max);
     proj_part_.alloc(2 * (size_t)std::max(gblock, 1));
     rec_.alloc(std::max<int64_t>(rec, 1));
+    gmat_.alloc(std::max<int64_t>(mat, 1));
 }
 
 BatchDev Batch::dev() const {
@@ -258,7 +271,6 @@ BatchDev Batch::dev() const {
     B.x = x_.p; B.z = z_.p; B.c = c_.p; B.v1 = v1_.p; B.u = u_.p; B.aty = aty_.p; B.q = q_.p;
     B.y = y_.p; B.r = r_.p; B.p = p_.p; B.w = w_.p; B.av1 = av1_.p; B.ax = ax_.p; B.aaty = aaty_.p;
     B.partials = partials_.p;
-    B.counters = counters_.p;
     B.proj_part = proj_part_.p;
     B.opt = opt_;
     return B;
@@ -268,6 +280,65 @@ void Batch::sync_state_to_host() {
     V2H_HIP(hipMemcpyAsync(pinned_, d_state_.p, sizeof(InstState) * count(), hipMemcpyDeviceToHost, stream_));
     V2H_HIP(hipStreamSynchronize(stream_));
     std::memcpy(state_.data(), pinned_, sizeof(InstState) * count());
+    tflush();
+}
+
+void Batch::tstart(int k) {
+    if (!profile_) return;
+    KTimer& t = timers_[k];
+    if (t.used == t.pool.size()) {
+        hipEvent_t a, b;
+        V2H_HIP(hipEventCreate(&a));
+        V2H_HIP(hipEventCreate(&b));
+        t.pool.push_back({a, b});
+        t.full.push_back(0);
+    }
+    t.full[t.used] = (k == 1 || k == 2) ? all_iterating_ : all_running_;
+    V2H_HIP(hipEventRecord(t.pool[t.used].first, stream_));
+}
+
+void Batch::tstop(int k) {
+    if (!profile_) return;
+    KTimer& t = timers_[k];
+    V2H_HIP(hipEventRecord(t.pool[t.used].second, stream_));
+    t.used++;
+}
+
+// Called after a stream synchronisation: all recorded events are complete.
+void Batch::tflush() {
+    for (auto& t : timers_) {
+        for (size_t i = 0; i < t.used; i++) {
+            float ms = 0.f;
+            V2H_HIP(hipEventElapsedTime(&ms, t.pool[i].first, t.pool[i].second));
+            t.ms += ms;
+            t.count++;
+            if (t.full[i]) {
+                t.full_ms += ms;
+                t.full_count++;
+            }
+        }
+        t.used = 0;
+    }
+}
+
+void Batch::kernel_times(double* ms, int64_t* counts, double* full_ms, int64_t* full_counts) {
+    V2H_HIP(hipStreamSynchronize(stream_));
+    tflush();
+    for (int k = 0; k < kKinds; k++) {
+        if (ms) ms[k] = timers_[k].ms;
+        if (counts) counts[k] = timers_[k].count;
+        if (full_ms) full_ms[k] = timers_[k].full_ms;
+        if (full_counts) full_counts[k] = timers_[k].full_count;
+    }
+}
+
+void Batch::reset_kernel_times() {
+    V2H_HIP(hipStreamSynchronize(stream_));
+    tflush();
+    for (auto& t : timers_) {
+        t.ms = t.full_ms = 0.0;
+        t.count = t.full_count = 0;
+    }
 }
 
 void Batch::sync_state_to_device() {
@@ -353,15 +424,26 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     }
     sync_state_to_device();
     const BatchDev B = dev();
-    const UnitsDev UA{unitsA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_};
-    const UnitsDev UAT{unitsAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_};
+    const UnitsDev UA{unitsA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_, count()};
+    const UnitsDev UAT{unitsAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_, count()};
     std::vector<int64_t> seen(count());
     for (int i = 0; i < count(); i++) seen[i] = state_[i].oiter;
 
     int cg_est = 4;
     for (const auto& st : state_) cg_est = std::max(cg_est, st.cg_last + 2);
+    auto refresh_flags = [&] {
+        all_running_ = all_iterating_ = true;
+        for (const auto& st : state_) {
+            all_running_ &= st.active != 0;
+            all_iterating_ &= st.active && !st.cg_done;
+        }
+    };
+    refresh_flags();
+    all_iterating_ = all_running_;
     for (int it = 0; it < maxiter; it++) {
+        tstart(0);
         launch_pre(B, UA, stream_);
+        tstop(0);
         launches_++;
         if (mon) {
             sync_state_to_host();
@@ -375,12 +457,17 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         bool any_active = true;
         for (;;) {
             for (int k = 0; k < chunk; k++) {
+                tstart(1);
                 launch_cg_at(B, UAT, stream_);
+                tstop(1);
+                tstart(2);
                 launch_cg_a(B, UA, stream_);
+                tstop(2);
             }
             launches_ += 2 * chunk;
             steps += chunk;
             sync_state_to_host();
+            refresh_flags();
             bool pending = false;
             any_active = false;
             for (const auto& st : state_) {
@@ -394,16 +481,28 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         cg_est = 4;
         for (const auto& st : state_)
             if (st.active) cg_est = std::max(cg_est, st.cg_last + 2);
+        all_iterating_ = all_running_;  // the next CG solve starts for every running instance
+        tstart(3);
         launch_post(B, UAT, stream_);
+        tstop(3);
         launches_++;
-        for (auto& c : pclass_) {
+        for (int k = 0; k < kProjClasses; k++) {
+            auto& c = pclass_[k];
             if (c.items.empty()) continue;
-            launch_project(c.nt, B, c.d_items.p, (int)c.items.size(), c.max_d, rec_.p, max_sweeps_, stream_);
+            int kind = 4 + (k < 4 ? k : 3);
+            tstart(kind);
+            launch_project(c.nt, c.global, B, c.d_items.p, (int)c.items.size(), c.max_d, rec_.p, max_sweeps_,
+                           gmat_.p, stream_);
+            tstop(kind);
             launches_++;
         }
+        V2H_HIP(hipGetLastError());
     }
+    tstart(0);
     launch_pre(B, UA, stream_);  // book-keeping of the last iteration
+    tstop(0);
     launches_++;
+    V2H_HIP(hipGetLastError());
     sync_state_to_host();
     if (mon)
         for (int i = 0; i < count(); i++)

@@ -100,6 +100,24 @@ class Batch {
     void cost_adjoint(int i, const double* xbar, double* dh, double* dg);
 
     hipStream_t stream() const { return stream_; }
+
+    // HIP-event timing of every kernel launch of solve(), per kernel kind:
+    // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..7 projection classes (64/256/512/1024 threads).
+    static constexpr int kKinds = 8;
+    void set_profiling(bool on) { profile_ = on; }
+    // ms/counts: all launches; full_ms/full_counts: launches made while every
+    // instance of the batch was active (and, for CG kernels, still iterating)
+    // at the last host poll, whose work is therefore the whole batch's.
+    void kernel_times(double* ms, int64_t* counts, double* full_ms, int64_t* full_counts);
+    void reset_kernel_times();
+    void launch_stats(int64_t* st) const {
+        int64_t nblk = 0;
+        for (const auto& c : pclass_) nblk += (int64_t)c.items.size();
+        st[0] = launches_;
+        st[1] = nwgA_;
+        st[2] = nwgAT_;
+        st[3] = nblk;
+    }
     double last_solve_seconds() const { return last_seconds_; }
     int64_t kernel_launches() const { return launches_; }
 
@@ -125,7 +143,6 @@ class Batch {
     DBuf<InstState> d_state_;
     DBuf<double> x_, z_, c_, v1_, u_, aty_, q_, y_, r_, p_, w_, av1_, ax_, aaty_;
     DBuf<double> src_, partials_, proj_part_, rec_, scratch_;
-    DBuf<int> counters_;
     DBuf<int4> unitsA_, unitsAT_;
     DBuf<int> wgA_inst_, wgAT_inst_, wgA_begin_, wgAT_begin_;
     int nwgA_ = 0, nwgAT_ = 0;
@@ -133,11 +150,27 @@ class Batch {
 
     struct ProjClass {
         int nt = 0, max_d = 0;
+        bool global = false;
         std::vector<ProjItem> items;
         DBuf<ProjItem> d_items;
     };
-    ProjClass pclass_[4];
+    ProjClass pclass_[5];
+    DBuf<double> gmat_;
     int max_sweeps_ = 24;
+
+    struct KTimer {
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+        std::vector<char> full;
+        size_t used = 0;
+        double ms = 0.0, full_ms = 0.0;
+        int64_t count = 0, full_count = 0;
+    };
+    bool all_running_ = false, all_iterating_ = false;
+    KTimer timers_[kKinds];
+    bool profile_ = false;
+    void tstart(int k);
+    void tstop(int k);
+    void tflush();
 
     InstState* pinned_ = nullptr;
     bool constructed_ = false;

@@ -125,6 +125,20 @@ int v2h_batch_get_b(v2h_batch *b, int inst, double *bvec);
 int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
 
+/* HIP-event timing of the kernels launched by v2h_batch_solve, recorded on
+ * the library's stream.  ms[8] and counts[8] per kernel kind: 0 iteration
+ * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..7 PSD
+ * projection classes of 64 / 256 / 512 / 1024 threads.  Cumulative since the
+ * last reset.  full_ms / full_counts (may be NULL) cover only the launches
+ * made while every instance was running (for CG kernels: still iterating)
+ * at the previous host poll, i.e. launches that processed the whole batch. */
+int v2h_batch_set_profiling(v2h_batch *b, int on);
+int v2h_batch_kernel_times(v2h_batch *b, double *ms, int64_t *counts, double *full_ms, int64_t *full_counts);
+int v2h_batch_reset_kernel_times(v2h_batch *b);
+/* Launch statistics: [kernel launches so far, A gather workgroups,
+ * A^T gather workgroups, projection blocks] */
+int v2h_batch_launch_stats(v2h_batch *b, int64_t *stats);
+
 /* Adjoint of the cost packing: dE/dh[nmo*nmo], dE/dg[nmo^4] (caller order)
  * for E = c . xbar; xbar = NULL uses the current primal solution x. */
 int v2h_batch_cost_adjoint(v2h_batch *b, int inst, const double *xbar, double *dh, double *dg);
