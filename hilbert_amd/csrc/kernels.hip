@@ -508,6 +508,19 @@ __device__ void apply_round(double* A, const int* pp, const int* qq, const doubl
     }
 }
 
+// Workgroup barrier that orders LDS only: outstanding global stores (the
+// rotation records) and loads (the prefetched rotations) stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+template <bool GLOBAL>
+__device__ __forceinline__ void round_sync() {
+    if constexpr (GLOBAL) __syncthreads();
+    else lds_barrier();
+}
+
 // GLOBAL: the packed matrix lives in global memory (blocks too large for LDS).
 template <int NT, bool GLOBAL>
 __global__ __launch_bounds__(NT) void k_project(BatchDev B, const ProjItem* items, double* rec_base, int max_sweeps,
@@ -554,36 +567,40 @@ __global__ __launch_bounds__(NT) void k_project(BatchDev B, const ProjItem* item
         }
         off = block_sum<NT>(off, red);
         if (off <= thr) break;
+        // one pair per thread (m <= NT is guaranteed by the host)
+        const int k = threadIdx.x;
         for (int t = 0; t < dp - 1; t++) {
-            for (int k = threadIdx.x; k < m; k += NT) {
+            if (k < m) {
                 int p, q;
                 round_pair(t, k, dp, p, q);
                 pp[k] = p;
                 qq[k] = q;
                 double apq = A[pk(p, q)];
-                double c = 1.0, s = 0.0, tn = 0.0;
+                double c = 1.0, s = 0.0;
                 if (apq != 0.0) {
                     double app = A[pk(p, p)], aqq = A[pk(q, q)];
                     double th = (aqq - app) / (2.0 * apq);
-                    tn = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                    double tn = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
                     c = 1.0 / sqrt(tn * tn + 1.0);
                     s = tn * c;
                 }
                 rc[k] = c;
                 rs[k] = s;
+                // recorded for the replay; the LDS-only barrier below does not
+                // wait for these stores
                 size_t ro = ((size_t)nsweep * (dp - 1) + t) * m + k;
                 rec[2 * ro] = c;
                 rec[2 * ro + 1] = s;
             }
-            __syncthreads();
+            round_sync<GLOBAL>();
             apply_round<NT>(A, pp, qq, rc, rs, m, 1.0, true);
-            __syncthreads();
+            round_sync<GLOBAL>();
         }
     }
     if (nsweep == max_sweeps && threadIdx.x == 0) atomicOr(&st.status, (int)kStatusJacobiNotConverged);
 
     // U+ = J diag(max(L, 0)) J^T by replaying the rounds backwards
-    __syncthreads();
+    __syncthreads();  // also completes the rotation-record stores
     for (int e = threadIdx.x; e < np; e += NT) {
         int j = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
         while ((j + 1) * (j + 2) / 2 <= e) j++;
@@ -593,21 +610,36 @@ __global__ __launch_bounds__(NT) void k_project(BatchDev B, const ProjItem* item
         A[e] = (i == j && v > 0.0) ? v : 0.0;
     }
     __syncthreads();
-    for (int sw = nsweep - 1; sw >= 0; sw--)
-        for (int t = dp - 2; t >= 0; t--) {
-            for (int k = threadIdx.x; k < m; k += NT) {
+    {
+        // round idx = sweep * (dp - 1) + t, replayed from last to first; the
+        // next round's rotation is fetched while the current one is applied
+        const int k = threadIdx.x;
+        const int nround = nsweep * (dp - 1);
+        double nc = 1.0, ns = 0.0;
+        if (nround > 0 && k < m) {
+            size_t ro = (size_t)(nround - 1) * m + k;
+            nc = rec[2 * ro];
+            ns = rec[2 * ro + 1];
+        }
+        for (int idx = nround - 1; idx >= 0; idx--) {
+            if (k < m) {
                 int p, q;
-                round_pair(t, k, dp, p, q);
+                round_pair(idx % (dp - 1), k, dp, p, q);
                 pp[k] = p;
                 qq[k] = q;
-                size_t ro = ((size_t)sw * (dp - 1) + t) * m + k;
-                rc[k] = rec[2 * ro];
-                rs[k] = rec[2 * ro + 1];
+                rc[k] = nc;
+                rs[k] = ns;
+                if (idx > 0) {
+                    size_t ro = (size_t)(idx - 1) * m + k;
+                    nc = rec[2 * ro];
+                    ns = rec[2 * ro + 1];
+                }
             }
-            __syncthreads();
+            round_sync<GLOBAL>();
             apply_round<NT>(A, pp, qq, rc, rs, m, -1.0, false);
-            __syncthreads();
+            round_sync<GLOBAL>();
         }
+    }
 
     // x = U+ / mu, z = U+ - sym(U), v1 = c - z; partial c.x and dual residual
     double ex = 0.0, dr = 0.0;

@@ -225,6 +225,9 @@ void Batch::build_projection() {
         int nb = 0;
         for (const Block& b : L.blocks) {
             if (b.dim == 0) continue;
+            // the projection kernels give each rotation pair one thread
+            if ((b.dim + 1) / 2 > kClassThreads[proj_class(b.dim)])
+                throw std::invalid_argument("PSD block larger than 2048 is not supported");
             ProjItem it{};
             it.inst = i;
             it.d = b.dim;
