@@ -78,6 +78,12 @@ SIGNATURES = {
     "v2h_batch_set_mu": (C.c_int, [_vp, C.c_int, C.c_double]),
     "v2h_batch_get_c": (C.c_int, [_vp, C.c_int, _dp]),
     "v2h_batch_get_b": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_get_opdm": (C.c_int, [_vp, C.c_int, C.c_char_p, _dp]),
+    "v2h_batch_get_tpdm": (C.c_int, [_vp, C.c_int, _dp]),
+    "v2h_batch_get_opdm_sparse": (C.c_int, [_vp, C.c_int, C.c_char_p, _vp, _vp, C.c_int64,
+                                            C.POINTER(C.c_int64)]),
+    "v2h_batch_get_tpdm_sparse": (C.c_int, [_vp, C.c_int, C.c_char_p, _vp, _vp, C.c_int64,
+                                            C.POINTER(C.c_int64)]),
     "v2h_batch_Au": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_ATu": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_cost_adjoint": (C.c_int, [_vp, C.c_int, _vp, _dp, _dp]),
@@ -94,6 +100,8 @@ SIGNATURES = {
     "v2h_structure_constraints": (C.c_int, [_vp, _lp, _ip, _dp, _dp]),
     "v2h_structure_cost_map": (C.c_int, [_vp, _ip, _ip]),
     "v2h_structure_gather_stats": (C.c_int, [_vp, _lp]),
+    "v2h_structure_opdm": (C.c_int, [_vp, _dp, C.c_char_p, _dp]),
+    "v2h_structure_tpdm": (C.c_int, [_vp, _dp, _dp]),
 }
 
 _lib = None

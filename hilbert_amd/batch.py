@@ -118,7 +118,8 @@ class Batch:
     def last_solve_seconds(self) -> float:
         return N.lib().v2h_batch_last_solve_seconds(self._h)
 
-    KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project64", "project256", "project512", "project1024")
+    KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project_jacobi16", "project_tri64", "project_tri128",
+                    "project_tri192", "project_jacobi200", "project_jacobi_big")
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))
@@ -126,10 +127,11 @@ class Batch:
     def kernel_times(self, full=False) -> dict:
         """{kind: (total ms, launches)}; full=True: only launches that
         processed the whole batch (every instance running)."""
-        ms = np.zeros(8)
-        cnt = np.zeros(8, dtype=np.int64)
-        fms = np.zeros(8)
-        fcnt = np.zeros(8, dtype=np.int64)
+        n = len(self.KERNEL_KINDS)
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int64)
+        fms = np.zeros(n)
+        fcnt = np.zeros(n, dtype=np.int64)
         N.check(N.lib().v2h_batch_kernel_times(self._h, ms, cnt, fms, fcnt))
         if full:
             ms, cnt = fms, fcnt
@@ -185,6 +187,39 @@ class Batch:
 
     def set_mu(self, i, mu):
         N.check(N.lib().v2h_batch_set_mu(self._h, i, float(mu)))
+
+    # ---- reduced density matrices (orbital indices in the caller's order) ------
+    def opdm(self, i, type="SUM"):
+        """Dense n x n 1-RDM of instance i: "A", "B" or "SUM"."""
+        n = self.problems[i].nmo
+        out = np.zeros((n, n))
+        N.check(N.lib().v2h_batch_get_opdm(self._h, i, type.encode(), out.reshape(-1)))
+        return out
+
+    def tpdm(self, i):
+        """Dense spin-free 2-RDM, shape (n, n, n, n), element [i, k, j, l] =
+        D(ij, kl) as in hilbert's get_tpdm matrix [i n + k][j n + l]."""
+        n = self.problems[i].nmo
+        out = np.zeros(n ** 4)
+        N.check(N.lib().v2h_batch_get_tpdm(self._h, i, out))
+        return out.reshape(n, n, n, n)
+
+    def _sparse(self, fn, i, type, width):
+        cnt = C.c_int64()
+        N.check(fn(self._h, i, type.encode(), None, None, 0, C.byref(cnt)))
+        idx = np.zeros((cnt.value, width), dtype=np.int32)
+        val = np.zeros(cnt.value)
+        N.check(fn(self._h, i, type.encode(), idx.ctypes.data_as(C.c_void_p), val.ctypes.data_as(C.c_void_p),
+                   cnt.value, C.byref(cnt)))
+        return idx, val
+
+    def opdm_sparse(self, i, type):
+        """(indices [m, 2], values [m]) in the order of hilbert's get_opdm_sparse."""
+        return self._sparse(N.lib().v2h_batch_get_opdm_sparse, i, type, 2)
+
+    def tpdm_sparse(self, i, type):
+        """(indices [m, 4], values [m]) in the order of hilbert's get_tpdm_sparse."""
+        return self._sparse(N.lib().v2h_batch_get_tpdm_sparse, i, type, 4)
 
     def Au(self, i, u):
         out = np.zeros(self.n_dual[i])

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/v2rdm_hip.h"
+#include "rdm.hpp"
 #include "solver.hpp"
 
 struct v2h_batch {
@@ -250,6 +251,62 @@ V2H_VEC_GETTER(v2h_batch_get_c, get_cost)
 V2H_VEC_GETTER(v2h_batch_get_b, get_b)
 #undef V2H_VEC_GETTER
 
+int v2h_batch_get_opdm(v2h_batch* b, int inst, const char* type, double* out) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        if (!type || !out) throw std::invalid_argument("null argument");
+        std::vector<double> x(B(b).n_primal(inst));
+        B(b).get_primal(inst, x.data());
+        v2h::opdm_dense(B(b).layout(inst), x.data(), type, out);
+    });
+}
+
+int v2h_batch_get_tpdm(v2h_batch* b, int inst, double* out) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        if (!out) throw std::invalid_argument("null argument");
+        std::vector<double> x(B(b).n_primal(inst));
+        B(b).get_primal(inst, x.data());
+        v2h::tpdm_dense(B(b).layout(inst), x.data(), out);
+    });
+}
+
+int v2h_batch_get_opdm_sparse(v2h_batch* b, int inst, const char* type, int32_t* idx, double* val, int64_t cap,
+                              int64_t* count) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        if (!type || !count || cap < 0 || (cap > 0 && (!idx || !val))) throw std::invalid_argument("bad argument");
+        std::vector<double> x(B(b).n_primal(inst));
+        B(b).get_primal(inst, x.data());
+        auto e = v2h::opdm_sparse(B(b).layout(inst), x.data(), type);
+        *count = (int64_t)e.size();
+        for (int64_t k = 0; k < cap && k < (int64_t)e.size(); k++) {
+            idx[2 * k] = e[k].i;
+            idx[2 * k + 1] = e[k].j;
+            val[k] = e[k].value;
+        }
+    });
+}
+
+int v2h_batch_get_tpdm_sparse(v2h_batch* b, int inst, const char* type, int32_t* idx, double* val, int64_t cap,
+                              int64_t* count) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        if (!type || !count || cap < 0 || (cap > 0 && (!idx || !val))) throw std::invalid_argument("bad argument");
+        std::vector<double> x(B(b).n_primal(inst));
+        B(b).get_primal(inst, x.data());
+        auto e = v2h::tpdm_sparse(B(b).layout(inst), x.data(), type);
+        *count = (int64_t)e.size();
+        for (int64_t k = 0; k < cap && k < (int64_t)e.size(); k++) {
+            idx[4 * k] = e[k].i;
+            idx[4 * k + 1] = e[k].j;
+            idx[4 * k + 2] = e[k].k;
+            idx[4 * k + 3] = e[k].l;
+            val[k] = e[k].value;
+        }
+    });
+}
+
 int v2h_batch_set_y(v2h_batch* b, int inst, const double* y) {
     return guarded([&] {
         check_inst(B(b), inst);
@@ -445,6 +502,23 @@ int v2h_structure_gather_stats(const v2h_structure* s, int64_t* st) {
         st[5] = (int64_t)S.sAT.ent.size();
         st[6] = S.sAT.nlong();
         st[7] = (int64_t)S.sAT.long_ent.size();
+    });
+}
+
+int v2h_structure_opdm(const v2h_structure* s, const double* x, const char* type, double* out) {
+    return guarded([&] {
+        check_ptr(x);
+        check_ptr(type);
+        check_ptr(out);
+        v2h::opdm_dense(Sref(s).L, x, type, out);
+    });
+}
+
+int v2h_structure_tpdm(const v2h_structure* s, const double* x, double* out) {
+    return guarded([&] {
+        check_ptr(x);
+        check_ptr(out);
+        v2h::tpdm_dense(Sref(s).L, x, out);
     });
 }
 

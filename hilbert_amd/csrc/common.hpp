@@ -104,6 +104,8 @@ struct ProjItem {
     int64_t off;            // primal offset inside the instance
     int64_t rec_off;        // offset of the rotation record (doubles)
     int64_t mat_off;        // offset of the packed matrix when it lives in global memory
+    int64_t work_off;       // tridiagonal path: offset of the block's work area (7 d^2 doubles)
+    int64_t lam_off;        // tridiagonal path: offset of the block's eigenvalues (d + 1 doubles)
 };
 
 }  // namespace v2h

@@ -1,0 +1,748 @@
+// eig_tri.hip -- PSD projection of one symmetric block per workgroup through
+// a tridiagonal eigendecomposition (gfx950).
+//
+//   U  = Q T Q^T        Householder reduction; the matrix is held in registers,
+//                       2-D cyclic over the workgroup (thread (tr, tc) owns rows
+//                       tr + PR r and columns tc + 32 c), one reflector per step
+//   T  = S L S^T        eigenvalues by Sturm counts with bracketed Newton steps
+//                       (warm-started from the previous call's eigenvalues of the
+//                       same block), eigenvectors by two steps of inverse
+//                       iteration, orthogonalised within clusters of close
+//                       eigenvalues
+//   T+ = S L+ S^T       staged through LDS into the register matrix (or
+//                       T - S L- S^T when there are fewer non-positive eigenvalues)
+//   U+ = Q T+ Q^T       the reflectors applied two-sidedly in reverse order
+//
+// followed by the BPSDP update of the block: x = U+ / mu, z = U+ - sym(U),
+// v1 = c - z and the block's partial c.x and dual-residual sums.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace v2h {
+
+namespace {
+
+constexpr double kEps = DBL_EPSILON;
+// Consecutive eigenvalues closer than kClusterRel |lambda| + kClusterAbs ||T||
+// share a cluster; its eigenvectors are orthogonalised against each other.
+constexpr double kClusterRel = 1e-5;
+constexpr double kClusterAbs = 1e-11;
+constexpr int kStage = 16;       // eigenvectors per LDS chunk when forming T+
+constexpr int kMaxPasses = 160;  // Sturm passes per eigenvalue (bisection needs about 60)
+constexpr int kChunk = 8;        // inverse-iteration loads issued ahead of their use
+
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Compiler-only fence between the rows of a register tile: keeps the
+// scheduler from hoisting every row's memory operations at once, which would
+// exceed the register budget.
+__device__ __forceinline__ void row_fence() { asm volatile("" ::: "memory"); }
+
+// ---- cross-lane exchange of doubles (DPP within rows of 16 lanes, ds_swizzle
+// within 32).  Every lane of the wavefront must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int PATTERN>
+__device__ __forceinline__ double swz(double v) {
+    const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), PATTERN);
+    const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), PATTERN);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+constexpr int kQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kRowHalfMirror = 0x141;
+constexpr int kRowMirror = 0x140;
+constexpr int kRowRor8 = 0x128;      // rotate by 8 in a row of 16 = lane xor 8
+constexpr int kSwzXor4 = (4 << 10) | 0x1F;
+constexpr int kSwzXor16 = (16 << 10) | 0x1F;
+
+// Sum over the 64 lanes; every lane receives the same total.
+__device__ __forceinline__ double wsum(double v) {
+    v += dpp<kQuadXor1>(v);
+    v += dpp<kQuadXor2>(v);
+    v += dpp<kRowHalfMirror>(v);
+    v += dpp<kRowMirror>(v);
+    v += swz<kSwzXor16>(v);
+    return lane_value(v, 0) + lane_value(v, 32);
+}
+__device__ __forceinline__ double wmax(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wmin(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wsum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; q++) t += red[q];
+    return t;
+}
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* red) {
+    v = wmax(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = red[0];
+#pragma unroll
+    for (int q = 1; q < NT / 64; q++) t = fmax(t, red[q]);
+    return t;
+}
+template <int NT>
+__device__ __forceinline__ double block_min(double v, double* red) {
+    v = wmin(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = red[0];
+#pragma unroll
+    for (int q = 1; q < NT / 64; q++) t = fmin(t, red[q]);
+    return t;
+}
+
+// Sums v[0..7] over the 32 lanes of a half-wavefront with 9 exchanges
+// (reduce-scatter).  Lane L ends up with the total of entry
+// 4 b4 + 2 b3 + b2 (b = bits of L); the four lanes differing in bits 0-1 agree.
+__device__ __forceinline__ double reduce_scatter8(double (&v)[8]) {
+    const int lane = threadIdx.x & 63;
+    {
+        const bool hi = lane & 16;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double send = hi ? v[k] : v[k + 4];
+            double keep = hi ? v[k + 4] : v[k];
+            v[k] = keep + swz<kSwzXor16>(send);
+        }
+    }
+    {
+        const bool hi = lane & 8;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            double send = hi ? v[k] : v[k + 2];
+            double keep = hi ? v[k + 2] : v[k];
+            v[k] = keep + dpp<kRowRor8>(send);
+        }
+    }
+    {
+        const bool hi = lane & 4;
+        double send = hi ? v[0] : v[1];
+        double keep = hi ? v[1] : v[0];
+        v[0] = keep + swz<kSwzXor4>(send);
+    }
+    v[0] += dpp<kQuadXor2>(v[0]);
+    v[0] += dpp<kQuadXor1>(v[0]);
+    return v[0];
+}
+
+// 1/q to about 1 ulp: hardware reciprocal and one Newton step.
+__device__ __forceinline__ double recip(double q) {
+    const double r = __builtin_amdgcn_rcp(q);
+    const double e = fma(-q, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Deterministic start vectors for inverse iteration, uniform in [-1, 1).
+__device__ __forceinline__ double start_value(uint32_t l, uint32_t i) {
+    uint32_t h = (l * 0x9E3779B1u) ^ ((i + 0x7F4A7C15u) * 0x85EBCA77u);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (double)(h & 0xFFFFFFu) * (1.0 / 8388608.0) - 1.0;
+}
+
+// Sturm count of T - s I (eigenvalues below s) and S = d/ds log|det(T - s I)|
+// = sum_j 1 / (s - lambda_j), from the pivots q_i of the LDL^T recurrence.
+__device__ int sturm(double s, int d, const double* dd, const double* e2, double pivmin, double& S) {
+    double q = dd[0] - s;
+    if (fabs(q) < pivmin) q = -pivmin;
+    double r = recip(q);
+    int c = q < 0.0;
+    double dq = -1.0;
+    double acc = dq * r;
+    for (int i = 1; i < d; i++) {
+        const double e = e2[i - 1];
+        double qn = fma(-e, r, dd[i] - s);
+        const double dqn = fma(e * dq, r * r, -1.0);
+        if (fabs(qn) < pivmin) qn = -pivmin;
+        r = recip(qn);
+        dq = dqn;
+        c += qn < 0.0;
+        acc = fma(dq, r, acc);
+    }
+    S = acc;
+    return c;
+}
+
+// Inverse-iteration helpers on one column x (stride d) of the work area, with
+// the LU factors of T - sg I in F (5 planes of stride d^2, same column).
+struct Column {
+    double* x;
+    double* F;
+    size_t ld, plane;  // d, d^2
+    __device__ double& X(int i) const { return x[(size_t)i * ld]; }
+    __device__ double& f(int q, int i) const { return F[q * plane + (size_t)i * ld]; }
+};
+
+// x <- L^{-1} P (scale x); loads run kChunk rows ahead of the recurrence.
+__device__ void apply_pl(const Column& c, int d, double scale) {
+    double bc = c.X(0) * scale;
+    for (int i0 = 0; i0 + 1 < d; i0 += kChunk) {
+        double xn[kChunk], f0[kChunk], sw[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) {
+            const int i = min(i0 + u, d - 2);
+            xn[u] = c.X(i + 1);
+            f0[u] = c.f(0, i);
+            sw[u] = c.f(4, i);
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) {
+            const int i = i0 + u;
+            if (i + 1 < d) {
+                const double bn = xn[u] * scale;
+                const bool s = sw[u] != 0.0;
+                c.X(i) = s ? bn : bc;
+                bc = s ? fma(-f0[u], bn, bc) : fma(-f0[u], bc, bn);
+            }
+        }
+    }
+    c.X(d - 1) = bc;
+}
+
+// x <- U^{-1} x; returns ||x||^2.
+__device__ double apply_u(const Column& c, int d) {
+    double x1 = 0.0, x2 = 0.0, s2 = 0.0;
+    for (int i0 = d - 1; i0 >= 0; i0 -= kChunk) {
+        double xb[kChunk], f1[kChunk], f2[kChunk], f3[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) {
+            const int i = max(i0 - u, 0);
+            xb[u] = c.X(i);
+            f1[u] = c.f(1, i);
+            f2[u] = c.f(2, i);
+            f3[u] = c.f(3, i);
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) {
+            const int i = i0 - u;
+            if (i >= 0) {
+                const double xi = fma(-f3[u], x2, fma(-f2[u], x1, xb[u])) * f1[u];
+                c.X(i) = xi;
+                s2 = fma(xi, xi, s2);
+                x2 = x1;
+                x1 = xi;
+            }
+        }
+    }
+    return s2;
+}
+
+// x <- x - sum_m (y_m . x / ||y_m||^2) y_m over the cluster's earlier columns
+// (twice); returns the new ||x||^2.
+__device__ double orthogonalise(const Column& c, int d, const double* Sv, int l0, int l, const double* xs) {
+    for (int pass = 0; pass < 2; pass++)
+        for (int m = l0; m < l; m++) {
+            const double* y = Sv + m;
+            double dot = 0.0;
+            for (int i = 0; i < d; i++) dot = fma(y[(size_t)i * d], c.X(i), dot);
+            const double f = dot / xs[m];
+            for (int i = 0; i < d; i++) c.X(i) = fma(-f, y[(size_t)i * d], c.X(i));
+        }
+    double s2 = 0.0;
+    for (int i = 0; i < d; i++) s2 = fma(c.X(i), c.X(i), s2);
+    return s2;
+}
+
+template <int NT, int RI, int CJ>
+__global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* items, double* work, double* lamst) {
+    constexpr int PR = NT / 32;
+    constexpr int DMAX = PR * RI;
+    static_assert(RI <= 8, "reduce_scatter8 handles at most 8 rows per thread");
+    static_assert(DMAX <= NT, "one eigenvalue per thread");
+    static_assert(PR * RI == 32 * CJ, "square register tiling");
+    __shared__ double colb[2][DMAX];
+    __shared__ double pbuf[DMAX];
+    __shared__ double dd[DMAX], ee[DMAX], e2[DMAX], tau[DMAX], lamv[DMAX], xs[DMAX];
+    __shared__ double stage[kStage][DMAX];
+    __shared__ double red[NT / 64];
+
+    const ProjItem it = items[blockIdx.x];
+    InstState& st = B.st[it.inst];
+    if (!st.active) return;
+    const InstDev I = B.inst[it.inst];
+    const int d = it.d;
+    const int t = threadIdx.x, tr = t >> 5, tc = t & 31, lane = t & 63;
+    const int rr = ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);  // reduce_scatter8 entry
+    const double mu = st.mu;
+    const double* U = B.u + I.poff + it.off;
+    double* hv = work + it.work_off;        // d x d, row k = reflector k
+    double* Sv = hv + (size_t)d * d;        // d x d, column l = eigenvector l (unnormalised)
+    double* fac = Sv + (size_t)d * d;       // 5 x d x d LU factors, [q][i][l]
+    double* lprev = lamst + it.lam_off;     // d eigenvalues + last largest change
+
+    for (int i = t; i < 2 * DMAX; i += NT) (&colb[0][0])[i] = 0.0;
+    double a[RI][CJ];
+#pragma unroll
+    for (int r = 0; r < RI; r++) {
+        const int i = tr + PR * r;
+#pragma unroll
+        for (int c = 0; c < CJ; c++) {
+            const int j = tc + 32 * c;
+            const int ic = min(i, d - 1), jc = min(j, d - 1);
+            const double s = 0.5 * (U[(size_t)ic * d + jc] + U[(size_t)jc * d + ic]);
+            a[r][c] = (i < d && j < d) ? s : 0.0;
+        }
+        row_fence();
+    }
+    __syncthreads();
+
+    // column k of the register matrix (below the diagonal) to colb[buf], diagonal to dd[k]
+    auto put_col = [&](int k, int buf) {
+        if (tc == (k & 31)) {
+            const int ck = k >> 5;
+#pragma unroll
+            for (int r = 0; r < RI; r++) {
+                const int i = tr + PR * r;
+                double v = 0.0;
+#pragma unroll
+                for (int c = 0; c < CJ; c++)
+                    if (c == ck) v = a[r][c];
+                if (i < d) {
+                    if (i > k) colb[buf][i] = v;
+                    else if (i == k) dd[k] = v;
+                }
+            }
+        }
+    };
+    // pbuf[i] = tk (A v)_i for every row i < DMAX
+    auto matvec = [&](const double (&vc)[CJ], double tk) {
+        double s8[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            double s = 0.0;
+            if (r < RI) {
+#pragma unroll
+                for (int c = 0; c < CJ; c++) s = fma(a[r][c], vc[c], s);
+            }
+            s8[r] = s;
+        }
+        const double ps = reduce_scatter8(s8);
+        if ((lane & 3) == 0 && rr < RI) pbuf[tr + PR * rr] = tk * ps;
+    };
+    // A -= v w^T + w v^T, w = p - kk v, over rows/columns i with lo < i < d:
+    // v_i = vbuf[i] * vscale except v_one = 1; p in pbuf.
+    auto rank2 = [&](const double* vbuf, const double (&vc)[CJ], double vscale, int lo, int one, double kk) {
+        double wc[CJ];
+#pragma unroll
+        for (int c = 0; c < CJ; c++) {
+            const int j = tc + 32 * c;
+            const double pj = pbuf[j];
+            wc[c] = (j > lo && j < d) ? fma(-kk, vc[c], pj) : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < RI; r++) {
+            const int i = tr + PR * r;
+            const double vraw = vbuf[i], pi = pbuf[i];
+            const bool in = i > lo && i < d;
+            const double vi = in ? (i == one ? 1.0 : vraw * vscale) : 0.0;
+            const double wi = in ? fma(-kk, vi, pi) : 0.0;
+#pragma unroll
+            for (int c = 0; c < CJ; c++) a[r][c] = fma(-wi, vc[c], fma(-vi, wc[c], a[r][c]));
+            row_fence();
+        }
+    };
+
+    // ---------------------------------------------------------------- U = Q T Q^T
+    put_col(0, 0);
+    for (int k = 0; k + 2 < d; k++) {
+        lds_sync();
+        const double* col = colb[k & 1];
+        double sg = 0.0;
+        for (int i = k + 2 + lane; i < DMAX; i += 64) sg = fma(col[i], col[i], sg);  // zero beyond d
+        sg = wsum(sg);
+        const double alpha = col[k + 1];
+        double tk = 0.0, sc = 0.0, beta = alpha;
+        if (sg != 0.0) {
+            beta = -copysign(sqrt(fma(alpha, alpha, sg)), alpha);
+            tk = (beta - alpha) / beta;
+            sc = 1.0 / (alpha - beta);
+        }
+        if (t == 0) {
+            ee[k] = beta;
+            tau[k] = tk;
+        }
+        if (tk != 0.0) {  // uniform: every wavefront computed the same sums
+            for (int i = k + 1 + t; i < d; i += NT) hv[(size_t)k * d + i] = (i == k + 1) ? 1.0 : col[i] * sc;
+            double vc[CJ];
+#pragma unroll
+            for (int c = 0; c < CJ; c++) {
+                const int j = tc + 32 * c;
+                const double cj = col[j];
+                vc[c] = (j == k + 1) ? 1.0 : ((j > k + 1 && j < d) ? cj * sc : 0.0);
+            }
+            matvec(vc, tk);
+            lds_sync();
+            double kk = 0.0;
+            for (int i = k + 1 + lane; i < d; i += 64) kk = fma((i == k + 1) ? 1.0 : col[i] * sc, pbuf[i], kk);
+            kk = 0.5 * tk * wsum(kk);
+            rank2(col, vc, sc, k, k + 1, kk);
+        }
+        put_col(k + 1, (k + 1) & 1);
+    }
+    if (d >= 2) put_col(d - 1, 0);  // writes dd[d-1] only
+    __syncthreads();                // also completes the reflector stores
+    if (t == 0 && d >= 2) ee[d - 2] = colb[(d - 2) & 1][d - 1];
+    __syncthreads();
+
+    // ---------------------------------------------------------------- T = S L S^T
+    double glo = INFINITY, ghi = -INFINITY, em = 0.0;
+    if (t < d) {
+        const double el = t > 0 ? fabs(ee[t - 1]) : 0.0;
+        const double er = t + 1 < d ? fabs(ee[t]) : 0.0;
+        glo = dd[t] - el - er;
+        ghi = dd[t] + el + er;
+        if (t + 1 < d) {
+            e2[t] = ee[t] * ee[t];
+            em = e2[t];
+        }
+    }
+    glo = block_min<NT>(glo, red);
+    ghi = block_max<NT>(ghi, red);
+    em = block_max<NT>(em, red);
+    const double tnorm = fmax(fabs(glo), fabs(ghi));
+    const double pivmin = DBL_MIN * fmax(1.0, em);
+    const double gl = glo - 2.0 * kEps * tnorm * d - 2.0 * pivmin;
+    const double gu = ghi + 2.0 * kEps * tnorm * d + 2.0 * pivmin;
+    const double abstol = 2.0 * kEps * tnorm;
+    const double dprev = lprev[d];
+    const bool zero_block = !(tnorm > 0.0);
+    double change = 0.0;
+    bool bad = false;
+    if (t < d) {
+        const int l = t;
+        double lam = 0.0;
+        if (!zero_block) {
+            double lo = gl, hi = gu;
+            int clo = 0, chi = d;
+            int passes = 0;
+            double s = 0.5 * (lo + hi), S = 0.0;
+            bool haveS = false;
+            auto eval = [&](double x, double& Sx) {
+                const int c = sturm(x, d, dd, e2, pivmin, Sx);
+                passes++;
+                if (c <= l) {
+                    lo = x;
+                    clo = c;
+                } else {
+                    hi = x;
+                    chi = c;
+                }
+                return c;
+            };
+            const double lp = lprev[l];
+            if (dprev >= 0.0 && isfinite(lp) && isfinite(dprev)) {
+                // expanding search around the previous eigenvalue of this block
+                double del = fmax(2.0 * dprev, 8.0 * abstol);
+                s = fmin(fmax(lp, lo), hi);
+                const int c0 = eval(s, S);
+                haveS = true;
+                for (int e = 0; e < 8; e++) {
+                    const double x = (c0 <= l) ? s + del : s - del;
+                    if (!(x > lo && x < hi)) break;
+                    double Sx;
+                    const int cx = eval(x, Sx);
+                    if ((c0 <= l) != (cx <= l)) break;
+                    del *= 4.0;
+                }
+            }
+            lam = 0.5 * (lo + hi);
+            double last = INFINITY;
+            while (passes < kMaxPasses) {
+                if (hi - lo <= abstol) {
+                    lam = 0.5 * (lo + hi);
+                    break;
+                }
+                double sn = 0.0;
+                bool newton = false;
+                if (clo == l && chi == l + 1 && haveS) {
+                    const double step = -1.0 / S;
+                    sn = s + step;
+                    if (fabs(step) <= 2.0 * abstol) {
+                        lam = fmin(fmax(sn, lo), hi);
+                        break;
+                    }
+                    newton = isfinite(sn) && sn > lo && sn < hi && fabs(step) <= 0.5 * last;
+                    if (newton) last = fabs(step);
+                }
+                if (!newton) {
+                    sn = 0.5 * (lo + hi);
+                    last = INFINITY;
+                }
+                s = sn;
+                eval(s, S);
+                haveS = true;
+                lam = s;
+            }
+            if (passes >= kMaxPasses) bad = true;
+            if (isfinite(lp)) change = fabs(lam - lp);
+        }
+        lamv[l] = lam;
+        xs[l] = 1.0;
+    }
+    change = block_max<NT>(change, red);  // also publishes lamv
+    if (t < d) lprev[t] = lamv[t];
+    if (t == 0) lprev[d] = (isfinite(dprev) && dprev >= 0.0) ? change : 1e-2 * tnorm;
+
+    // inverse iteration, one thread per cluster of close eigenvalues; the
+    // vectors stay unnormalised, xs[l] = ||S(:, l)||^2
+    if (!zero_block && t < d) {
+        auto linked = [&](int l) {
+            return l > 0 && (lamv[l] - lamv[l - 1]) <= kClusterRel * fmax(fabs(lamv[l]), fabs(lamv[l - 1])) +
+                                                           kClusterAbs * tnorm;
+        };
+        if (!linked(t)) {
+            const int l0 = t;
+            int l1 = l0;
+            while (l1 + 1 < d && linked(l1 + 1)) l1++;
+            const double ptol = kEps * tnorm;
+            double sprev = -INFINITY;
+            for (int l = l0; l <= l1; l++) {
+                double sg = lamv[l];
+                const double pert = 10.0 * kEps * fmax(fabs(sg), 1e-3 * tnorm);
+                if (l > l0 && sg <= sprev + pert) sg = sprev + pert;
+                sprev = sg;
+                const Column col{Sv + l, fac + l, (size_t)d, (size_t)d * d};
+                // T - sg I = P L U with partial pivoting ([q][i]: 0 multiplier,
+                // 1 reciprocal pivot, 2-3 superdiagonals of U, 4 interchange),
+                // fused with L^{-1} P applied to the start vector
+                double dcur = dd[0] - sg, ucur = d > 1 ? ee[0] : 0.0;
+                double bc = start_value((uint32_t)l, 0u);
+                for (int i = 0; i + 1 < d; i++) {
+                    const double sub = ee[i];
+                    double dnext = dd[i + 1] - sg;
+                    double unext = (i + 2 < d) ? ee[i + 1] : 0.0;
+                    const double bn = start_value((uint32_t)l, (uint32_t)(i + 1));
+                    double f, u0, u1, u2, sw, xi;
+                    if (fabs(dcur) >= fabs(sub)) {
+                        if (fabs(dcur) < ptol) dcur = (dcur < 0.0) ? -ptol : ptol;
+                        f = sub * recip(dcur);
+                        u0 = dcur;
+                        u1 = ucur;
+                        u2 = 0.0;
+                        sw = 0.0;
+                        dnext = fma(-f, ucur, dnext);
+                        xi = bc;
+                        bc = fma(-f, bc, bn);
+                    } else {
+                        f = dcur * recip(sub);
+                        u0 = sub;
+                        u1 = dnext;
+                        u2 = unext;
+                        sw = 1.0;
+                        dnext = fma(-f, dnext, ucur);
+                        unext = -f * unext;
+                        xi = bn;
+                        bc = fma(-f, bn, bc);
+                    }
+                    col.f(0, i) = f;
+                    col.f(1, i) = recip(u0);
+                    col.f(2, i) = u1;
+                    col.f(3, i) = u2;
+                    col.f(4, i) = sw;
+                    col.X(i) = xi;
+                    dcur = dnext;
+                    ucur = unext;
+                }
+                if (fabs(dcur) < ptol) dcur = (dcur < 0.0) ? -ptol : ptol;
+                col.f(1, d - 1) = recip(dcur);
+                col.f(2, d - 1) = 0.0;
+                col.f(3, d - 1) = 0.0;
+                col.X(d - 1) = bc;
+                double s2 = apply_u(col, d);
+                if (l > l0) s2 = orthogonalise(col, d, Sv, l0, l, xs);
+                apply_pl(col, d, 1.0 / sqrt(s2));
+                s2 = apply_u(col, d);
+                if (l > l0) s2 = orthogonalise(col, d, Sv, l0, l, xs);
+                if (!(s2 > 0.0) || !isfinite(s2)) {
+                    bad = true;
+                    s2 = INFINITY;  // drops the vector from T+
+                }
+                xs[l] = s2;
+            }
+        }
+    }
+    if (bad) atomicOr(&st.status, (int)kStatusJacobiNotConverged);
+    // eigenvalues are ascending: the first ps are <= 0 (a count, so that the
+    // range below stays inside [0, d] whatever the values)
+    const int ps = d - (int)block_sum<NT>((t < d && !(lamv[t] <= 0.0)) ? 1.0 : 0.0, red);  // also completes Sv, xs
+
+    // ---------------------------------------------------------------- T+ in registers
+    const bool use_pos = zero_block || (d - ps) <= ps;
+    const int lb = use_pos ? ps : 0, le = use_pos ? d : ps;
+    const double sgn = use_pos ? 1.0 : -1.0;
+#pragma unroll
+    for (int r = 0; r < RI; r++) {
+        const int i = tr + PR * r;
+#pragma unroll
+        for (int c = 0; c < CJ; c++) {
+            const int j = tc + 32 * c;
+            double v = 0.0;
+            if (!use_pos && i < d && j < d) {
+                if (i == j) v = dd[i];
+                else if (i == j + 1) v = ee[j];
+                else if (j == i + 1) v = ee[i];
+            }
+            a[r][c] = v;
+        }
+    }
+    if (!zero_block) {
+        for (int l0 = lb; l0 < le; l0 += kStage) {
+            lds_sync();
+            for (int e = t; e < kStage * DMAX; e += NT) {
+                const int i = e / kStage, m = e - i * kStage;
+                stage[m][i] = (l0 + m < le && i < d) ? Sv[(size_t)i * d + l0 + m] : 0.0;
+            }
+            lds_sync();
+            const int nm = (le - l0 < kStage) ? le - l0 : kStage;
+            for (int m = 0; m < nm; m++) {
+                const double wl = sgn * lamv[l0 + m] / xs[l0 + m];
+                double sj[CJ];
+#pragma unroll
+                for (int c = 0; c < CJ; c++) sj[c] = stage[m][tc + 32 * c];
+#pragma unroll
+                for (int r = 0; r < RI; r++) {
+                    const double ar = wl * stage[m][tr + PR * r];
+#pragma unroll
+                    for (int c = 0; c < CJ; c++) a[r][c] = fma(ar, sj[c], a[r][c]);
+                    row_fence();
+                }
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- U+ = Q T+ Q^T
+    if (d >= 3) {
+        auto load_v = [&](int k, int buf) {
+            for (int i = t; i < DMAX; i += NT) colb[buf][i] = (i > k && i < d) ? hv[(size_t)k * d + i] : 0.0;
+        };
+        load_v(d - 3, (d - 3) & 1);
+        for (int k = d - 3; k >= 0; k--) {
+            lds_sync();
+            const double tk = tau[k];
+            if (k > 0) load_v(k - 1, (k - 1) & 1);
+            if (tk != 0.0) {
+                const double* v = colb[k & 1];
+                double vc[CJ];
+#pragma unroll
+                for (int c = 0; c < CJ; c++) vc[c] = v[tc + 32 * c];
+                matvec(vc, tk);
+                lds_sync();
+                double kk = 0.0;
+                for (int i = k + 1 + lane; i < d; i += 64) kk = fma(v[i], pbuf[i], kk);
+                kk = 0.5 * tk * wsum(kk);
+                rank2(v, vc, 1.0, -1, -1, kk);
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- BPSDP block update
+    double ex = 0.0, dr = 0.0;
+    double* X = B.x + I.poff + it.off;
+    double* Z = B.z + I.poff + it.off;
+    double* V1 = B.v1 + I.poff + it.off;
+    const double* Cc = B.c + I.poff + it.off;
+    const double inv_mu = 1.0 / mu;
+#pragma unroll
+    for (int r = 0; r < RI; r++) {
+        const int i = tr + PR * r;
+#pragma unroll
+        for (int c = 0; c < CJ; c++) {
+            const int j = tc + 32 * c;
+            if (i < d && j < d) {
+                const size_t e = (size_t)i * d + j;
+                const double up = a[r][c];
+                const double uij = U[e];
+                const double sym = 0.5 * (uij + U[(size_t)j * d + i]);
+                const double xn = up * inv_mu;
+                const double zn = up - sym;
+                const double xo = X[e];
+                const double res = (uij - mu * xo) + zn;
+                X[e] = xn;
+                Z[e] = zn;
+                const double ce = Cc[e];
+                V1[e] = ce - zn;
+                ex += ce * xn;
+                dr += res * res;
+            }
+        }
+        row_fence();
+    }
+    ex = block_sum<NT>(ex, red);
+    dr = block_sum<NT>(dr, red);
+    if (t == 0) {
+        st.pending = 1;
+        B.proj_part[2 * (size_t)it.gblock] = ex;
+        B.proj_part[2 * (size_t)it.gblock + 1] = dr;
+    }
+}
+
+}  // namespace
+
+int project_tri_max_d(int cls) {
+    switch (cls) {
+        case 0: return 64;
+        case 1: return 128;
+        case 2: return 192;
+    }
+    return 0;
+}
+
+void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int nitems, double* work, double* lam,
+                        hipStream_t s) {
+    if (nitems <= 0) return;
+    switch (cls) {
+        case 0:
+            hipLaunchKernelGGL((k_project_tri<256, 8, 2>), dim3(nitems), dim3(256), 0, s, B, items, work, lam);
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_project_tri<512, 8, 4>), dim3(nitems), dim3(512), 0, s, B, items, work, lam);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_project_tri<768, 8, 6>), dim3(nitems), dim3(768), 0, s, B, items, work, lam);
+            break;
+        default:
+            break;
+    }
+}
+
+}  // namespace v2h

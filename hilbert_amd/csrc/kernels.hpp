@@ -23,6 +23,13 @@ void launch_project(int nt, bool global_matrix, const BatchDev& B, const ProjIte
                     double* rec, int max_sweeps, double* gmat, hipStream_t s);
 void set_project_lds_limit(size_t bytes);
 
+// Tridiagonal projection (eig_tri.hip), classes 0/1/2 for blocks up to
+// 64 / 128 / 192.  work: 7 d^2 doubles per block; lam: d + 1 doubles per block,
+// kept between calls (NaN = no previous eigenvalues).
+int project_tri_max_d(int cls);
+void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int nitems, double* work, double* lam,
+                        hipStream_t s);
+
 void launch_fill(double* p, int64_t n, double v, hipStream_t s);
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 void launch_pack_cost(const int32_t* i1, const int32_t* i2, const double* src, double* c, int64_t n,

@@ -102,8 +102,11 @@ class Batch {
     hipStream_t stream() const { return stream_; }
 
     // HIP-event timing of every kernel launch of solve(), per kernel kind:
-    // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..7 projection classes (64/256/512/1024 threads).
-    static constexpr int kKinds = 8;
+    // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..9 projection classes: Jacobi d <= 16,
+    // tridiagonal d <= 64 / 128 / 192, Jacobi d <= 200 (LDS), Jacobi d > 200
+    // (matrix in global memory).
+    static constexpr int kProjClasses = 6;
+    static constexpr int kKinds = 4 + kProjClasses;
     void set_profiling(bool on) { profile_ = on; }
     // ms/counts: all launches; full_ms/full_counts: launches made while every
     // instance of the batch was active (and, for CG kernels, still iterating)
@@ -151,11 +154,13 @@ class Batch {
     struct ProjClass {
         int nt = 0, max_d = 0;
         bool global = false;
+        int tri = -1;               // tridiagonal kernel class, or -1 for Jacobi
         std::vector<ProjItem> items;
         DBuf<ProjItem> d_items;
     };
-    ProjClass pclass_[5];
-    DBuf<double> gmat_;
+    ProjClass pclass_[kProjClasses];
+    DBuf<double> gmat_, work_, lam_;
+    std::vector<int64_t> lam_begin_;  // [count + 1] eigenvalue-store range of each instance
     int max_sweeps_ = 24;
 
     struct KTimer {

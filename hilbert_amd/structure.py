@@ -73,3 +73,17 @@ class Structure:
         keys = ("a_slices", "a_entries", "a_long_rows", "a_long_entries",
                 "at_slices", "at_entries", "at_long_rows", "at_long_entries")
         return dict(zip(keys, s.tolist()))
+
+    def opdm(self, x, type="SUM"):
+        """Dense n x n 1-RDM ("A", "B" or "SUM") of primal vector x, caller order."""
+        out = np.zeros((self.nmo, self.nmo))
+        N.check(N.lib().v2h_structure_opdm(self._h, np.ascontiguousarray(x, dtype=np.float64), type.encode(),
+                                           out.reshape(-1)))
+        return out
+
+    def tpdm(self, x):
+        """Dense spin-free 2-RDM of x, shape (n, n, n, n), [i, k, j, l] = D(ij, kl)."""
+        n = self.nmo
+        out = np.zeros(n ** 4)
+        N.check(N.lib().v2h_structure_tpdm(self._h, np.ascontiguousarray(x, dtype=np.float64), out))
+        return out.reshape(n, n, n, n)

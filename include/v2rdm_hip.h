@@ -121,14 +121,32 @@ int v2h_batch_set_mu(v2h_batch *b, int inst, double mu);       /* libsdp set_mu 
 int v2h_batch_get_c(v2h_batch *b, int inst, double *c);
 int v2h_batch_get_b(v2h_batch *b, int inst, double *bvec);
 
+/* Reduced density matrices of the current primal x, orbital indices in the
+ * caller's order (no frozen or restricted core).
+ * get_opdm: dense n x n, type "A", "B" or "SUM" (v2RDMSolver::get_opdm,
+ *   v2rdm_casscf/python_api.cc:374-394, which returns "SUM").
+ * get_tpdm: dense spin-free 2-RDM, out[(i n + k) n^2 + j n + l]
+ *   (v2RDMSolver::get_tpdm, python_api.cc:396-456; also for nirrep > 1).
+ * get_*_sparse: the reference's entry lists (python_api.cc:39-371); index
+ *   tuples into idx (2 or 4 per entry), values into val, at most cap entries
+ *   written, *count = the full length (call with cap = 0 to size the arrays). */
+int v2h_batch_get_opdm(v2h_batch *b, int inst, const char *type, double *out);
+int v2h_batch_get_tpdm(v2h_batch *b, int inst, double *out);
+int v2h_batch_get_opdm_sparse(v2h_batch *b, int inst, const char *type, int32_t *idx, double *val, int64_t cap,
+                              int64_t *count);
+int v2h_batch_get_tpdm_sparse(v2h_batch *b, int inst, const char *type, int32_t *idx, double *val, int64_t cap,
+                              int64_t *count);
+
 /* The constraint maps (bpsdp_Au / bpsdp_ATu, v2rdm_solver.cc:2995-3108). */
 int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
 
 /* HIP-event timing of the kernels launched by v2h_batch_solve, recorded on
- * the library's stream.  ms[8] and counts[8] per kernel kind: 0 iteration
- * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..7 PSD
- * projection classes of 64 / 256 / 512 / 1024 threads.  Cumulative since the
+ * the library's stream.  ms[10] and counts[10] per kernel kind: 0 iteration
+ * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..9 PSD
+ * projection classes: Jacobi for blocks up to 16, tridiagonal for blocks up to
+ * 64 / 128 / 192, Jacobi up to 200 (LDS) and beyond (global memory).
+ * Cumulative since the
  * last reset.  full_ms / full_counts (may be NULL) cover only the launches
  * made while every instance was running (for CG kernels: still iterating)
  * at the previous host poll, i.e. launches that processed the whole batch. */
@@ -161,6 +179,9 @@ int v2h_structure_cost_map(const v2h_structure *s, int32_t *i1, int32_t *i2);
 /* SELL-64 statistics: [A slices, A entries, A long rows, A long entries,
  * A^T slices, A^T entries, A^T long rows, A^T long entries] */
 int v2h_structure_gather_stats(const v2h_structure *s, int64_t *stats);
+/* RDM exports of a given primal vector x (as v2h_batch_get_opdm / _tpdm). */
+int v2h_structure_opdm(const v2h_structure *s, const double *x, const char *type, double *out);
+int v2h_structure_tpdm(const v2h_structure *s, const double *x, double *out);
 
 #ifdef __cplusplus
 }
