@@ -38,8 +38,13 @@ def bytes_per_instance(kind, n_p, n_d):
                 "post": 4 * n_p + n_d}[kind]   # read y, x, c; write A^T y, U
 
 
+PROJ_CLASSES = (("project_jacobi16", 0, 16), ("project_tri64", 17, 64), ("project_tri128", 65, 128),
+                ("project_tri192", 129, 192), ("project_tri256", 193, 256), ("project_jacobi_big", 257, 1 << 30))
+
+
 def project_flops(dims):
-    """Nominal FP64 flops of one PSD projection sweep: (16/3) d^3 per block."""
+    """Nominal FP64 flops of projecting blocks of these sizes: (16/3) d^3 each
+    ((4/3) d^3 reduction + 2 d^3 back-transform + 2 d^3 for U diag(l+) U^T)."""
     return sum(16.0 / 3.0 * float(d) ** 3 for d in dims if d > 0)
 
 
@@ -58,30 +63,42 @@ def reset_all(B):
         B.reset(i)
 
 
-def roofline(B, kt_full):
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    rec = data.get(workload, {}).get(kernel)
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def roofline(B, kt_full, workload):
     """Dominant kernel (largest time over whole-batch launches) -> roofline dict."""
     n_p, n_d = B.n_primal, B.n_dual
-    proj_kinds = [k for k in kt_full if k.startswith("project")]
-    totals = {k: kt_full[k][0] for k in ("pre", "cg_at", "cg_a", "post")}
-    totals["project"] = sum(kt_full[k][0] for k in proj_kinds)
+    totals = {k: v[0] for k, v in kt_full.items() if v[1]}
     dom = max(totals, key=totals.get)
-    if dom == "project":
-        # one launch per class per iteration; an iteration projects every block once
-        iters = max(kt_full[k][1] for k in proj_kinds)
-        per_iter_ms = totals["project"] / max(iters, 1)
-        flops = sum(project_flops(B.block_dims(i)) for i in range(B.count))
-        achieved = flops / (per_iter_ms * 1e-3) / 1e12
-        return dom, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_project (all size classes, one outer iteration)",
-                     "per_launch_ms": per_iter_ms, "work_per_launch": flops}
     ms, cnt = kt_full[dom]
     per_launch_ms = ms / max(cnt, 1)
+    traffic = load_traffic(dom, workload)
+    if dom.startswith("project"):
+        lo, hi = [(a, b) for k, a, b in PROJ_CLASSES if k == dom][0]
+        dims = [d for i in range(B.count) for d in B.block_dims(i) if lo <= d <= hi]
+        flops = project_flops(dims)
+        achieved = flops / (per_launch_ms * 1e-3) / 1e12
+        return dom, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": dom,
+                     "per_launch_ms": per_launch_ms, "work_per_launch": flops, "blocks_per_launch": len(dims),
+                     "work": "(16/3) d^3 FP64 flops per block"}
     nbytes = sum(bytes_per_instance(dom, n_p[i], n_d[i]) for i in range(B.count))
     achieved = nbytes / (per_launch_ms * 1e-3) / 1e9
     return dom, {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_" + dom,
-                 "per_launch_ms": per_launch_ms, "work_per_launch": nbytes}
+                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_" + dom,
+                 "per_launch_ms": per_launch_ms, "work_per_launch": nbytes,
+                 "work": "FP64 vector bytes per launch (tables excluded)"}
 
 
 def cpu_baseline(item, mean_oiter, seconds):
@@ -106,6 +123,20 @@ def cpu_baseline(item, mean_oiter, seconds):
                       f"({r['iiter']} CG steps, {dt:.1f} s, {cores} OpenMP threads); SDPs/s = 1 / "
                       f"(s per outer iteration x {mean_oiter:.0f} outer iterations per SDP)",
             "seconds_per_outer_iteration": per_it}
+
+
+def solve_with_progress(B, label, chunk=1000):
+    """B.solve_to_convergence() with a progress line on stderr per chunk of
+    outer iterations."""
+    t0 = time.perf_counter()
+    while True:
+        B.solve(chunk)
+        st = [B.status(i) for i in range(B.count)]
+        done = sum(s["converged"] for s in st)
+        print(f"[bench] {label}: {max(s['oiter'] for s in st)} outer iterations, {done}/{B.count} converged, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if all(s["converged"] or s["oiter"] >= B.opts.maxiter or s["status_bits"] for s in st):
+            return done
 
 
 def host_info():
@@ -139,16 +170,16 @@ def main():
     items = configs.c2v_16(args.batch)
     B = make_batch(items, args.maxiter)
     B.set_profiling(True)
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         reset_all(B)
-        B.solve_to_convergence()
+        solve_with_progress(B, f"warmup {w}")
     B.reset_kernel_times()
 
     step_times, conv, oiters = [], [], []
-    for _ in range(args.steps):
+    for k in range(args.steps):
         reset_all(B)
         t0 = time.perf_counter()
-        nconv = B.solve_to_convergence()
+        nconv = solve_with_progress(B, f"step {k}")
         step_times.append(time.perf_counter() - t0)
         conv.append(nconv)
         oiters.append([B.status(i)["oiter"] for i in range(B.count)])
@@ -158,7 +189,8 @@ def main():
     value = sum(conv) / total
     mean_oiter = sum(sum(o) for o in oiters) / max(1, sum(len(o) for o in oiters))
     st = [B.status(i) for i in range(B.count)]
-    dom, roof = roofline(B, kt_full)
+    workload = f"c2v16x{args.batch}"
+    dom, roof = roofline(B, kt_full, workload)
     B.close()
 
     single = None
@@ -166,13 +198,14 @@ def main():
         S = make_batch(items[:1], args.maxiter)
         reset_all(S)
         t0 = time.perf_counter()
-        S.solve_to_convergence()
+        solve_with_progress(S, "single instance")
         dt = time.perf_counter() - t0
         single = {"value": 1.0 / dt, "unit": "SDPs/s", "seconds": dt, "oiter": S.status(0)["oiter"]}
         S.close()
 
     cpu = None
     if args.cpu_seconds > 0:
+        print(f"[bench] CPU oracle sample ({args.cpu_seconds:.0f} s)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(items[0], mean_oiter, args.cpu_seconds)
 
     out = {

@@ -65,6 +65,7 @@ SIGNATURES = {
     "v2h_batch_set_integrals": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_set_cost": (C.c_int, [_vp, C.c_int, _dp]),
     "v2h_batch_set_guess": (C.c_int, [_vp, C.c_int, _vp]),
+    "v2h_batch_set_guess_hf": (C.c_int, [_vp, C.c_int]),
     "v2h_batch_reset": (C.c_int, [_vp, C.c_int]),
     "v2h_batch_solve": (C.c_int, [_vp, C.c_int, _vp, _vp, C.POINTER(C.c_int)]),
     "v2h_batch_status": (C.c_int, [_vp, C.c_int, C.POINTER(Status)]),
@@ -102,6 +103,7 @@ SIGNATURES = {
     "v2h_structure_gather_stats": (C.c_int, [_vp, _lp]),
     "v2h_structure_opdm": (C.c_int, [_vp, _dp, C.c_char_p, _dp]),
     "v2h_structure_tpdm": (C.c_int, [_vp, _dp, _dp]),
+    "v2h_structure_hf_guess": (C.c_int, [_vp, _dp]),
 }
 
 _lib = None

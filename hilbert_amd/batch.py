@@ -84,6 +84,10 @@ class Batch:
             x = np.ascontiguousarray(x, dtype=np.float64)
             N.check(N.lib().v2h_batch_set_guess(self._h, i, x.ctypes.data_as(C.c_void_p)))
 
+    def set_guess_hf(self, i):
+        """hilbert's TPDM_GUESS HF (see v2h_batch_set_guess_hf)."""
+        N.check(N.lib().v2h_batch_set_guess_hf(self._h, i))
+
     def reset(self, i):
         N.check(N.lib().v2h_batch_reset(self._h, i))
 
@@ -119,7 +123,7 @@ class Batch:
         return N.lib().v2h_batch_last_solve_seconds(self._h)
 
     KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project_jacobi16", "project_tri64", "project_tri128",
-                    "project_tri192", "project_jacobi200", "project_jacobi_big")
+                    "project_tri192", "project_tri256", "project_jacobi_big")
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))

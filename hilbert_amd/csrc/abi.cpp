@@ -186,6 +186,13 @@ int v2h_batch_set_cost(v2h_batch* b, int inst, const double* c) {
     });
 }
 
+int v2h_batch_set_guess_hf(v2h_batch* b, int inst) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        B(b).set_guess_hf(inst);
+    });
+}
+
 int v2h_batch_set_guess(v2h_batch* b, int inst, const double* x) {
     return guarded([&] {
         check_inst(B(b), inst);
@@ -511,6 +518,15 @@ int v2h_structure_opdm(const v2h_structure* s, const double* x, const char* type
         check_ptr(type);
         check_ptr(out);
         v2h::opdm_dense(Sref(s).L, x, type, out);
+    });
+}
+
+int v2h_structure_hf_guess(const v2h_structure* s, double* x) {
+    return guarded([&] {
+        check_ptr(x);
+        const auto& S = Sref(s);
+        std::vector<double> g = v2h::hf_guess(S.L, S.A, S.L.na, S.L.na);
+        std::copy(g.begin(), g.end(), x);
     });
 }
 

@@ -361,6 +361,54 @@ Csr Csr::transpose() const {
     return T;
 }
 
+std::vector<double> hf_guess(const Layout& L, const Csr& A, int nocc_alpha, int nocc_beta) {
+    std::vector<double> x(L.n_primal, 0.0);
+    const int n = L.n;
+    std::vector<double> oa(n), ob(n);
+    for (int p = 0; p < n; p++) {
+        oa[p] = L.to_user[p] < nocc_alpha ? 1.0 : 0.0;
+        ob[p] = L.to_user[p] < nocc_beta ? 1.0 : 0.0;
+    }
+    for (int p = 0; p < n; p++) {
+        x[L.one_body(D1A, p, p)] = oa[p];
+        x[L.one_body(D1B, p, p)] = ob[p];
+        x[L.one_body(Q1A, p, p)] = 1.0 - oa[p];
+        x[L.one_body(Q1B, p, p)] = 1.0 - ob[p];
+    }
+    for (int h = 0; h < L.H; h++) {
+        for (int ij = 0; ij < L.gab[h]; ij++) {
+            const int i = L.ab_pairs[h][ij][0], j = L.ab_pairs[h][ij][1];
+            x[L.off[D2AB][h] + (int64_t)ij * L.gab[h] + ij] = oa[i] * ob[j];
+        }
+        for (int ij = 0; ij < L.gaa[h]; ij++) {
+            const int i = L.aa_pairs[h][ij][0], j = L.aa_pairs[h][ij][1];
+            x[L.off[D2AA][h] + (int64_t)ij * L.gaa[h] + ij] = oa[i] * oa[j];
+            x[L.off[D2BB][h] + (int64_t)ij * L.gaa[h] + ij] = ob[i] * ob[j];
+        }
+    }
+    // family of every primal index
+    std::vector<int8_t> fam(L.n_primal, -1);
+    for (const Block& b : L.blocks)
+        for (int64_t e = 0; e < (int64_t)b.dim * b.dim; e++) fam[b.off + e] = (int8_t)b.fam;
+    auto derived = [&](int32_t c) { return fam[c] >= Q2AB && fam[c] <= G2AA; };
+    // every Q2 / G2 row holds exactly one Q2 / G2 element; solve the row for it
+    for (int64_t r = 0; r < A.nrows; r++) {
+        int64_t e = -1;
+        int nd = 0;
+        for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; k++)
+            if (derived(A.col[k])) {
+                e = k;
+                nd++;
+            }
+        if (nd != 1) continue;
+        double s = A.rhs[r];
+        for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; k++)
+            if (k != e) s -= A.val[k] * x[A.col[k]];
+        x[A.col[e]] = s / A.val[e];
+    }
+    return x;
+}
+
 CostMap build_cost_map(const Layout& L) {
     CostMap M;
     M.i1.assign(L.n_primal, -1);

@@ -27,6 +27,15 @@ struct Csr {
 // exact zeros dropped.
 Csr build_constraints(const Layout& L);
 
+// TPDM_GUESS HF (v2rdm_solver.cc:1711-1798 with Q2_constraints_guess,
+// q2.cc:45-167, and G2_constraints_guess, g2.cc:44-242): D1, Q1, D2ab/aa/bb
+// of the determinant that occupies the first nocc_alpha / nocc_beta orbitals
+// of the caller's order, and every Q2 / G2 element from the constraint row
+// that defines it (the rows are linear in D1, Q1, D2, so this is the same
+// value the reference's guess loops compute).  D200 stays zero, as in the
+// reference.
+std::vector<double> hf_guess(const Layout& L, const Csr& A, int nocc_alpha, int nocc_beta);
+
 // Gather map of the cost vector: c[e] = src[i1[e]] - src[i2[e]] with
 // src = [h (n*n, user order) ; g (n^4, user order)] and index -1 meaning 0
 // (v2rdm_solver.cc:164-193, tei.cc:139-199).

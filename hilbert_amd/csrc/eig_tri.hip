@@ -283,7 +283,34 @@ __device__ double orthogonalise(const Column& c, int d, const double* Sv, int l0
     return s2;
 }
 
+// The block's share of one thread: RI x CJ elements, in registers, or (GM)
+// in global memory laid out [wavefront][r][c][lane]: a wavefront's accesses
+// coalesce and its elements lie within a few immediate offsets of one base
+// (blocks too large for the register budget).
+template <int NT, int RI, int CJ, bool GM>
+struct Tile;
 template <int NT, int RI, int CJ>
+struct Tile<NT, RI, CJ, false> {
+    double v[RI][CJ];
+    __device__ __forceinline__ double& at(int r, int c) { return v[r][c]; }
+    __device__ __forceinline__ double col(int r, int ck) {
+        double x = 0.0;
+#pragma unroll
+        for (int c = 0; c < CJ; c++)
+            if (c == ck) x = v[r][c];
+        return x;
+    }
+};
+template <int NT, int RI, int CJ>
+struct Tile<NT, RI, CJ, true> {
+    double* p;
+    __device__ __forceinline__ double& at(int r, int c) {
+        return p[((size_t)(threadIdx.x >> 6) * RI * CJ + r * CJ + c) * 64 + (threadIdx.x & 63)];
+    }
+    __device__ __forceinline__ double col(int r, int ck) { return at(r, ck); }
+};
+
+template <int NT, int RI, int CJ, bool GM>
 __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* items, double* work, double* lamst) {
     constexpr int PR = NT / 32;
     constexpr int DMAX = PR * RI;
@@ -311,7 +338,8 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     double* lprev = lamst + it.lam_off;     // d eigenvalues + last largest change
 
     for (int i = t; i < 2 * DMAX; i += NT) (&colb[0][0])[i] = 0.0;
-    double a[RI][CJ];
+    Tile<NT, RI, CJ, GM> A;
+    if constexpr (GM) A.p = fac + 5 * (size_t)d * d;  // NT RI CJ doubles after the factors
 #pragma unroll
     for (int r = 0; r < RI; r++) {
         const int i = tr + PR * r;
@@ -320,7 +348,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             const int j = tc + 32 * c;
             const int ic = min(i, d - 1), jc = min(j, d - 1);
             const double s = 0.5 * (U[(size_t)ic * d + jc] + U[(size_t)jc * d + ic]);
-            a[r][c] = (i < d && j < d) ? s : 0.0;
+            A.at(r, c) = (i < d && j < d) ? s : 0.0;
         }
         row_fence();
     }
@@ -333,10 +361,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
 #pragma unroll
             for (int r = 0; r < RI; r++) {
                 const int i = tr + PR * r;
-                double v = 0.0;
-#pragma unroll
-                for (int c = 0; c < CJ; c++)
-                    if (c == ck) v = a[r][c];
+                const double v = A.col(r, ck);
                 if (i < d) {
                     if (i > k) colb[buf][i] = v;
                     else if (i == k) dd[k] = v;
@@ -352,9 +377,10 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             double s = 0.0;
             if (r < RI) {
 #pragma unroll
-                for (int c = 0; c < CJ; c++) s = fma(a[r][c], vc[c], s);
+                for (int c = 0; c < CJ; c++) s = fma(A.at(r, c), vc[c], s);
             }
             s8[r] = s;
+            row_fence();
         }
         const double ps = reduce_scatter8(s8);
         if ((lane & 3) == 0 && rr < RI) pbuf[tr + PR * rr] = tk * ps;
@@ -377,7 +403,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             const double vi = in ? (i == one ? 1.0 : vraw * vscale) : 0.0;
             const double wi = in ? fma(-kk, vi, pi) : 0.0;
 #pragma unroll
-            for (int c = 0; c < CJ; c++) a[r][c] = fma(-wi, vc[c], fma(-vi, wc[c], a[r][c]));
+            for (int c = 0; c < CJ; c++) A.at(r, c) = fma(-wi, vc[c], fma(-vi, wc[c], A.at(r, c)));
             row_fence();
         }
     };
@@ -622,7 +648,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
                 else if (i == j + 1) v = ee[j];
                 else if (j == i + 1) v = ee[i];
             }
-            a[r][c] = v;
+            A.at(r, c) = v;
         }
     }
     if (!zero_block) {
@@ -643,7 +669,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
                 for (int r = 0; r < RI; r++) {
                     const double ar = wl * stage[m][tr + PR * r];
 #pragma unroll
-                    for (int c = 0; c < CJ; c++) a[r][c] = fma(ar, sj[c], a[r][c]);
+                    for (int c = 0; c < CJ; c++) A.at(r, c) = fma(ar, sj[c], A.at(r, c));
                     row_fence();
                 }
             }
@@ -690,7 +716,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             const int j = tc + 32 * c;
             if (i < d && j < d) {
                 const size_t e = (size_t)i * d + j;
-                const double up = a[r][c];
+                const double up = A.at(r, c);
                 const double uij = U[e];
                 const double sym = 0.5 * (uij + U[(size_t)j * d + i]);
                 const double xn = up * inv_mu;
@@ -723,6 +749,7 @@ int project_tri_max_d(int cls) {
         case 0: return 64;
         case 1: return 128;
         case 2: return 192;
+        case 3: return 256;
     }
     return 0;
 }
@@ -732,13 +759,18 @@ void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int n
     if (nitems <= 0) return;
     switch (cls) {
         case 0:
-            hipLaunchKernelGGL((k_project_tri<256, 8, 2>), dim3(nitems), dim3(256), 0, s, B, items, work, lam);
+            hipLaunchKernelGGL((k_project_tri<256, 8, 2, false>), dim3(nitems), dim3(256), 0, s, B, items, work, lam);
             break;
         case 1:
-            hipLaunchKernelGGL((k_project_tri<512, 8, 4>), dim3(nitems), dim3(512), 0, s, B, items, work, lam);
+            hipLaunchKernelGGL((k_project_tri<512, 8, 4, false>), dim3(nitems), dim3(512), 0, s, B, items, work, lam);
             break;
         case 2:
-            hipLaunchKernelGGL((k_project_tri<768, 8, 6>), dim3(nitems), dim3(768), 0, s, B, items, work, lam);
+            hipLaunchKernelGGL((k_project_tri<768, 8, 6, false>), dim3(nitems), dim3(768), 0, s, B, items, work,
+                               lam);
+            break;
+        case 3:
+            hipLaunchKernelGGL((k_project_tri<1024, 8, 8, true>), dim3(nitems), dim3(1024), 0, s, B, items, work,
+                               lam);
             break;
         default:
             break;

@@ -23,9 +23,10 @@ void launch_project(int nt, bool global_matrix, const BatchDev& B, const ProjIte
                     double* rec, int max_sweeps, double* gmat, hipStream_t s);
 void set_project_lds_limit(size_t bytes);
 
-// Tridiagonal projection (eig_tri.hip), classes 0/1/2 for blocks up to
-// 64 / 128 / 192.  work: 7 d^2 doubles per block; lam: d + 1 doubles per block,
-// kept between calls (NaN = no previous eigenvalues).
+// Tridiagonal projection (eig_tri.hip), classes 0/1/2/3 for blocks up to
+// 64 / 128 / 192 / 256.  work: 7 d^2 doubles per block (class 3: plus
+// 1024 x 64 for the tile); lam: d + 1 doubles per block, kept between calls
+// (NaN = no previous eigenvalues).
 int project_tri_max_d(int cls);
 void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int nitems, double* work, double* lam,
                         hipStream_t s);

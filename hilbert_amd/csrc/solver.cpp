@@ -34,21 +34,24 @@ std::vector<int4> unit_list(const SellHost& S) {
     return out;
 }
 
-// Projection classes by block size.  Blocks of 17..192 take the tridiagonal
-// path (register-tiled Householder reduction); the smallest blocks and those
-// above 192 take parallel Jacobi, with the packed matrix in LDS up to d = 200
-// and in global memory beyond.
+// Projection classes by block size.  Blocks of 17..256 take the tridiagonal
+// path (register-tiled Householder reduction up to 192, the tile in global
+// memory up to 256); the smallest blocks and those above 256 take parallel
+// Jacobi (packed matrix in LDS for d <= 16, in global memory beyond 256).
 constexpr int kProjClasses = Batch::kProjClasses, kGlobalClass = 5;
 constexpr int kClassThreads[kProjClasses] = {64, 256, 512, 768, 1024, 1024};
-constexpr int kClassTri[kProjClasses] = {-1, 0, 1, 2, -1, -1};
+constexpr int kClassTri[kProjClasses] = {-1, 0, 1, 2, 3, -1};
 int proj_class(int d) {
     if (d <= 16) return 0;
     if (d <= 64) return 1;
     if (d <= 128) return 2;
     if (d <= 192) return 3;
-    if (d <= 200) return 4;
+    if (d <= 256) return 4;
     return kGlobalClass;
 }
+// work doubles of a tridiagonal-path block: reflectors, eigenvectors, LU
+// factors (7 d^2), plus the tile of the global-memory class
+int64_t tri_work(int tri, int d) { return 7 * (int64_t)d * d + (tri == 3 ? (int64_t)1024 * 8 * 8 : 0); }
 }  // namespace
 
 // ------------------------------------------------------------------ Structure
@@ -242,7 +245,7 @@ void Batch::build_projection() {
             it.off = b.off;
             if (kClassTri[k] >= 0) {
                 it.work_off = work;
-                work += 7 * (int64_t)b.dim * b.dim;
+                work += tri_work(kClassTri[k], b.dim);
                 it.lam_off = lam;
                 lam += b.dim + 1;
             } else {
@@ -414,6 +417,14 @@ void Batch::set_guess(int i, const double* x) {
     }
     V2H_HIP(hipMemcpyAsync(x_.p + I.poff, x, sizeof(double) * np, hipMemcpyHostToDevice, stream_));
     V2H_HIP(hipStreamSynchronize(stream_));
+}
+
+void Batch::set_guess_hf(int i) {
+    const Layout& L = layout(i);
+    // the external-Hamiltonian path sets doccpi = nalpha, soccpi = 0
+    // (v2rdm_solver.cc:1004-1007), so Guess() occupies nalpha beta orbitals too
+    std::vector<double> x = hf_guess(L, build_constraints(L), L.na, L.na);
+    set_guess(i, x.data());
 }
 
 void Batch::reset_state(int i) {

@@ -76,6 +76,7 @@ class Batch {
     void set_integrals(int i, const double* h, const double* g);
     void set_cost(int i, const double* c);        // primal-layout cost vector
     void set_guess(int i, const double* x);       // nullptr: hilbert's random guess
+    void set_guess_hf(int i);                     // hilbert's TPDM_GUESS HF
     void reset_state(int i);                      // y = z = 0, mu = mu0, counters cleared
 
     // Run up to maxiter outer iterations for every unconverged instance.
@@ -103,8 +104,8 @@ class Batch {
 
     // HIP-event timing of every kernel launch of solve(), per kernel kind:
     // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..9 projection classes: Jacobi d <= 16,
-    // tridiagonal d <= 64 / 128 / 192, Jacobi d <= 200 (LDS), Jacobi d > 200
-    // (matrix in global memory).
+    // tridiagonal d <= 64 / 128 / 192 / 256, Jacobi d > 256 (matrix in global
+    // memory).
     static constexpr int kProjClasses = 6;
     static constexpr int kKinds = 4 + kProjClasses;
     void set_profiling(bool on) { profile_ = on; }

@@ -87,3 +87,9 @@ class Structure:
         out = np.zeros(n ** 4)
         N.check(N.lib().v2h_structure_tpdm(self._h, np.ascontiguousarray(x, dtype=np.float64), out))
         return out.reshape(n, n, n, n)
+
+    def hf_guess(self):
+        """TPDM_GUESS HF primal vector (v2h_structure_hf_guess)."""
+        x = np.zeros(self.n_primal)
+        N.check(N.lib().v2h_structure_hf_guess(self._h, x))
+        return x

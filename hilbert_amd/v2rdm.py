@@ -75,8 +75,8 @@ def resolve_options(options=None) -> dict:
         raise NotImplementedError(f"POSITIVITY {opts['POSITIVITY']}: only D, DQ, DG and DQG are implemented")
     if opts["SDP_SOLVER"] != "BPSDP":
         raise NotImplementedError(f"SDP_SOLVER {opts['SDP_SOLVER']}: only BPSDP is implemented")
-    if opts["TPDM_GUESS"] != "RANDOM":
-        raise NotImplementedError(f"TPDM_GUESS {opts['TPDM_GUESS']}: only RANDOM is implemented")
+    if opts["TPDM_GUESS"] not in ("RANDOM", "HF"):
+        raise ValueError(f"TPDM_GUESS {opts['TPDM_GUESS']}: RANDOM or HF")
     if opts["GPC_CONSTRAINTS"] != "NONE":
         raise NotImplementedError("GPC_CONSTRAINTS are not implemented")
     for k in ("CONSTRAIN_D3", "CONSTRAIN_E3", "CONSTRAIN_F3", "CONSTRAIN_Q3", "CONSTRAIN_D4"):
@@ -147,6 +147,8 @@ class v2RDMHelper:
         if self._batch is None:
             self._batch = Batch([self.problem], self._sdp_options())
             self._batch.set_integrals(0, self.h, self.g)
+            if self.options["TPDM_GUESS"] == "HF":
+                self._batch.set_guess_hf(0)
         self._batch.solve_to_convergence()
         st = self._batch.status(0)
         self.status = st

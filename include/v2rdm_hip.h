@@ -101,6 +101,12 @@ int v2h_batch_set_integrals(v2h_batch *b, int inst, const double *h, const doubl
 int v2h_batch_set_cost(v2h_batch *b, int inst, const double *c);
 /* Primal guess; NULL = hilbert's TPDM_GUESS RANDOM (v2rdm_solver.cc:1799-1806). */
 int v2h_batch_set_guess(v2h_batch *b, int inst, const double *x);
+/* TPDM_GUESS HF (Guess, v2rdm_solver.cc:1711-1798; Q2_constraints_guess,
+ * q2.cc:45-167; G2_constraints_guess, g2.cc:44-242): the determinant of the
+ * first orbitals of the caller's order; nalpha orbitals of each spin are
+ * occupied, as the external path's doccpi = nalpha (v2rdm_solver.cc:1004-1007)
+ * makes the reference do; D200 left zero. */
+int v2h_batch_set_guess_hf(v2h_batch *b, int inst);
 /* y = z = 0, mu = mu0, iteration counters cleared. */
 int v2h_batch_reset(v2h_batch *b, int inst);
 
@@ -145,7 +151,7 @@ int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
  * the library's stream.  ms[10] and counts[10] per kernel kind: 0 iteration
  * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..9 PSD
  * projection classes: Jacobi for blocks up to 16, tridiagonal for blocks up to
- * 64 / 128 / 192, Jacobi up to 200 (LDS) and beyond (global memory).
+ * 64 / 128 / 192 / 256, Jacobi beyond 256 (matrix in global memory).
  * Cumulative since the
  * last reset.  full_ms / full_counts (may be NULL) cover only the launches
  * made while every instance was running (for CG kernels: still iterating)
@@ -182,6 +188,8 @@ int v2h_structure_gather_stats(const v2h_structure *s, int64_t *stats);
 /* RDM exports of a given primal vector x (as v2h_batch_get_opdm / _tpdm). */
 int v2h_structure_opdm(const v2h_structure *s, const double *x, const char *type, double *out);
 int v2h_structure_tpdm(const v2h_structure *s, const double *x, double *out);
+/* The TPDM_GUESS HF primal vector (as v2h_batch_set_guess_hf). */
+int v2h_structure_hf_guess(const v2h_structure *s, double *x);
 
 #ifdef __cplusplus
 }

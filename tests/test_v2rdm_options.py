@@ -18,7 +18,7 @@ def test_case_insensitive_names_and_values():
     assert o["POSITIVITY"] == "DQ" and o["MAXITER"] == 20000
 
 
-@pytest.mark.parametrize("opt", [{"POSITIVITY": "DQGT2"}, {"SDP_SOLVER": "RRSDP"}, {"TPDM_GUESS": "HF"},
+@pytest.mark.parametrize("opt", [{"POSITIVITY": "DQGT2"}, {"SDP_SOLVER": "RRSDP"},
                                  {"CONSTRAIN_D3": True}, {"GPC_CONSTRAINTS": "1RDM"}, {"TAU_PARAMETER": 1.5}])
 def test_unimplemented_settings_raise(opt):
     with pytest.raises(NotImplementedError):
@@ -29,3 +29,9 @@ def test_unknown_option_raises():
     with pytest.raises(KeyError):
         resolve_options({"NOT_AN_OPTION": 1})
     assert "IRREPS" in DEFAULTS
+
+
+def test_guess_values():
+    assert resolve_options({"TPDM_GUESS": "hf"})["TPDM_GUESS"] == "HF"
+    with pytest.raises(ValueError):
+        resolve_options({"TPDM_GUESS": "SAD"})
