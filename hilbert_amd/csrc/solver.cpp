@@ -22,7 +22,6 @@ namespace v2h {
 
 namespace {
 constexpr int kLongThreshold = 32;
-constexpr size_t kLdsLimit = 160 * 1024;
 
 std::vector<int4> unit_list(const SellHost& S) {
     std::vector<int> u;
@@ -34,24 +33,6 @@ std::vector<int4> unit_list(const SellHost& S) {
     return out;
 }
 
-// Projection classes by block size.  Blocks of 17..256 take the tridiagonal
-// path (register-tiled Householder reduction up to 192, the tile in global
-// memory up to 256); the smallest blocks and those above 256 take parallel
-// Jacobi (packed matrix in LDS for d <= 16, in global memory beyond 256).
-constexpr int kProjClasses = Batch::kProjClasses, kGlobalClass = 5;
-constexpr int kClassThreads[kProjClasses] = {64, 256, 512, 768, 1024, 1024};
-constexpr int kClassTri[kProjClasses] = {-1, 0, 1, 2, 3, -1};
-int proj_class(int d) {
-    if (d <= 16) return 0;
-    if (d <= 64) return 1;
-    if (d <= 128) return 2;
-    if (d <= 192) return 3;
-    if (d <= 256) return 4;
-    return kGlobalClass;
-}
-// work doubles of a tridiagonal-path block: reflectors, eigenvectors, LU
-// factors (7 d^2), plus the tile of the global-memory class
-int64_t tri_work(int tri, int d) { return 7 * (int64_t)d * d + (tri == 3 ? (int64_t)1024 * 8 * 8 : 0); }
 }  // namespace
 
 // ------------------------------------------------------------------ Structure
@@ -223,70 +204,16 @@ void Batch::build_units() {
 }
 
 void Batch::build_projection() {
-    int gblock = 0;
-    int64_t rec = 0, mat = 0, work = 0, lam = 0;
-    for (auto& c : pclass_) c.items.clear();
-    lam_begin_.assign(count() + 1, 0);
+    std::vector<std::vector<int>> dims(count());
     for (int i = 0; i < count(); i++) {
-        const Layout& L = structs_[inst_sid_[i]]->L;
-        inst_[i].block_first = gblock;
-        lam_begin_[i] = lam;
-        int nb = 0;
-        for (const Block& b : L.blocks) {
-            if (b.dim == 0) continue;
-            const int k = proj_class(b.dim);
-            // the Jacobi kernels give each rotation pair one thread
-            if (kClassTri[k] < 0 && (b.dim + 1) / 2 > kClassThreads[k])
-                throw std::invalid_argument("PSD block larger than 2048 is not supported");
-            ProjItem it{};
-            it.inst = i;
-            it.d = b.dim;
-            it.gblock = gblock++;
-            it.off = b.off;
-            if (kClassTri[k] >= 0) {
-                it.work_off = work;
-                work += tri_work(kClassTri[k], b.dim);
-                it.lam_off = lam;
-                lam += b.dim + 1;
-            } else {
-                int dp = b.dim + (b.dim & 1);
-                it.rec_off = rec;
-                rec += (int64_t)max_sweeps_ * (dp - 1) * (dp / 2) * 2;
-                if (k == kGlobalClass) {
-                    it.mat_off = mat;
-                    mat += (int64_t)dp * (dp + 1) / 2;
-                }
-            }
-            pclass_[k].items.push_back(it);
-            nb++;
+        int64_t off = 0;
+        for (const Block& b : structs_[inst_sid_[i]]->L.blocks) {
+            if (b.off != off) throw std::logic_error("PSD blocks are not contiguous in the primal vector");
+            off += (int64_t)b.dim * b.dim;
+            dims[i].push_back(b.dim);
         }
-        inst_[i].nblocks = nb;
     }
-    lam_begin_[count()] = lam;
-    size_t lds_max = 0;
-    for (int k = 0; k < kProjClasses; k++) {
-        auto& c = pclass_[k];
-        c.nt = kClassThreads[k];
-        c.global = k == kGlobalClass;
-        c.tri = kClassTri[k];
-        std::stable_sort(c.items.begin(), c.items.end(), [](const ProjItem& a, const ProjItem& b) { return a.d > b.d; });
-        c.max_d = c.items.empty() ? 0 : c.items.front().d;
-        if (!c.items.empty() && c.tri < 0) {
-            size_t lds = project_lds_bytes(c.max_d, c.nt, c.global);
-            if (lds > kLdsLimit) throw std::invalid_argument("PSD block does not fit in LDS");
-            lds_max = std::max(lds_max, lds);
-        }
-        if (!c.items.empty() && c.tri >= 0 && c.max_d > project_tri_max_d(c.tri))
-            throw std::logic_error("projection class table is inconsistent");
-        c.d_items.upload(c.items);
-    }
-    if (lds_max > 64 * 1024) set_project_lds_limit(lds_max);
-    proj_part_.alloc(2 * (size_t)std::max(gblock, 1));
-    rec_.alloc(std::max<int64_t>(rec, 1));
-    gmat_.alloc(std::max<int64_t>(mat, 1));
-    work_.alloc(std::max<int64_t>(work, 1));
-    lam_.alloc(std::max<int64_t>(lam, 1));
-    launch_fill(lam_.p, std::max<int64_t>(lam, 1), std::nan(""), stream_);
+    proj_.build(dims, inst_, stream_);
 }
 
 BatchDev Batch::dev() const {
@@ -297,7 +224,7 @@ BatchDev Batch::dev() const {
     B.x = x_.p; B.z = z_.p; B.c = c_.p; B.v1 = v1_.p; B.u = u_.p; B.aty = aty_.p; B.q = q_.p;
     B.y = y_.p; B.r = r_.p; B.p = p_.p; B.w = w_.p; B.av1 = av1_.p; B.ax = ax_.p; B.aaty = aaty_.p;
     B.partials = partials_.p;
-    B.proj_part = proj_part_.p;
+    B.proj_part = proj_.proj_part();
     B.opt = opt_;
     return B;
 }
@@ -434,9 +361,7 @@ void Batch::reset_state(int i) {
     launch_fill(z_.p + I.poff, S.n_primal, 0.0, stream_);
     launch_fill(aty_.p + I.poff, S.n_primal, 0.0, stream_);
     launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);
-    // forget the eigenvalues that warm-start the tridiagonal projection
-    if (lam_begin_[i + 1] > lam_begin_[i])
-        launch_fill(lam_.p + lam_begin_[i], lam_begin_[i + 1] - lam_begin_[i], std::nan(""), stream_);
+    proj_.forget(i, stream_);  // the eigenvalues that warm-start the tridiagonal projection
     if (constructed_) sync_state_to_host();
     InstState& st = state_[i];
     double bn = st.bnorm, cn = st.cnorm;
@@ -524,15 +449,10 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         tstop(3);
         launches_++;
         for (int k = 0; k < kProjClasses; k++) {
-            auto& c = pclass_[k];
-            if (c.items.empty()) continue;
+            if (!proj_.has(k)) continue;
             const int kind = 4 + k;
             tstart(kind);
-            if (c.tri >= 0)
-                launch_project_tri(c.tri, B, c.d_items.p, (int)c.items.size(), work_.p, lam_.p, stream_);
-            else
-                launch_project(c.nt, c.global, B, c.d_items.p, (int)c.items.size(), c.max_d, rec_.p, max_sweeps_,
-                               gmat_.p, stream_);
+            proj_.launch(k, B, stream_);
             tstop(kind);
             launches_++;
         }

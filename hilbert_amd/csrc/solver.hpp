@@ -12,31 +12,14 @@
 #include <vector>
 
 #include "common.hpp"
+#include "dbuf.hpp"
 #include "constraints.hpp"
 #include "layout.hpp"
+#include "projector.hpp"
 #include "sell.hpp"
 
 namespace v2h {
 
-// Device buffer owned by the solver.
-template <class T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { if (p) (void)hipFree(p); }
-    void alloc(size_t count) {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        n = count;
-        if (count) V2H_HIP(hipMalloc(&p, sizeof(T) * count));
-    }
-    void upload(const std::vector<T>& v) {
-        alloc(v.size());
-        if (!v.empty()) V2H_HIP(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-    }
-};
 
 // Gather tables and maps of one problem shape, resident in HBM.
 struct Structure {
@@ -106,7 +89,7 @@ class Batch {
     // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..9 projection classes: Jacobi d <= 16,
     // tridiagonal d <= 64 / 128 / 192 / 256, Jacobi d > 256 (matrix in global
     // memory).
-    static constexpr int kProjClasses = 6;
+    static constexpr int kProjClasses = Projector::kClasses;
     static constexpr int kKinds = 4 + kProjClasses;
     void set_profiling(bool on) { profile_ = on; }
     // ms/counts: all launches; full_ms/full_counts: launches made while every
@@ -115,8 +98,7 @@ class Batch {
     void kernel_times(double* ms, int64_t* counts, double* full_ms, int64_t* full_counts);
     void reset_kernel_times();
     void launch_stats(int64_t* st) const {
-        int64_t nblk = 0;
-        for (const auto& c : pclass_) nblk += (int64_t)c.items.size();
+        const int64_t nblk = proj_.nblocks();
         st[0] = launches_;
         st[1] = nwgA_;
         st[2] = nwgAT_;
@@ -146,23 +128,13 @@ class Batch {
     DBuf<InstDev> d_inst_;
     DBuf<InstState> d_state_;
     DBuf<double> x_, z_, c_, v1_, u_, aty_, q_, y_, r_, p_, w_, av1_, ax_, aaty_;
-    DBuf<double> src_, partials_, proj_part_, rec_, scratch_;
+    DBuf<double> src_, partials_, scratch_;
     DBuf<int4> unitsA_, unitsAT_;
     DBuf<int> wgA_inst_, wgAT_inst_, wgA_begin_, wgAT_begin_;
     int nwgA_ = 0, nwgAT_ = 0;
     std::vector<int> hwgA_begin_, hwgAT_begin_;
 
-    struct ProjClass {
-        int nt = 0, max_d = 0;
-        bool global = false;
-        int tri = -1;               // tridiagonal kernel class, or -1 for Jacobi
-        std::vector<ProjItem> items;
-        DBuf<ProjItem> d_items;
-    };
-    ProjClass pclass_[kProjClasses];
-    DBuf<double> gmat_, work_, lam_;
-    std::vector<int64_t> lam_begin_;  // [count + 1] eigenvalue-store range of each instance
-    int max_sweeps_ = 24;
+    Projector proj_;
 
     struct KTimer {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
