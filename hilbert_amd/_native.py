@@ -92,6 +92,18 @@ SIGNATURES = {
     "v2h_batch_kernel_times": (C.c_int, [_vp, _dp, _lp, _dp, _lp]),
     "v2h_batch_reset_kernel_times": (C.c_int, [_vp]),
     "v2h_batch_launch_stats": (C.c_int, [_vp, _lp]),
+    "v2h_sdp_create": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(SdpOptions), C.c_int, C.POINTER(_vp)]),
+    "v2h_sdp_destroy": (C.c_int, [_vp]),
+    "v2h_sdp_set_options": (C.c_int, [_vp, C.POINTER(SdpOptions)]),
+    "v2h_sdp_solve": (C.c_int, [_vp, _dp, _dp, _dp, _ip, C.c_int, C.c_int, _vp, _vp, _vp, _vp,
+                                C.POINTER(C.c_int)]),
+    "v2h_sdp_status": (C.c_int, [_vp, C.POINTER(Status)]),
+    "v2h_sdp_get_y": (C.c_int, [_vp, _dp]),
+    "v2h_sdp_get_z": (C.c_int, [_vp, _dp]),
+    "v2h_sdp_set_y": (C.c_int, [_vp, _dp]),
+    "v2h_sdp_set_z": (C.c_int, [_vp, _dp]),
+    "v2h_sdp_get_mu": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "v2h_sdp_set_mu": (C.c_int, [_vp, C.c_double]),
     "v2h_structure_create": (C.c_int, [C.POINTER(ProblemDesc), C.POINTER(_vp)]),
     "v2h_structure_destroy": (C.c_int, [_vp]),
     "v2h_structure_sizes": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),

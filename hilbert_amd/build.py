@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libv2rdm_hip.so")
-SOURCES = ["layout.cpp", "constraints.cpp", "sell.cpp", "rdm.cpp", "projector.cpp", "solver.cpp", "abi.cpp",
+SOURCES = ["layout.cpp", "constraints.cpp", "sell.cpp", "rdm.cpp", "projector.cpp", "solver.cpp", "sdp_host.cpp",
+           "abi.cpp",
            "kernels.hip",
            "eig_tri.hip"]
 ARCH = os.environ.get("V2H_OFFLOAD_ARCH", "gfx950")

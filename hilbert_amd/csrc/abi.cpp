@@ -2,6 +2,8 @@
 // caught here and turned into status codes plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -9,6 +11,7 @@
 
 #include "../../include/v2rdm_hip.h"
 #include "rdm.hpp"
+#include "sdp_host.hpp"
 #include "solver.hpp"
 
 struct v2h_batch {
@@ -395,6 +398,108 @@ int v2h_batch_cost_adjoint(v2h_batch* b, int inst, const double* xbar, double* d
 }  // extern "C"
 
 // ---- host-only structure inspection -------------------------------------
+
+struct v2h_sdp {
+    std::unique_ptr<v2h::CallbackSdp> impl;
+};
+
+namespace {
+v2h::CallbackSdp& S(v2h_sdp* s) {
+    if (!s || !s->impl) throw std::invalid_argument("null solver handle");
+    return *s->impl;
+}
+const v2h::CallbackSdp& S(const v2h_sdp* s) {
+    if (!s || !s->impl) throw std::invalid_argument("null solver handle");
+    return *s->impl;
+}
+}  // namespace
+
+extern "C" {
+
+int v2h_sdp_create(int64_t n_primal, int64_t n_dual, const v2h_sdp_options* opt, int device, v2h_sdp** out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null output handle");
+        *out = nullptr;
+        v2h_sdp_options o;
+        v2h_default_options(&o);
+        auto h = std::make_unique<v2h_sdp>();
+        h->impl = std::make_unique<v2h::CallbackSdp>(n_primal, n_dual, to_opts(opt ? opt : &o), device);
+        *out = h.release();
+    });
+}
+
+int v2h_sdp_destroy(v2h_sdp* s) {
+    return guarded([&] { delete s; });
+}
+
+int v2h_sdp_set_options(v2h_sdp* s, const v2h_sdp_options* opt) {
+    return guarded([&] {
+        if (!opt) throw std::invalid_argument("null options");
+        S(s).set_options(to_opts(opt));
+    });
+}
+
+int v2h_sdp_solve(v2h_sdp* s, double* x, const double* b, const double* c, const int* block_dims, int n_blocks,
+                  int maxiter, v2h_map_fn Au, v2h_map_fn ATu, v2h_monitor_fn mon, void* data, int* converged) {
+    return guarded([&] {
+        int cv = S(s).solve(x, b, c, block_dims, n_blocks, maxiter, Au, ATu, mon, data);
+        if (converged) *converged = cv;
+    });
+}
+
+int v2h_sdp_status(const v2h_sdp* s, v2h_status* st) {
+    return guarded([&] {
+        if (!st) throw std::invalid_argument("null status");
+        const auto& q = S(s);
+        st->mu = q.mu();
+        st->energy_primal = q.energy_primal;
+        st->energy_dual = q.energy_dual;
+        st->primal_error = q.eps_p;
+        st->dual_error = q.eps_d;
+        st->oiter_total = q.oiter;
+        st->iiter_total = q.iiter;
+        st->converged = q.converged;
+        st->active = 0;
+        st->status_bits = q.status;
+        st->last_cg_iterations = q.last_cg;
+    });
+}
+
+int v2h_sdp_get_y(const v2h_sdp* s, double* y) {
+    return guarded([&] {
+        if (!y) throw std::invalid_argument("null argument");
+        std::copy(S(s).y().begin(), S(s).y().end(), y);
+    });
+}
+int v2h_sdp_get_z(const v2h_sdp* s, double* z) {
+    return guarded([&] {
+        if (!z) throw std::invalid_argument("null argument");
+        std::copy(S(s).z().begin(), S(s).z().end(), z);
+    });
+}
+int v2h_sdp_set_y(v2h_sdp* s, const double* y) {
+    return guarded([&] {
+        if (!y) throw std::invalid_argument("null argument");
+        S(s).set_y(y);
+    });
+}
+int v2h_sdp_set_z(v2h_sdp* s, const double* z) {
+    return guarded([&] {
+        if (!z) throw std::invalid_argument("null argument");
+        S(s).set_z(z);
+    });
+}
+int v2h_sdp_get_mu(const v2h_sdp* s, double* mu) {
+    return guarded([&] {
+        if (!mu) throw std::invalid_argument("null argument");
+        *mu = S(s).mu();
+    });
+}
+int v2h_sdp_set_mu(v2h_sdp* s, double mu) {
+    return guarded([&] { S(s).set_mu(mu); });
+}
+
+}  // extern "C"
 
 struct v2h_structure {
     v2h::Layout L;

@@ -167,6 +167,33 @@ int v2h_batch_launch_stats(v2h_batch *b, int64_t *stats);
  * for E = c . xbar; xbar = NULL uses the current primal solution x. */
 int v2h_batch_cost_adjoint(v2h_batch *b, int inst, const double *xbar, double *dh, double *dg);
 
+/* ---- BPSDP with caller-supplied maps ----
+ * The generic libsdp contract: BPSDPSolver(n_primal, n_dual, options)
+ * (v2rdm_solver.cc:433) and solve(x, b, c, block_dims, maxiter, Au, ATu,
+ * monitor, print_level, data) (v2rdm_solver.cc:1340), with the callback types
+ * of v2rdm_solver.cc:97-124: Au writes out[n_dual] = A in, ATu writes
+ * out[n_primal] = A^T in, both on host memory.  For solvers whose maps are not
+ * the built-in DQG ones (v2RDM-DOCI, other constraint families).  The CG
+ * vectors live on the host beside the maps; the PSD projection of the blocks
+ * (sizes in block_dims, row-major and contiguous in x) runs on the GPU.
+ * y, z and mu persist across solve calls, as libsdp's do. */
+typedef void (*v2h_map_fn)(double *out, double *in, void *data);
+typedef struct v2h_sdp v2h_sdp;
+int v2h_sdp_create(int64_t n_primal, int64_t n_dual, const v2h_sdp_options *opt, int device, v2h_sdp **out);
+int v2h_sdp_destroy(v2h_sdp *s);
+int v2h_sdp_set_options(v2h_sdp *s, const v2h_sdp_options *opt);
+/* Up to maxiter outer iterations from x, which receives the result;
+ * *converged = libsdp is_converged() (v2rdm_solver.cc:1361). */
+int v2h_sdp_solve(v2h_sdp *s, double *x, const double *b, const double *c, const int *block_dims, int n_blocks,
+                  int maxiter, v2h_map_fn Au, v2h_map_fn ATu, v2h_monitor_fn mon, void *data, int *converged);
+int v2h_sdp_status(const v2h_sdp *s, v2h_status *st);
+int v2h_sdp_get_y(const v2h_sdp *s, double *y);   /* libsdp get_y (checkpoint.cc:58-64) */
+int v2h_sdp_get_z(const v2h_sdp *s, double *z);   /* libsdp get_z */
+int v2h_sdp_set_y(v2h_sdp *s, const double *y);   /* libsdp set_y (checkpoint.cc:106-120) */
+int v2h_sdp_set_z(v2h_sdp *s, const double *z);   /* libsdp set_z */
+int v2h_sdp_get_mu(const v2h_sdp *s, double *mu); /* libsdp get_mu */
+int v2h_sdp_set_mu(v2h_sdp *s, double mu);        /* libsdp set_mu */
+
 /* ---- host-only inspection of one problem structure (no GPU needed) ----
  * The tables the device path is built from: geminal tables
  * (build_mapping_arrays, basis.cc:427-533), the constraint matrix A with b in
