@@ -6,8 +6,10 @@ differentiable energy (torch.autograd.Function over the adjoint kernel).
   lists element by element and with the contraction sum_j Gamma[i,k,j,j] =
   (N - 1) D1[i, k];
 * dE/dh and dE/d(pq|rs) from the adjoint agree with central finite
-  differences of tightly converged solves (the envelope theorem holds only at
-  the optimum, so both sides are converged to 1e-10 / 1e-9), to 1e-6.
+  differences to 1e-6.  The envelope theorem holds at the optimum, so the
+  solves are converged to 1e-10 / 1e-9; the step 2e-3 keeps the solve noise
+  (about 1e-9 Eh / step) and the O(step^2) truncation both near 5e-7, which
+  is what the oracle shows on the same Hamiltonian.
 """
 import json
 import os
@@ -41,7 +43,7 @@ def test_helper_hubbard8_matches_golden():
     lst = hp.get_tpdm_sparse("SUM")
     assert len(lst) == n ** 4
     worst = max(abs(D2[t.i * n + t.k, t.j * n + t.l] - t.value) for t in lst)
-    assert worst == 0.0
+    assert worst < 1e-14  # the same four terms, summed in another order
     G = D2.reshape(n, n, n, n)
     assert np.abs(np.einsum("ikjj->ik", G) - (8 - 1) * d1).max() < 1e-4
     ab = {(t.i, t.j, t.k, t.l): t.value for t in hp.get_tpdm_sparse("AB")}
@@ -56,12 +58,12 @@ def test_adjoint_matches_finite_differences():
     rng = np.random.default_rng(5)
     n, na, nb = 4, 2, 2
     h0, g0 = synthetic_integrals(n, 12, rng)
-    opts = {"MAXITER": 100000, "E_CONVERGENCE": 1e-10, "R_CONVERGENCE": 1e-9}
+    opts = {"MAXITER": 200000, "E_CONVERGENCE": 1e-10, "R_CONVERGENCE": 1e-9}
     h = torch.tensor(h0, requires_grad=True)
     g = torch.tensor(g0, requires_grad=True)
     E = v2rdm_energy(h, g, na, nb, opts)
     E.backward()
-    step = 1e-4
+    step = 2e-3
 
     def energy(hh, gg):
         return float(v2rdm_energy(torch.tensor(hh), torch.tensor(gg), na, nb, opts))

@@ -4,7 +4,9 @@
   mixed batch (C1, 2/4/8 irreps, singlet / doublet / triplet);
 * cost packing bit-exact; its adjoint to 1e-13;
 * full solves at e_convergence 1e-7 / r_convergence 1e-6 (both sides):
-  total energy within 1e-6 Eh of the oracle, 1-RDM within 1e-5;
+  total energy within 1e-6 Eh of the oracle; at 1e-9 / 1e-8 (where the
+  RDMs are determined well enough to compare): energy within 1e-8 Eh, 1-RDM
+  and <S^2> within 1e-5;
 * full solves at the default 1e-6 / 1e-5: the stopping criteria hold and
   the energy lies inside the band those thresholds allow (5e-5 Eh); CG on the
   singular A A^T is chaotic in finite precision, so two correct
@@ -95,6 +97,11 @@ SOLVE_CASES = [
 
 
 TIGHT = dict(e_convergence=1e-7, r_convergence=1e-6)
+# RDM elements are only determined to the stopping tolerance divided by the
+# conditioning of the optimal face: on c2v_n6 the oracle's own 1-RDM moves by
+# 1.8e-5 between its LAPACK and Jacobi projections at TIGHT, and by 1.7e-7 at
+# TIGHTER, so the 1e-5 RDM comparison runs at TIGHTER.
+TIGHTER = dict(e_convergence=1e-9, r_convergence=1e-8)
 
 
 def _oracle_solve(prob, h, g, **kw):
@@ -122,6 +129,18 @@ def test_solve_matches_oracle(name, make):
     assert abs(st["energy_primal"] - ref["energy_primal"]) < 1e-6
     assert abs(st["energy_primal"] - st["energy_dual"]) < 1e-7
     assert st["eps_p"] < 1e-6 and st["eps_d"] < 1e-6
+    B.close()
+
+
+@pytest.mark.parametrize("name,make", SOLVE_CASES)
+def test_rdms_match_oracle(name, make):
+    prob, h, g = make()
+    P, ref = _oracle_solve(prob, h, g, **TIGHTER)
+    assert ref["converged"]
+    B = _gpu_solve(prob, h, g, **TIGHTER)
+    st = B.status(0)
+    assert st["converged"] and st["status_bits"] == 0
+    assert abs(st["energy_primal"] - ref["energy_primal"]) < 1e-8
     x = B.x(0)
     assert np.abs(P.opdm(x) - P.opdm(ref["x"])).max() < 1e-5
     assert abs(P.spin_squared(x) - P.spin_squared(ref["x"])) < 1e-5
