@@ -319,6 +319,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     static_assert(PR * RI == 32 * CJ, "square register tiling");
     __shared__ double colb[2][DMAX];
     __shared__ double pbuf[DMAX];
+    __shared__ double vbuf[DMAX], wbuf[DMAX];  // rank-2 update not yet applied to the tile
     __shared__ double dd[DMAX], ee[DMAX], e2[DMAX], tau[DMAX], lamv[DMAX], xs[DMAX];
     __shared__ double stage[kStage][DMAX];
     __shared__ double red[NT / 64];
@@ -338,6 +339,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     double* lprev = lamst + it.lam_off;     // d eigenvalues + last largest change
 
     for (int i = t; i < 2 * DMAX; i += NT) (&colb[0][0])[i] = 0.0;
+    for (int i = t; i < DMAX; i += NT) vbuf[i] = wbuf[i] = 0.0;
     Tile<NT, RI, CJ, GM> A;
     if constexpr (GM) A.p = fac + 5 * (size_t)d * d;  // NT RI CJ doubles after the factors
 #pragma unroll
@@ -408,7 +410,56 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
         }
     };
 
+    // One pass over the tile: apply the pending rank-2 update
+    // A -= v w^T + w v^T (v, w in vbuf / wbuf, zero where it does not apply)
+    // and accumulate A v_new; pbuf[i] = tk (A v_new)_i for every row i < DMAX.
+    auto fused = [&](const double (&vn)[CJ], double tk) {
+        double vo[CJ], wo[CJ];
+#pragma unroll
+        for (int c = 0; c < CJ; c++) {
+            vo[c] = vbuf[tc + 32 * c];
+            wo[c] = wbuf[tc + 32 * c];
+        }
+        double s8[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            double sr = 0.0;
+            if (r < RI) {
+                const int i = tr + PR * r;
+                const double vi = vbuf[i], wi = wbuf[i];
+#pragma unroll
+                for (int c = 0; c < CJ; c++) {
+                    const double a = fma(-wi, vo[c], fma(-vi, wo[c], A.at(r, c)));
+                    A.at(r, c) = a;
+                    sr = fma(a, vn[c], sr);
+                }
+            }
+            s8[r] = sr;
+            row_fence();
+        }
+        const double ps = reduce_scatter8(s8);
+        if ((lane & 3) == 0 && rr < RI) pbuf[tr + PR * rr] = tk * ps;
+    };
+    // apply the pending update without a product
+    auto apply_pending = [&]() {
+        double vo[CJ], wo[CJ];
+#pragma unroll
+        for (int c = 0; c < CJ; c++) {
+            vo[c] = vbuf[tc + 32 * c];
+            wo[c] = wbuf[tc + 32 * c];
+        }
+#pragma unroll
+        for (int r = 0; r < RI; r++) {
+            const int i = tr + PR * r;
+            const double vi = vbuf[i], wi = wbuf[i];
+#pragma unroll
+            for (int c = 0; c < CJ; c++) A.at(r, c) = fma(-wi, vo[c], fma(-vi, wo[c], A.at(r, c)));
+            row_fence();
+        }
+    };
+
     // ---------------------------------------------------------------- U = Q T Q^T
+    if constexpr (!GM) {
     put_col(0, 0);
     for (int k = 0; k + 2 < d; k++) {
         lds_sync();
@@ -450,6 +501,81 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     if (t == 0 && d >= 2) ee[d - 2] = colb[(d - 2) & 1][d - 1];
     __syncthreads();
 
+    } else {
+    // Step k reduces column k.  The rank-2 update of step k is applied to the
+    // tile during the product of step k + 1 (one pass per step); only the
+    // owners of column k + 1 apply it at once, to publish that column.
+    put_col(0, 0);
+    for (int k = 0; k + 2 < d; k++) {
+        lds_sync();  // column k, the pending update of step k - 1
+        const double* col = colb[k & 1];
+        double sg = 0.0;
+        for (int i = k + 2 + lane; i < DMAX; i += 64) sg = fma(col[i], col[i], sg);  // zero beyond d
+        sg = wsum(sg);
+        const double alpha = col[k + 1];
+        double tk = 0.0, sc = 0.0, beta = alpha;
+        if (sg != 0.0) {
+            beta = -copysign(sqrt(fma(alpha, alpha, sg)), alpha);
+            tk = (beta - alpha) / beta;
+            sc = 1.0 / (alpha - beta);
+        }
+        if (t == 0) {
+            ee[k] = beta;
+            tau[k] = tk;
+        }
+        // v_k = (0.., 1, col[k+2..] sc), zero when no reflection is needed
+        auto vk = [&](int i) {
+            const double ci = col[i];
+            return tk == 0.0 ? 0.0 : (i == k + 1 ? 1.0 : ((i > k + 1 && i < d) ? ci * sc : 0.0));
+        };
+        if (tk != 0.0)
+            for (int i = k + 1 + t; i < d; i += NT) hv[(size_t)k * d + i] = vk(i);
+        double vn[CJ];
+#pragma unroll
+        for (int c = 0; c < CJ; c++) vn[c] = vk(tc + 32 * c);
+        fused(vn, tk);
+        lds_sync();  // pbuf; every wavefront is done with the old vbuf / wbuf
+        double kk = 0.0;
+        for (int i = k + 1 + lane; i < d; i += 64) kk = fma(vk(i), pbuf[i], kk);
+        kk = 0.5 * tk * wsum(kk);
+        auto wk = [&](int i) { return (i > k && i < d) ? fma(-kk, vk(i), pbuf[i]) : 0.0; };
+        for (int i = t; i < DMAX; i += NT) {
+            vbuf[i] = vk(i);
+            wbuf[i] = wk(i);
+        }
+        // column k + 1 of the reduced matrix, from the owners' tile
+        if (tc == ((k + 1) & 31)) {
+            const int ck = (k + 1) >> 5;
+            const double vj = vk(k + 1), wj = wk(k + 1);
+#pragma unroll
+            for (int r = 0; r < RI; r++) {
+                const int i = tr + PR * r;
+                if (i > k && i < d) {
+                    const double a = fma(-wk(i), vj, fma(-vk(i), wj, A.col(r, ck)));
+                    if (i > k + 1) colb[(k + 1) & 1][i] = a;
+                    else dd[k + 1] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();  // last column, pending update, reflector stores
+    if (d >= 2) {
+        // the last diagonal element, with the pending update applied
+        const int k = d - 1;
+        if (tc == (k & 31)) {
+            const int ck = k >> 5;
+#pragma unroll
+            for (int r = 0; r < RI; r++)
+                if (tr + PR * r == k) {
+                    const double v = vbuf[k], w = wbuf[k];
+                    dd[k] = fma(-w, v, fma(-v, w, A.col(r, ck)));
+                }
+        }
+        if (t == 0) ee[d - 2] = colb[(d - 2) & 1][d - 1];
+    }
+    __syncthreads();
+
+    }
     // ---------------------------------------------------------------- T = S L S^T
     double glo = INFINITY, ghi = -INFINITY, em = 0.0;
     if (t < d) {
@@ -677,6 +803,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     }
 
     // ---------------------------------------------------------------- U+ = Q T+ Q^T
+    if constexpr (!GM) {
     if (d >= 3) {
         auto load_v = [&](int k, int buf) {
             for (int i = t; i < DMAX; i += NT) colb[buf][i] = (i > k && i < d) ? hv[(size_t)k * d + i] : 0.0;
@@ -701,6 +828,41 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
         }
     }
 
+    } else {
+    // M <- H_k M H_k for k = d-3 .. 0; as in the reduction, each step's
+    // rank-2 update is applied during the next step's product.
+    if (d >= 3) {
+        auto load_v = [&](int k, int buf) {
+            for (int i = t; i < DMAX; i += NT) colb[buf][i] = (i > k && i < d) ? hv[(size_t)k * d + i] : 0.0;
+        };
+        lds_sync();  // the tile's last readers of vbuf / wbuf are done
+        for (int i = t; i < DMAX; i += NT) vbuf[i] = wbuf[i] = 0.0;
+        load_v(d - 3, (d - 3) & 1);
+        for (int k = d - 3; k >= 0; k--) {
+            lds_sync();  // v_k, the pending update
+            const double tk = tau[k];
+            if (k > 0) load_v(k - 1, (k - 1) & 1);
+            if (tk != 0.0) {
+                const double* v = colb[k & 1];
+                double vn[CJ];
+#pragma unroll
+                for (int c = 0; c < CJ; c++) vn[c] = v[tc + 32 * c];
+                fused(vn, tk);
+                lds_sync();
+                double kk = 0.0;
+                for (int i = k + 1 + lane; i < d; i += 64) kk = fma(v[i], pbuf[i], kk);
+                kk = 0.5 * tk * wsum(kk);
+                for (int i = t; i < DMAX; i += NT) {
+                    vbuf[i] = v[i];
+                    wbuf[i] = i < d ? fma(-kk, v[i], pbuf[i]) : 0.0;
+                }
+            }
+        }
+        lds_sync();
+        apply_pending();
+    }
+
+    }
     // ---------------------------------------------------------------- BPSDP block update
     double ex = 0.0, dr = 0.0;
     double* X = B.x + I.poff + it.off;

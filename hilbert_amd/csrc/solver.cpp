@@ -98,6 +98,11 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     if (descs.empty()) throw std::invalid_argument("empty batch");
     V2H_HIP(hipSetDevice(device_));
     V2H_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (int k = 0; k < kProjClasses; k++) {
+        V2H_HIP(hipStreamCreateWithFlags(&pstream_[k], hipStreamNonBlocking));
+        V2H_HIP(hipEventCreateWithFlags(&ev_proj_[k], hipEventDisableTiming));
+    }
+    V2H_HIP(hipEventCreateWithFlags(&ev_post_, hipEventDisableTiming));
 
     std::map<std::string, int> sid_of;
     for (const auto& d : descs) {
@@ -176,6 +181,11 @@ Batch::~Batch() {
             (void)hipEventDestroy(e.second);
         }
     if (pinned_) (void)hipHostFree(pinned_);
+    for (int k = 0; k < kProjClasses; k++) {
+        if (pstream_[k]) (void)hipStreamDestroy(pstream_[k]);
+        if (ev_proj_[k]) (void)hipEventDestroy(ev_proj_[k]);
+    }
+    if (ev_post_) (void)hipEventDestroy(ev_post_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -236,8 +246,9 @@ void Batch::sync_state_to_host() {
     tflush();
 }
 
-void Batch::tstart(int k) {
+void Batch::tstart(int k, hipStream_t s) {
     if (!profile_) return;
+    if (!s) s = stream_;
     KTimer& t = timers_[k];
     if (t.used == t.pool.size()) {
         hipEvent_t a, b;
@@ -247,13 +258,14 @@ void Batch::tstart(int k) {
         t.full.push_back(0);
     }
     t.full[t.used] = (k == 1 || k == 2) ? all_iterating_ : all_running_;
-    V2H_HIP(hipEventRecord(t.pool[t.used].first, stream_));
+    V2H_HIP(hipEventRecord(t.pool[t.used].first, s));
 }
 
-void Batch::tstop(int k) {
+void Batch::tstop(int k, hipStream_t s) {
     if (!profile_) return;
+    if (!s) s = stream_;
     KTimer& t = timers_[k];
-    V2H_HIP(hipEventRecord(t.pool[t.used].second, stream_));
+    V2H_HIP(hipEventRecord(t.pool[t.used].second, s));
     t.used++;
 }
 
@@ -448,12 +460,17 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         launch_post(B, UAT, stream_);
         tstop(3);
         launches_++;
-        for (int k = 0; k < kProjClasses; k++) {
+        // the size classes are independent: largest first, each on its stream
+        V2H_HIP(hipEventRecord(ev_post_, stream_));
+        for (int k = kProjClasses - 1; k >= 0; k--) {
             if (!proj_.has(k)) continue;
             const int kind = 4 + k;
-            tstart(kind);
-            proj_.launch(k, B, stream_);
-            tstop(kind);
+            V2H_HIP(hipStreamWaitEvent(pstream_[k], ev_post_, 0));
+            tstart(kind, pstream_[k]);
+            proj_.launch(k, B, pstream_[k]);
+            tstop(kind, pstream_[k]);
+            V2H_HIP(hipEventRecord(ev_proj_[k], pstream_[k]));
+            V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[k], 0));
             launches_++;
         }
         V2H_HIP(hipGetLastError());

@@ -135,6 +135,10 @@ class Batch {
     std::vector<int> hwgA_begin_, hwgAT_begin_;
 
     Projector proj_;
+    // the projection classes run concurrently, each on its own stream, joined
+    // back into stream_ by events
+    hipStream_t pstream_[kProjClasses] = {};
+    hipEvent_t ev_post_ = nullptr, ev_proj_[kProjClasses] = {};
 
     struct KTimer {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
@@ -146,8 +150,8 @@ class Batch {
     bool all_running_ = false, all_iterating_ = false;
     KTimer timers_[kKinds];
     bool profile_ = false;
-    void tstart(int k);
-    void tstop(int k);
+    void tstart(int k, hipStream_t s = nullptr);
+    void tstop(int k, hipStream_t s = nullptr);
     void tflush();
 
     InstState* pinned_ = nullptr;
