@@ -39,7 +39,8 @@ def bytes_per_instance(kind, n_p, n_d):
 
 
 PROJ_CLASSES = (("project_jacobi16", 0, 16), ("project_tri64", 17, 64), ("project_tri128", 65, 128),
-                ("project_tri192", 129, 192), ("project_tri256", 193, 256), ("project_jacobi_big", 257, 1 << 30))
+                ("project_tri192", 129, 192), ("project_tri224", 193, 224), ("project_tri256", 225, 256),
+                ("project_jacobi_big", 257, 1 << 30))
 
 
 def project_flops(dims):

@@ -33,7 +33,7 @@ constexpr double kEps = DBL_EPSILON;
 // share a cluster; its eigenvectors are orthogonalised against each other.
 constexpr double kClusterRel = 1e-5;
 constexpr double kClusterAbs = 1e-11;
-constexpr int kStage = 16;       // eigenvectors per LDS chunk when forming T+
+constexpr int kStageMax = 16;    // eigenvectors per LDS chunk when forming T+
 constexpr int kMaxPasses = 160;  // Sturm passes per eigenvalue (bisection needs about 60)
 constexpr int kChunk = 8;        // inverse-iteration loads issued ahead of their use
 
@@ -283,14 +283,19 @@ __device__ double orthogonalise(const Column& c, int d, const double* Sv, int l0
     return s2;
 }
 
-// The block's share of one thread: RI x CJ elements, in registers, or (GM)
-// in global memory laid out [wavefront][r][c][lane]: a wavefront's accesses
-// coalesce and its elements lie within a few immediate offsets of one base
-// (blocks too large for the register budget).
-template <int NT, int RI, int CJ, bool GM>
+// The block's share of one thread: RI x CJ elements.  Three homes:
+//   kTileReg   all in registers;
+//   kTileSplit the first kSplitRegs in registers, the rest in LDS laid out
+//              [slot][thread] (blocks just above the register budget);
+//   kTileGlobal in global memory laid out [wavefront][r][c][lane]: a
+//              wavefront's accesses coalesce (the largest blocks).
+constexpr int kTileReg = 0, kTileGlobal = 1, kTileSplit = 2;
+constexpr int kSplitRegs = 33;
+
+template <int NT, int RI, int CJ, int MODE>
 struct Tile;
 template <int NT, int RI, int CJ>
-struct Tile<NT, RI, CJ, false> {
+struct Tile<NT, RI, CJ, kTileReg> {
     double v[RI][CJ];
     __device__ __forceinline__ double& at(int r, int c) { return v[r][c]; }
     __device__ __forceinline__ double col(int r, int ck) {
@@ -302,7 +307,24 @@ struct Tile<NT, RI, CJ, false> {
     }
 };
 template <int NT, int RI, int CJ>
-struct Tile<NT, RI, CJ, true> {
+struct Tile<NT, RI, CJ, kTileSplit> {
+    double v[kSplitRegs];
+    double* lds;  // (RI CJ - kSplitRegs) x NT
+    // r, c are compile-time constants wherever the tile is swept (unrolled loops)
+    __device__ __forceinline__ double& at(int r, int c) {
+        const int e = r * CJ + c;
+        return e < kSplitRegs ? v[e] : lds[(e - kSplitRegs) * NT + threadIdx.x];
+    }
+    __device__ __forceinline__ double col(int r, int ck) {
+        double x = 0.0;
+#pragma unroll
+        for (int c = 0; c < CJ; c++)
+            if (c == ck) x = at(r, c);
+        return x;
+    }
+};
+template <int NT, int RI, int CJ>
+struct Tile<NT, RI, CJ, kTileGlobal> {
     double* p;
     __device__ __forceinline__ double& at(int r, int c) {
         return p[((size_t)(threadIdx.x >> 6) * RI * CJ + r * CJ + c) * 64 + (threadIdx.x & 63)];
@@ -310,10 +332,14 @@ struct Tile<NT, RI, CJ, true> {
     __device__ __forceinline__ double col(int r, int ck) { return at(r, ck); }
 };
 
-template <int NT, int RI, int CJ, bool GM>
+template <int NT, int RI, int CJ, int MODE>
 __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* items, double* work, double* lamst) {
+    constexpr bool GM = MODE == kTileGlobal;
     constexpr int PR = NT / 32;
     constexpr int DMAX = PR * RI;
+    constexpr int kStage = MODE == kTileSplit ? 4 : kStageMax;  // LDS left for staging eigenvectors
+    constexpr int NLDS = MODE == kTileSplit ? RI * CJ - kSplitRegs : 0;
+    static_assert(MODE != kTileSplit || NLDS > 0, "split tile needs an LDS part");
     static_assert(RI <= 8, "reduce_scatter8 handles at most 8 rows per thread");
     static_assert(DMAX <= NT, "one eigenvalue per thread");
     static_assert(PR * RI == 32 * CJ, "square register tiling");
@@ -323,6 +349,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     __shared__ double dd[DMAX], ee[DMAX], e2[DMAX], tau[DMAX], lamv[DMAX], xs[DMAX];
     __shared__ double stage[kStage][DMAX];
     __shared__ double red[NT / 64];
+    __shared__ double tile_lds[NLDS > 0 ? NLDS * NT : 1];
 
     const ProjItem it = items[blockIdx.x];
     InstState& st = B.st[it.inst];
@@ -340,8 +367,9 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
 
     for (int i = t; i < 2 * DMAX; i += NT) (&colb[0][0])[i] = 0.0;
     for (int i = t; i < DMAX; i += NT) vbuf[i] = wbuf[i] = 0.0;
-    Tile<NT, RI, CJ, GM> A;
+    Tile<NT, RI, CJ, MODE> A;
     if constexpr (GM) A.p = fac + 5 * (size_t)d * d;  // NT RI CJ doubles after the factors
+    if constexpr (MODE == kTileSplit) A.lds = tile_lds;
 #pragma unroll
     for (int r = 0; r < RI; r++) {
         const int i = tr + PR * r;
@@ -911,7 +939,8 @@ int project_tri_max_d(int cls) {
         case 0: return 64;
         case 1: return 128;
         case 2: return 192;
-        case 3: return 256;
+        case 3: return 224;
+        case 4: return 256;
     }
     return 0;
 }
@@ -921,18 +950,22 @@ void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int n
     if (nitems <= 0) return;
     switch (cls) {
         case 0:
-            hipLaunchKernelGGL((k_project_tri<256, 8, 2, false>), dim3(nitems), dim3(256), 0, s, B, items, work, lam);
+            hipLaunchKernelGGL((k_project_tri<256, 8, 2, kTileReg>), dim3(nitems), dim3(256), 0, s, B, items, work, lam);
             break;
         case 1:
-            hipLaunchKernelGGL((k_project_tri<512, 8, 4, false>), dim3(nitems), dim3(512), 0, s, B, items, work, lam);
+            hipLaunchKernelGGL((k_project_tri<512, 8, 4, kTileReg>), dim3(nitems), dim3(512), 0, s, B, items, work, lam);
             break;
         case 2:
-            hipLaunchKernelGGL((k_project_tri<768, 8, 6, false>), dim3(nitems), dim3(768), 0, s, B, items, work,
+            hipLaunchKernelGGL((k_project_tri<768, 8, 6, kTileReg>), dim3(nitems), dim3(768), 0, s, B, items, work,
                                lam);
             break;
         case 3:
-            hipLaunchKernelGGL((k_project_tri<1024, 8, 8, true>), dim3(nitems), dim3(1024), 0, s, B, items, work,
-                               lam);
+            hipLaunchKernelGGL((k_project_tri<1024, 7, 7, kTileSplit>), dim3(nitems), dim3(1024), 0, s, B, items,
+                               work, lam);
+            break;
+        case 4:
+            hipLaunchKernelGGL((k_project_tri<1024, 8, 8, kTileGlobal>), dim3(nitems), dim3(1024), 0, s, B, items,
+                               work, lam);
             break;
         default:
             break;

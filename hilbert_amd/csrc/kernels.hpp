@@ -23,8 +23,8 @@ void launch_project(int nt, bool global_matrix, const BatchDev& B, const ProjIte
                     double* rec, int max_sweeps, double* gmat, hipStream_t s);
 void set_project_lds_limit(size_t bytes);
 
-// Tridiagonal projection (eig_tri.hip), classes 0/1/2/3 for blocks up to
-// 64 / 128 / 192 / 256.  work: 7 d^2 doubles per block (class 3: plus
+// Tridiagonal projection (eig_tri.hip), classes 0..4 for blocks up to
+// 64 / 128 / 192 / 224 / 256.  work: 7 d^2 doubles per block (class 4: plus
 // 1024 x 64 for the tile); lam: d + 1 doubles per block, kept between calls
 // (NaN = no previous eigenvalues).
 int project_tri_max_d(int cls);

@@ -11,24 +11,26 @@ namespace v2h {
 
 namespace {
 // Projection classes by block size.  Blocks of 17..256 take the tridiagonal
-// path (register-tiled Householder reduction up to 192, the tile in global
-// memory up to 256); the smallest blocks and those above 256 take parallel
-// Jacobi (packed matrix in LDS for d <= 16, in global memory beyond 256).
-constexpr int kGlobalClass = 5;
+// path (register tiles up to 192, registers + LDS up to 224, the tile in
+// global memory up to 256); the smallest blocks and those above 256 take
+// parallel Jacobi (packed matrix in LDS for d <= 16, in global memory beyond
+// 256).
+constexpr int kGlobalClass = 6;
 constexpr size_t kLdsLimit = 160 * 1024;
-constexpr int kClassThreads[Projector::kClasses] = {64, 256, 512, 768, 1024, 1024};
-constexpr int kClassTri[Projector::kClasses] = {-1, 0, 1, 2, 3, -1};
+constexpr int kClassThreads[Projector::kClasses] = {64, 256, 512, 768, 1024, 1024, 1024};
+constexpr int kClassTri[Projector::kClasses] = {-1, 0, 1, 2, 3, 4, -1};
 int proj_class(int d) {
     if (d <= 16) return 0;
     if (d <= 64) return 1;
     if (d <= 128) return 2;
     if (d <= 192) return 3;
-    if (d <= 256) return 4;
+    if (d <= 224) return 4;
+    if (d <= 256) return 5;
     return kGlobalClass;
 }
 // work doubles of a tridiagonal-path block: reflectors, eigenvectors, LU
 // factors (7 d^2), plus the tile of the global-memory class
-int64_t tri_work(int tri, int d) { return 7 * (int64_t)d * d + (tri == 3 ? (int64_t)1024 * 8 * 8 : 0); }
+int64_t tri_work(int tri, int d) { return 7 * (int64_t)d * d + (tri == 4 ? (int64_t)1024 * 8 * 8 : 0); }
 }  // namespace
 
 void Projector::build(const std::vector<std::vector<int>>& dims, std::vector<InstDev>& inst, hipStream_t s) {

@@ -15,8 +15,8 @@ namespace v2h {
 
 class Projector {
   public:
-    // Jacobi d <= 16, tridiagonal d <= 64 / 128 / 192 / 256, Jacobi d > 256
-    static constexpr int kClasses = 6;
+    // Jacobi d <= 16, tridiagonal d <= 64 / 128 / 192 / 224 / 256, Jacobi d > 256
+    static constexpr int kClasses = 7;
 
     // dims[i]: PSD block sizes of instance i in primal order (zeros allowed);
     // fills block_first / nblocks of inst[i].  Blocks are numbered

@@ -86,9 +86,9 @@ class Batch {
     hipStream_t stream() const { return stream_; }
 
     // HIP-event timing of every kernel launch of solve(), per kernel kind:
-    // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..9 projection classes: Jacobi d <= 16,
-    // tridiagonal d <= 64 / 128 / 192 / 256, Jacobi d > 256 (matrix in global
-    // memory).
+    // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..10 projection classes: Jacobi d <= 16,
+    // tridiagonal d <= 64 / 128 / 192 / 224 / 256, Jacobi d > 256 (matrix in
+    // global memory).
     static constexpr int kProjClasses = Projector::kClasses;
     static constexpr int kKinds = 4 + kProjClasses;
     void set_profiling(bool on) { profile_ = on; }

@@ -37,8 +37,10 @@ def short_name(kernel: str) -> str:
     name = m.group(1)
     if name == "k_project_tri" and m.group(3):
         nt = m.group(3).split(",")[0].strip()
-        return {"256": "project_tri64", "512": "project_tri128", "768": "project_tri192",
-                "1024": "project_tri256"}.get(nt, name)
+        ri = m.group(3).split(",")[1].strip() if m.group(3).count(",") >= 1 else ""
+        if nt == "1024":
+            return "project_tri224" if ri == "7" else "project_tri256"
+        return {"256": "project_tri64", "512": "project_tri128", "768": "project_tri192"}.get(nt, name)
     return {"k_cg_at": "cg_at", "k_cg_a": "cg_a", "k_pre": "pre", "k_post": "post"}.get(name, name)
 
 
