@@ -8,8 +8,10 @@ resident in HBM before the timed region; the timed region is reset + solve.
 
 Printed (one JSON line): value = converged SDPs/s of the batch; the single
 instance rate and the CPU oracle rate are reported beside it; `roofline` is
-the dominant kernel's achieved rate over its launches that processed the whole
-batch, timed with HIP events on the library's stream inside the timed region.
+the dominant kernel's achieved rate: the mean work of its launches (each
+launch processes the instances still iterating) over the mean duration of its
+launches, timed with HIP events on the library's stream inside the timed
+region.
 
 Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
 """
@@ -77,29 +79,46 @@ def load_traffic(kernel, workload):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
-def roofline(B, kt_full, workload):
-    """Dominant kernel (largest time over whole-batch launches) -> roofline dict."""
+def executions(kind, oiter, iiter):
+    """Launches of `kind` that one instance takes part in, from its outer and
+    CG iteration counts: every CG solve runs one step more than it counts
+    (the step whose residual stops it)."""
+    if kind in ("cg_at", "cg_a"):
+        return iiter + oiter
+    if kind == "pre":
+        return oiter + 1
+    return oiter  # post and the projection classes
+
+
+def roofline(B, kt, counts, workload):
+    """Dominant kernel (largest total time in the timed region) -> roofline
+    dict.  Work per launch is the mean over the kernel's launches of the work
+    of the instances each launch processed (instances leave a launch when
+    their CG, or their solve, has stopped); time per launch is the mean HIP
+    event time of the kernel's launches."""
     n_p, n_d = B.n_primal, B.n_dual
-    totals = {k: v[0] for k, v in kt_full.items() if v[1]}
+    totals = {k: v[0] for k, v in kt.items() if v[1]}
     dom = max(totals, key=totals.get)
-    ms, cnt = kt_full[dom]
+    ms, cnt = kt[dom]
     per_launch_ms = ms / max(cnt, 1)
     traffic = load_traffic(dom, workload)
     if dom.startswith("project"):
         lo, hi = [(a, b) for k, a, b in PROJ_CLASSES if k == dom][0]
-        dims = [d for i in range(B.count) for d in B.block_dims(i) if lo <= d <= hi]
-        flops = project_flops(dims)
-        achieved = flops / (per_launch_ms * 1e-3) / 1e12
+        work = sum(executions(dom, o, it) * project_flops([d for d in B.block_dims(i) if lo <= d <= hi])
+                   for i, (o, it) in enumerate(counts)) / max(cnt, 1)
+        achieved = work / (per_launch_ms * 1e-3) / 1e12
         return dom, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": dom,
-                     "per_launch_ms": per_launch_ms, "work_per_launch": flops, "blocks_per_launch": len(dims),
-                     "work": "(16/3) d^3 FP64 flops per block"}
-    nbytes = sum(bytes_per_instance(dom, n_p[i], n_d[i]) for i in range(B.count))
-    achieved = nbytes / (per_launch_ms * 1e-3) / 1e9
+                     "launches": cnt, "per_launch_ms": per_launch_ms, "work_per_launch": work,
+                     "work": "(16/3) d^3 FP64 flops per block, mean over launches"}
+    work = sum(executions(dom, o, it) * bytes_per_instance(dom, n_p[i], n_d[i])
+               for i, (o, it) in enumerate(counts)) / max(cnt, 1)
+    achieved = work / (per_launch_ms * 1e-3) / 1e9
     return dom, {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_" + dom,
-                 "per_launch_ms": per_launch_ms, "work_per_launch": nbytes,
-                 "work": "FP64 vector bytes per launch (tables excluded)"}
+                 "launches": cnt, "per_launch_ms": per_launch_ms, "work_per_launch": work,
+                 "work": "FP64 vector bytes of the instances each launch processed (tables excluded), "
+                         "mean over launches"}
 
 
 def cpu_baseline(item, mean_oiter, seconds):
@@ -157,6 +176,9 @@ def main():
     ap.add_argument("--maxiter", type=int, default=20000)
     ap.add_argument("--single", type=int, default=1, help="also time one instance alone (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU oracle sample (0 = skip)")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="after the solve, three plain copies of this many MiB (k_copy), which calibrate "
+                         "the traffic counters of a profile (0 = none)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
@@ -176,7 +198,7 @@ def main():
         solve_with_progress(B, f"warmup {w}")
     B.reset_kernel_times()
 
-    step_times, conv, oiters = [], [], []
+    step_times, conv, oiters, counts = [], [], [], [[0, 0] for _ in range(args.batch)]
     for k in range(args.steps):
         reset_all(B)
         t0 = time.perf_counter()
@@ -184,15 +206,24 @@ def main():
         step_times.append(time.perf_counter() - t0)
         conv.append(nconv)
         oiters.append([B.status(i)["oiter"] for i in range(B.count)])
+        for i in range(B.count):
+            st = B.status(i)
+            counts[i][0] += st["oiter"]
+            counts[i][1] += st["iiter"]
     kt = B.kernel_times()
-    kt_full = B.kernel_times(full=True)
     total = sum(step_times)
     value = sum(conv) / total
     mean_oiter = sum(sum(o) for o in oiters) / max(1, sum(len(o) for o in oiters))
     st = [B.status(i) for i in range(B.count)]
     workload = f"c2v16x{args.batch}"
-    dom, roof = roofline(B, kt_full, workload)
+    dom, roof = roofline(B, kt, counts, workload)
     B.close()
+
+    if args.calibration_copy_mib > 0:
+        n = args.calibration_copy_mib * (1 << 20) // 8
+        ms = _native.calibration_copy(n)
+        print(f"[bench] calibration copy of {args.calibration_copy_mib} MiB: {ms:.3f} ms, "
+              f"{2 * 8 * n / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
 
     single = None
     if args.single:

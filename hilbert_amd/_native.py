@@ -53,6 +53,7 @@ SIGNATURES = {
     "v2h_last_error": (C.c_char_p, []),
     "v2h_default_options": (C.c_int, [C.POINTER(SdpOptions)]),
     "v2h_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "v2h_calibration_copy": (C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     "v2h_batch_create": (C.c_int, [C.POINTER(ProblemDesc), C.c_int, C.POINTER(SdpOptions), C.c_int,
                                    C.POINTER(_vp)]),
     "v2h_batch_destroy": (C.c_int, [_vp]),
@@ -164,6 +165,13 @@ def default_options() -> SdpOptions:
     o = SdpOptions()
     check(lib().v2h_default_options(C.byref(o)))
     return o
+
+
+def calibration_copy(n: int, repeats: int = 3, device: int = 0) -> float:
+    """Mean milliseconds of one plain device copy of n doubles (k_copy)."""
+    ms = C.c_double(0.0)
+    check(lib().v2h_calibration_copy(device, n, repeats, C.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

@@ -10,6 +10,8 @@
 #include <vector>
 
 #include "../../include/v2rdm_hip.h"
+#include "dbuf.hpp"
+#include "kernels.hpp"
 #include "rdm.hpp"
 #include "sdp_host.hpp"
 #include "solver.hpp"
@@ -95,6 +97,35 @@ int v2h_device_count(int* count) {
     return guarded([&] {
         check_ptr(count);
         V2H_HIP(hipGetDeviceCount(count));
+    });
+}
+
+int v2h_calibration_copy(int device, int64_t n, int repeats, double* ms_per_copy) {
+    return guarded([&] {
+        check_ptr(ms_per_copy);
+        if (n <= 0 || repeats <= 0) throw std::invalid_argument("n and repeats must be positive");
+        V2H_HIP(hipSetDevice(device));
+        v2h::DBuf<double> src, dst;
+        src.alloc(n);
+        dst.alloc(n);
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        V2H_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        V2H_HIP(hipEventCreate(&e0));
+        V2H_HIP(hipEventCreate(&e1));
+        v2h::launch_fill(src.p, n, 1.0, s);
+        v2h::launch_fill(dst.p, n, 0.0, s);
+        V2H_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < repeats; r++) v2h::launch_copy(dst.p, src.p, n, s);
+        V2H_HIP(hipEventRecord(e1, s));
+        V2H_HIP(hipGetLastError());
+        V2H_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        V2H_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *ms_per_copy = ms / repeats;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
     });
 }
 

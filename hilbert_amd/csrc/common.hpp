@@ -97,6 +97,13 @@ struct UnitsDev {
     const int* wg_begin;    // [ninst + 1] first workgroup of each instance
     int nwg;
     int ninst;
+    // Launch plan: the launched workgroups are runs of logical workgroups
+    // (the workgroups of the instances still iterating).  plan[0..plan_m] are
+    // the prefix counts of the runs, plan[plan_m + 1 + j] the first logical
+    // workgroup of run j.  plan == nullptr launches every workgroup.
+    const int* plan;
+    int plan_m;
+    int grid;               // workgroups launched
 };
 
 struct ProjItem {

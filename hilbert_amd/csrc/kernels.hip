@@ -45,6 +45,22 @@ __device__ __forceinline__ int xcd_remap(int bid, int n) {
     return x * per + (x < rem ? x : rem) + loc;
 }
 
+// Logical workgroup of this block under the launch plan (runs of the
+// workgroups of selected instances; identity without a plan).
+__device__ __forceinline__ int unit_wg(const UnitsDev& U) {
+    const int c = xcd_remap(blockIdx.x, U.grid);
+    if (U.plan == nullptr) return c;
+    int lo = 0, hi = U.plan_m;  // plan[lo] <= c < plan[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (U.plan[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return U.plan[U.plan_m + 1 + lo] + (c - U.plan[lo]);
+}
+
 // The CG vector recurrences, shared by the gather (on the fly) and the
 // owning-row update so both produce identical bits.
 __device__ __forceinline__ double r0_of(double av1, double b, double ax, double aaty, double mu) {
@@ -160,7 +176,7 @@ __device__ __forceinline__ void load_coef(double* dst, const double* src) {
 // ------------------------------------------------------------------------
 // CG step, part 1: q = A^T p
 __global__ __launch_bounds__(kWG) void k_cg_at(BatchDev B, UnitsDev U) {
-    const int wg = xcd_remap(blockIdx.x, U.nwg);
+    const int wg = unit_wg(U);
     const int inst = U.wg_inst[wg];
     const InstState& st = B.st[inst];
     if (!st.active || st.cg_done) return;
@@ -193,7 +209,7 @@ __global__ __launch_bounds__(kWG) void k_cg_at(BatchDev B, UnitsDev U) {
 // CG step, part 2: w = A q, own-row updates and dots; last workgroup advances
 // the CG state of the instance.
 __global__ __launch_bounds__(kWG) void k_cg_a(BatchDev B, UnitsDev U) {
-    const int wg = xcd_remap(blockIdx.x, U.nwg);
+    const int wg = unit_wg(U);
     const int inst = U.wg_inst[wg];
     InstState& st = B.st[inst];
     if (!st.active || st.cg_done) return;
@@ -275,7 +291,7 @@ __global__ __launch_bounds__(kWG) void k_cg_reduce(BatchDev B, UnitsDev U) {
 // Start of an outer iteration: A [v1, x, aty] -> av1, ax, aaty; finishes the
 // previous iteration (errors, energies, mu, convergence) and sets up the CG.
 __global__ __launch_bounds__(kWG) void k_pre(BatchDev B, UnitsDev U) {
-    const int wg = xcd_remap(blockIdx.x, U.nwg);
+    const int wg = unit_wg(U);
     const int inst = U.wg_inst[wg];
     InstState& st = B.st[inst];
     if (!st.active) return;
@@ -372,7 +388,7 @@ __global__ __launch_bounds__(kWG) void k_pre_reduce(BatchDev B, UnitsDev U) {
 
 // After the CG: aty = A^T y, u = mu x + aty - c.
 __global__ __launch_bounds__(kWG) void k_post(BatchDev B, UnitsDev U) {
-    const int wg = xcd_remap(blockIdx.x, U.nwg);
+    const int wg = unit_wg(U);
     const int inst = U.wg_inst[wg];
     const InstState& st = B.st[inst];
     if (!st.active) return;
@@ -730,20 +746,22 @@ __global__ __launch_bounds__(1024) void k_dot1(const double* a, const double* b,
 // launchers
 
 void launch_cg_at(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
-    if (U.nwg > 0) hipLaunchKernelGGL(k_cg_at, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    if (U.grid > 0) hipLaunchKernelGGL(k_cg_at, dim3(U.grid), dim3(kWG), 0, s, B, U);
 }
 void launch_cg_a(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
-    if (U.nwg <= 0) return;
-    hipLaunchKernelGGL(k_cg_a, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    if (U.grid <= 0) return;
+    hipLaunchKernelGGL(k_cg_a, dim3(U.grid), dim3(kWG), 0, s, B, U);
     hipLaunchKernelGGL(k_cg_reduce, dim3(U.ninst), dim3(kWG), 0, s, B, U);
 }
 void launch_pre(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
+    // every instance: k_pre_reduce also finishes the bookkeeping of instances
+    // outside the plan
     if (U.nwg <= 0) return;
-    hipLaunchKernelGGL(k_pre, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    if (U.grid > 0) hipLaunchKernelGGL(k_pre, dim3(U.grid), dim3(kWG), 0, s, B, U);
     hipLaunchKernelGGL(k_pre_reduce, dim3(U.ninst), dim3(kWG), 0, s, B, U);
 }
 void launch_post(const BatchDev& B, const UnitsDev& U, hipStream_t s) {
-    if (U.nwg > 0) hipLaunchKernelGGL(k_post, dim3(U.nwg), dim3(kWG), 0, s, B, U);
+    if (U.grid > 0) hipLaunchKernelGGL(k_post, dim3(U.grid), dim3(kWG), 0, s, B, U);
 }
 void launch_spmv_plain(const SellDev& S, const int4* units, int wg0, int nwg, const double* in, double* out,
                        hipStream_t s) {

@@ -164,6 +164,11 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     }
     d_state_.alloc(N);
     V2H_HIP(hipHostMalloc((void**)&pinned_, sizeof(InstState) * N, hipHostMallocDefault));
+    V2H_HIP(hipHostMalloc((void**)&plan_pinned_, sizeof(int) * 2 * (2 * (size_t)N + 1), hipHostMallocDefault));
+    plan_dev_.alloc(2 * (2 * (size_t)N + 1));
+    plan_sel_.assign(N, 1);
+    gridA_ = nwgA_;
+    gridAT_ = nwgAT_;
     for (int i = 0; i < N; i++) {
         set_guess(i, nullptr);
         reset_state(i);
@@ -181,6 +186,7 @@ Batch::~Batch() {
             (void)hipEventDestroy(e.second);
         }
     if (pinned_) (void)hipHostFree(pinned_);
+    if (plan_pinned_) (void)hipHostFree(plan_pinned_);
     for (int k = 0; k < kProjClasses; k++) {
         if (pstream_[k]) (void)hipStreamDestroy(pstream_[k]);
         if (ev_proj_[k]) (void)hipEventDestroy(ev_proj_[k]);
@@ -239,9 +245,62 @@ BatchDev Batch::dev() const {
     return B;
 }
 
+UnitsDev Batch::units_a() const {
+    const int N = count();
+    return UnitsDev{unitsA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_, N, plan_all_ ? nullptr : plan_dev_.p, planA_m_,
+                    gridA_};
+}
+
+UnitsDev Batch::units_at() const {
+    const int N = count();
+    return UnitsDev{unitsAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_, N,
+                    plan_all_ ? nullptr : plan_dev_.p + 2 * N + 1, planAT_m_, gridAT_};
+}
+
+// Launch only the workgroups of the selected instances.  The selection must
+// contain every instance that the kernels would not skip (the host's view of
+// the state is never behind the device's in that sense: instances only stop).
+void Batch::set_plan(const std::vector<char>& sel) {
+    if (sel == plan_sel_) return;
+    const int N = count();
+    if (plan_copy_pending_) V2H_HIP(hipStreamSynchronize(stream_));  // the pinned copy is still being read
+    plan_sel_ = sel;
+    plan_all_ = std::all_of(sel.begin(), sel.end(), [](char c) { return c != 0; });
+    if (plan_all_) {
+        gridA_ = nwgA_;
+        gridAT_ = nwgAT_;
+        return;
+    }
+    auto build = [&](const std::vector<int>& wgb, int* out, int& m) {
+        // runs of consecutive selected instances
+        std::vector<int> pre{0}, first;
+        for (int i = 0; i < N;) {
+            if (!sel[i] || wgb[i + 1] == wgb[i]) {
+                i++;
+                continue;
+            }
+            int j = i;
+            while (j < N && sel[j]) j++;
+            first.push_back(wgb[i]);
+            pre.push_back(pre.back() + wgb[j] - wgb[i]);
+            i = j;
+        }
+        m = (int)first.size();
+        std::copy(pre.begin(), pre.end(), out);
+        std::copy(first.begin(), first.end(), out + m + 1);
+        return pre.back();
+    };
+    gridA_ = build(hwgA_begin_, plan_pinned_, planA_m_);
+    gridAT_ = build(hwgAT_begin_, plan_pinned_ + 2 * N + 1, planAT_m_);
+    V2H_HIP(hipMemcpyAsync(plan_dev_.p, plan_pinned_, sizeof(int) * 2 * (2 * (size_t)N + 1), hipMemcpyHostToDevice,
+                           stream_));
+    plan_copy_pending_ = true;
+}
+
 void Batch::sync_state_to_host() {
     V2H_HIP(hipMemcpyAsync(pinned_, d_state_.p, sizeof(InstState) * count(), hipMemcpyDeviceToHost, stream_));
     V2H_HIP(hipStreamSynchronize(stream_));
+    plan_copy_pending_ = false;
     std::memcpy(state_.data(), pinned_, sizeof(InstState) * count());
     tflush();
 }
@@ -398,8 +457,12 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     }
     sync_state_to_device();
     const BatchDev B = dev();
-    const UnitsDev UA{unitsA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_, count()};
-    const UnitsDev UAT{unitsAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_, count()};
+    std::vector<char> sel(count());
+    auto select = [&](bool iterating) {
+        for (int i = 0; i < count(); i++)
+            sel[i] = state_[i].active && (!iterating || !state_[i].cg_done) ? 1 : 0;
+        set_plan(sel);
+    };
     std::vector<int64_t> seen(count());
     for (int i = 0; i < count(); i++) seen[i] = state_[i].oiter;
 
@@ -415,8 +478,9 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     refresh_flags();
     all_iterating_ = all_running_;
     for (int it = 0; it < maxiter; it++) {
+        select(false);  // the instances running, as far as the host knows
         tstart(0);
-        launch_pre(B, UA, stream_);
+        launch_pre(B, units_a(), stream_);
         tstop(0);
         launches_++;
         if (mon) {
@@ -430,6 +494,7 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         int steps = 0, chunk = cg_est;
         bool any_active = true;
         for (;;) {
+            const UnitsDev UA = units_a(), UAT = units_at();
             for (int k = 0; k < chunk; k++) {
                 tstart(1);
                 launch_cg_at(B, UAT, stream_);
@@ -449,6 +514,7 @@ int Batch::solve(int maxiter, const Monitor& mon) {
                 pending |= st.active && !st.cg_done;
             }
             if (!pending || steps > opt_.cg_maxiter) break;
+            select(true);  // the instances whose CG goes on
             chunk = 4;
         }
         if (!any_active) break;
@@ -456,8 +522,9 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         for (const auto& st : state_)
             if (st.active) cg_est = std::max(cg_est, st.cg_last + 2);
         all_iterating_ = all_running_;  // the next CG solve starts for every running instance
+        select(false);
         tstart(3);
-        launch_post(B, UAT, stream_);
+        launch_post(B, units_at(), stream_);
         tstop(3);
         launches_++;
         // the size classes are independent: largest first, each on its stream
@@ -475,8 +542,9 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         }
         V2H_HIP(hipGetLastError());
     }
+    select(false);
     tstart(0);
-    launch_pre(B, UA, stream_);  // book-keeping of the last iteration
+    launch_pre(B, units_a(), stream_);  // book-keeping of the last iteration
     tstop(0);
     launches_++;
     V2H_HIP(hipGetLastError());

@@ -154,6 +154,18 @@ class Batch {
     void tstop(int k, hipStream_t s = nullptr);
     void tflush();
 
+    // Launch plans of the gather kernels (UnitsDev::plan): the instances whose
+    // workgroups are launched.  Host copy in pinned memory, device copy
+    // [A plan | A^T plan], each 2 count + 1 ints.
+    void set_plan(const std::vector<char>& sel);
+    UnitsDev units_a() const;
+    UnitsDev units_at() const;
+    std::vector<char> plan_sel_;
+    int* plan_pinned_ = nullptr;
+    DBuf<int> plan_dev_;
+    int planA_m_ = 0, planAT_m_ = 0, gridA_ = 0, gridAT_ = 0;
+    bool plan_all_ = true, plan_copy_pending_ = false;
+
     InstState* pinned_ = nullptr;
     bool constructed_ = false;
     double last_seconds_ = 0.0;

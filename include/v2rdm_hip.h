@@ -77,6 +77,12 @@ typedef void (*v2h_monitor_fn)(int instance, int oiter, int iiter, double energy
 const char *v2h_last_error(void);
 int v2h_default_options(v2h_sdp_options *opt);
 int v2h_device_count(int *count);
+/* Plain copy of n doubles between two fresh device buffers, repeated
+ * `repeats` times with the library's copy kernel (k_copy); *ms_per_copy is the
+ * mean time of one copy.  It moves 8 n bytes each way and calibrates the HBM
+ * traffic counters (FETCH_SIZE, WRITE_SIZE) of a profile.  No counterpart in
+ * the reference. */
+int v2h_calibration_copy(int device, int64_t n, int repeats, double *ms_per_copy);
 
 typedef struct v2h_batch v2h_batch;
 
