@@ -75,6 +75,12 @@ struct SolverOptions {
     int cg_maxiter, maxiter, mu_update_frequency;
 };
 
+// CG state of one dual row, stored together so that a gather of the row
+// touches one 32-byte sector: residual r, w = A A^T p, direction p, dual y.
+struct alignas(32) CgRow {
+    double r, w, p, y;
+};
+
 // Batch-wide device arrays.
 struct BatchDev {
     const StructDev* structs;
@@ -83,7 +89,8 @@ struct BatchDev {
     // primal-length arrays
     double *x, *z, *c, *v1, *u, *aty, *q;
     // dual-length arrays
-    double *y, *r, *p, *w, *av1, *ax, *aaty;
+    CgRow* cg;
+    double *av1, *ax, *aaty;
     // reductions
     double* partials;       // per-workgroup partial dot products
     double* proj_part;      // per-block projection partials (c.x, dual residual)

@@ -189,16 +189,27 @@ __device__ int sturm(double s, int d, const double* dd, const double* e2, double
     int c = q < 0.0;
     double dq = -1.0;
     double acc = dq * r;
-    for (int i = 1; i < d; i++) {
-        const double e = e2[i - 1];
-        double qn = fma(-e, r, dd[i] - s);
+    auto step = [&](double di, double e) {
+        double qn = fma(-e, r, di - s);
         const double dqn = fma(e * dq, r * r, -1.0);
         if (fabs(qn) < pivmin) qn = -pivmin;
         r = recip(qn);
         dq = dqn;
         c += qn < 0.0;
         acc = fma(dq, r, acc);
+    };
+    int i = 1;
+    for (; i + 4 <= d; i += 4) {  // the loads of four steps ahead of the recurrence
+        double dv[4], ev[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            dv[u] = dd[i + u];
+            ev[u] = e2[i + u - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) step(dv[u], ev[u]);
     }
+    for (; i < d; i++) step(dd[i], e2[i - 1]);
     S = acc;
     return c;
 }
@@ -274,12 +285,41 @@ __device__ double orthogonalise(const Column& c, int d, const double* Sv, int l0
         for (int m = l0; m < l; m++) {
             const double* y = Sv + m;
             double dot = 0.0;
-            for (int i = 0; i < d; i++) dot = fma(y[(size_t)i * d], c.X(i), dot);
+            for (int i0 = 0; i0 < d; i0 += kChunk) {
+                double yv[kChunk], xv[kChunk];
+#pragma unroll
+                for (int u = 0; u < kChunk; u++) {
+                    const int i = min(i0 + u, d - 1);
+                    yv[u] = y[(size_t)i * d];
+                    xv[u] = c.X(i);
+                }
+#pragma unroll
+                for (int u = 0; u < kChunk; u++)
+                    if (i0 + u < d) dot = fma(yv[u], xv[u], dot);
+            }
             const double f = dot / xs[m];
-            for (int i = 0; i < d; i++) c.X(i) = fma(-f, y[(size_t)i * d], c.X(i));
+            for (int i0 = 0; i0 < d; i0 += kChunk) {
+                double yv[kChunk], xv[kChunk];
+#pragma unroll
+                for (int u = 0; u < kChunk; u++) {
+                    const int i = min(i0 + u, d - 1);
+                    yv[u] = y[(size_t)i * d];
+                    xv[u] = c.X(i);
+                }
+#pragma unroll
+                for (int u = 0; u < kChunk; u++)
+                    if (i0 + u < d) c.X(i0 + u) = fma(-f, yv[u], xv[u]);
+            }
         }
     double s2 = 0.0;
-    for (int i = 0; i < d; i++) s2 = fma(c.X(i), c.X(i), s2);
+    for (int i0 = 0; i0 < d; i0 += kChunk) {
+        double xv[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; u++) xv[u] = c.X(min(i0 + u, d - 1));
+#pragma unroll
+        for (int u = 0; u < kChunk; u++)
+            if (i0 + u < d) s2 = fma(xv[u], xv[u], s2);
+    }
     return s2;
 }
 
@@ -331,6 +371,201 @@ struct Tile<NT, RI, CJ, kTileGlobal> {
     }
     __device__ __forceinline__ double col(int r, int ck) { return at(r, ck); }
 };
+
+// Eigen-decomposition of the tridiagonal T (diagonal dd, off-diagonal ee, in
+// LDS), one thread per eigenvalue: eigenvalues ascending in lamv (Sturm counts
+// and bracketed Newton steps, warm-started from lprev, which receives the new
+// ones), unnormalised eigenvectors in the columns of Sv (inverse iteration
+// with the LU factors in fac), xs[l] = ||S(:, l)||^2.  Returns the number of
+// eigenvalues <= 0; tnorm = ||T|| estimate, zero_block when T = 0.
+template <int NT>
+__device__ __forceinline__ int tri_eigen(int d, const double* dd, const double* ee, double* e2, double* lamv,
+                                         double* xs, double* red, double* lprev, double* Sv, double* fac,
+                                         int* status, double& tnorm_out, bool& zero_block_out) {
+    const int t = threadIdx.x;
+    double glo = INFINITY, ghi = -INFINITY, em = 0.0;
+    if (t < d) {
+        const double el = t > 0 ? fabs(ee[t - 1]) : 0.0;
+        const double er = t + 1 < d ? fabs(ee[t]) : 0.0;
+        glo = dd[t] - el - er;
+        ghi = dd[t] + el + er;
+        if (t + 1 < d) {
+            e2[t] = ee[t] * ee[t];
+            em = e2[t];
+        }
+    }
+    glo = block_min<NT>(glo, red);
+    ghi = block_max<NT>(ghi, red);
+    em = block_max<NT>(em, red);
+    const double tnorm = fmax(fabs(glo), fabs(ghi));
+    tnorm_out = tnorm;
+    const double pivmin = DBL_MIN * fmax(1.0, em);
+    const double gl = glo - 2.0 * kEps * tnorm * d - 2.0 * pivmin;
+    const double gu = ghi + 2.0 * kEps * tnorm * d + 2.0 * pivmin;
+    const double abstol = 2.0 * kEps * tnorm;
+    const double dprev = lprev[d];
+    const bool zero_block = !(tnorm > 0.0);
+    zero_block_out = zero_block;
+    double change = 0.0;
+    bool bad = false;
+    if (t < d) {
+        const int l = t;
+        double lam = 0.0;
+        if (!zero_block) {
+            double lo = gl, hi = gu;
+            int clo = 0, chi = d;
+            int passes = 0;
+            double s = 0.5 * (lo + hi), S = 0.0;
+            bool haveS = false;
+            auto eval = [&](double x, double& Sx) {
+                const int c = sturm(x, d, dd, e2, pivmin, Sx);
+                passes++;
+                if (c <= l) {
+                    lo = x;
+                    clo = c;
+                } else {
+                    hi = x;
+                    chi = c;
+                }
+                return c;
+            };
+            const double lp = lprev[l];
+            if (dprev >= 0.0 && isfinite(lp) && isfinite(dprev)) {
+                // expanding search around the previous eigenvalue of this block
+                double del = fmax(2.0 * dprev, 8.0 * abstol);
+                s = fmin(fmax(lp, lo), hi);
+                const int c0 = eval(s, S);
+                haveS = true;
+                for (int e = 0; e < 8; e++) {
+                    const double x = (c0 <= l) ? s + del : s - del;
+                    if (!(x > lo && x < hi)) break;
+                    double Sx;
+                    const int cx = eval(x, Sx);
+                    if ((c0 <= l) != (cx <= l)) break;
+                    del *= 4.0;
+                }
+            }
+            lam = 0.5 * (lo + hi);
+            double last = INFINITY;
+            while (passes < kMaxPasses) {
+                if (hi - lo <= abstol) {
+                    lam = 0.5 * (lo + hi);
+                    break;
+                }
+                double sn = 0.0;
+                bool newton = false;
+                if (clo == l && chi == l + 1 && haveS) {
+                    const double step = -1.0 / S;
+                    sn = s + step;
+                    if (fabs(step) <= 2.0 * abstol) {
+                        lam = fmin(fmax(sn, lo), hi);
+                        break;
+                    }
+                    newton = isfinite(sn) && sn > lo && sn < hi && fabs(step) <= 0.5 * last;
+                    if (newton) last = fabs(step);
+                }
+                if (!newton) {
+                    sn = 0.5 * (lo + hi);
+                    last = INFINITY;
+                }
+                s = sn;
+                eval(s, S);
+                haveS = true;
+                lam = s;
+            }
+            if (passes >= kMaxPasses) bad = true;
+            if (isfinite(lp)) change = fabs(lam - lp);
+        }
+        lamv[l] = lam;
+        xs[l] = 1.0;
+    }
+    change = block_max<NT>(change, red);  // also publishes lamv
+    if (t < d) lprev[t] = lamv[t];
+    if (t == 0) lprev[d] = (isfinite(dprev) && dprev >= 0.0) ? change : 1e-2 * tnorm;
+
+    // inverse iteration, one thread per cluster of close eigenvalues; the
+    // vectors stay unnormalised, xs[l] = ||S(:, l)||^2
+    if (!zero_block && t < d) {
+        auto linked = [&](int l) {
+            return l > 0 && (lamv[l] - lamv[l - 1]) <= kClusterRel * fmax(fabs(lamv[l]), fabs(lamv[l - 1])) +
+                                                           kClusterAbs * tnorm;
+        };
+        if (!linked(t)) {
+            const int l0 = t;
+            int l1 = l0;
+            while (l1 + 1 < d && linked(l1 + 1)) l1++;
+            const double ptol = kEps * tnorm;
+            double sprev = -INFINITY;
+            for (int l = l0; l <= l1; l++) {
+                double sg = lamv[l];
+                const double pert = 10.0 * kEps * fmax(fabs(sg), 1e-3 * tnorm);
+                if (l > l0 && sg <= sprev + pert) sg = sprev + pert;
+                sprev = sg;
+                const Column col{Sv + l, fac + l, (size_t)d, (size_t)d * d};
+                // T - sg I = P L U with partial pivoting ([q][i]: 0 multiplier,
+                // 1 reciprocal pivot, 2-3 superdiagonals of U, 4 interchange),
+                // fused with L^{-1} P applied to the start vector
+                double dcur = dd[0] - sg, ucur = d > 1 ? ee[0] : 0.0;
+                double bc = start_value((uint32_t)l, 0u);
+                for (int i = 0; i + 1 < d; i++) {
+                    const double sub = ee[i];
+                    double dnext = dd[i + 1] - sg;
+                    double unext = (i + 2 < d) ? ee[i + 1] : 0.0;
+                    const double bn = start_value((uint32_t)l, (uint32_t)(i + 1));
+                    double f, u0, u1, u2, sw, xi;
+                    if (fabs(dcur) >= fabs(sub)) {
+                        if (fabs(dcur) < ptol) dcur = (dcur < 0.0) ? -ptol : ptol;
+                        f = sub * recip(dcur);
+                        u0 = dcur;
+                        u1 = ucur;
+                        u2 = 0.0;
+                        sw = 0.0;
+                        dnext = fma(-f, ucur, dnext);
+                        xi = bc;
+                        bc = fma(-f, bc, bn);
+                    } else {
+                        f = dcur * recip(sub);
+                        u0 = sub;
+                        u1 = dnext;
+                        u2 = unext;
+                        sw = 1.0;
+                        dnext = fma(-f, dnext, ucur);
+                        unext = -f * unext;
+                        xi = bn;
+                        bc = fma(-f, bn, bc);
+                    }
+                    col.f(0, i) = f;
+                    col.f(1, i) = recip(u0);
+                    col.f(2, i) = u1;
+                    col.f(3, i) = u2;
+                    col.f(4, i) = sw;
+                    col.X(i) = xi;
+                    dcur = dnext;
+                    ucur = unext;
+                }
+                if (fabs(dcur) < ptol) dcur = (dcur < 0.0) ? -ptol : ptol;
+                col.f(1, d - 1) = recip(dcur);
+                col.f(2, d - 1) = 0.0;
+                col.f(3, d - 1) = 0.0;
+                col.X(d - 1) = bc;
+                double s2 = apply_u(col, d);
+                if (l > l0) s2 = orthogonalise(col, d, Sv, l0, l, xs);
+                apply_pl(col, d, 1.0 / sqrt(s2));
+                s2 = apply_u(col, d);
+                if (l > l0) s2 = orthogonalise(col, d, Sv, l0, l, xs);
+                if (!(s2 > 0.0) || !isfinite(s2)) {
+                    bad = true;
+                    s2 = INFINITY;  // drops the vector from T+
+                }
+                xs[l] = s2;
+            }
+        }
+    }
+    if (bad) atomicOr(status, (int)kStatusJacobiNotConverged);
+    // eigenvalues are ascending: the first ps are <= 0 (a count, so that the
+    // range below stays inside [0, d] whatever the values)
+    return d - (int)block_sum<NT>((t < d && !(lamv[t] <= 0.0)) ? 1.0 : 0.0, red);  // also completes Sv, xs
+}
 
 template <int NT, int RI, int CJ, int MODE>
 __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* items, double* work, double* lamst) {
@@ -605,186 +840,9 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
 
     }
     // ---------------------------------------------------------------- T = S L S^T
-    double glo = INFINITY, ghi = -INFINITY, em = 0.0;
-    if (t < d) {
-        const double el = t > 0 ? fabs(ee[t - 1]) : 0.0;
-        const double er = t + 1 < d ? fabs(ee[t]) : 0.0;
-        glo = dd[t] - el - er;
-        ghi = dd[t] + el + er;
-        if (t + 1 < d) {
-            e2[t] = ee[t] * ee[t];
-            em = e2[t];
-        }
-    }
-    glo = block_min<NT>(glo, red);
-    ghi = block_max<NT>(ghi, red);
-    em = block_max<NT>(em, red);
-    const double tnorm = fmax(fabs(glo), fabs(ghi));
-    const double pivmin = DBL_MIN * fmax(1.0, em);
-    const double gl = glo - 2.0 * kEps * tnorm * d - 2.0 * pivmin;
-    const double gu = ghi + 2.0 * kEps * tnorm * d + 2.0 * pivmin;
-    const double abstol = 2.0 * kEps * tnorm;
-    const double dprev = lprev[d];
-    const bool zero_block = !(tnorm > 0.0);
-    double change = 0.0;
-    bool bad = false;
-    if (t < d) {
-        const int l = t;
-        double lam = 0.0;
-        if (!zero_block) {
-            double lo = gl, hi = gu;
-            int clo = 0, chi = d;
-            int passes = 0;
-            double s = 0.5 * (lo + hi), S = 0.0;
-            bool haveS = false;
-            auto eval = [&](double x, double& Sx) {
-                const int c = sturm(x, d, dd, e2, pivmin, Sx);
-                passes++;
-                if (c <= l) {
-                    lo = x;
-                    clo = c;
-                } else {
-                    hi = x;
-                    chi = c;
-                }
-                return c;
-            };
-            const double lp = lprev[l];
-            if (dprev >= 0.0 && isfinite(lp) && isfinite(dprev)) {
-                // expanding search around the previous eigenvalue of this block
-                double del = fmax(2.0 * dprev, 8.0 * abstol);
-                s = fmin(fmax(lp, lo), hi);
-                const int c0 = eval(s, S);
-                haveS = true;
-                for (int e = 0; e < 8; e++) {
-                    const double x = (c0 <= l) ? s + del : s - del;
-                    if (!(x > lo && x < hi)) break;
-                    double Sx;
-                    const int cx = eval(x, Sx);
-                    if ((c0 <= l) != (cx <= l)) break;
-                    del *= 4.0;
-                }
-            }
-            lam = 0.5 * (lo + hi);
-            double last = INFINITY;
-            while (passes < kMaxPasses) {
-                if (hi - lo <= abstol) {
-                    lam = 0.5 * (lo + hi);
-                    break;
-                }
-                double sn = 0.0;
-                bool newton = false;
-                if (clo == l && chi == l + 1 && haveS) {
-                    const double step = -1.0 / S;
-                    sn = s + step;
-                    if (fabs(step) <= 2.0 * abstol) {
-                        lam = fmin(fmax(sn, lo), hi);
-                        break;
-                    }
-                    newton = isfinite(sn) && sn > lo && sn < hi && fabs(step) <= 0.5 * last;
-                    if (newton) last = fabs(step);
-                }
-                if (!newton) {
-                    sn = 0.5 * (lo + hi);
-                    last = INFINITY;
-                }
-                s = sn;
-                eval(s, S);
-                haveS = true;
-                lam = s;
-            }
-            if (passes >= kMaxPasses) bad = true;
-            if (isfinite(lp)) change = fabs(lam - lp);
-        }
-        lamv[l] = lam;
-        xs[l] = 1.0;
-    }
-    change = block_max<NT>(change, red);  // also publishes lamv
-    if (t < d) lprev[t] = lamv[t];
-    if (t == 0) lprev[d] = (isfinite(dprev) && dprev >= 0.0) ? change : 1e-2 * tnorm;
-
-    // inverse iteration, one thread per cluster of close eigenvalues; the
-    // vectors stay unnormalised, xs[l] = ||S(:, l)||^2
-    if (!zero_block && t < d) {
-        auto linked = [&](int l) {
-            return l > 0 && (lamv[l] - lamv[l - 1]) <= kClusterRel * fmax(fabs(lamv[l]), fabs(lamv[l - 1])) +
-                                                           kClusterAbs * tnorm;
-        };
-        if (!linked(t)) {
-            const int l0 = t;
-            int l1 = l0;
-            while (l1 + 1 < d && linked(l1 + 1)) l1++;
-            const double ptol = kEps * tnorm;
-            double sprev = -INFINITY;
-            for (int l = l0; l <= l1; l++) {
-                double sg = lamv[l];
-                const double pert = 10.0 * kEps * fmax(fabs(sg), 1e-3 * tnorm);
-                if (l > l0 && sg <= sprev + pert) sg = sprev + pert;
-                sprev = sg;
-                const Column col{Sv + l, fac + l, (size_t)d, (size_t)d * d};
-                // T - sg I = P L U with partial pivoting ([q][i]: 0 multiplier,
-                // 1 reciprocal pivot, 2-3 superdiagonals of U, 4 interchange),
-                // fused with L^{-1} P applied to the start vector
-                double dcur = dd[0] - sg, ucur = d > 1 ? ee[0] : 0.0;
-                double bc = start_value((uint32_t)l, 0u);
-                for (int i = 0; i + 1 < d; i++) {
-                    const double sub = ee[i];
-                    double dnext = dd[i + 1] - sg;
-                    double unext = (i + 2 < d) ? ee[i + 1] : 0.0;
-                    const double bn = start_value((uint32_t)l, (uint32_t)(i + 1));
-                    double f, u0, u1, u2, sw, xi;
-                    if (fabs(dcur) >= fabs(sub)) {
-                        if (fabs(dcur) < ptol) dcur = (dcur < 0.0) ? -ptol : ptol;
-                        f = sub * recip(dcur);
-                        u0 = dcur;
-                        u1 = ucur;
-                        u2 = 0.0;
-                        sw = 0.0;
-                        dnext = fma(-f, ucur, dnext);
-                        xi = bc;
-                        bc = fma(-f, bc, bn);
-                    } else {
-                        f = dcur * recip(sub);
-                        u0 = sub;
-                        u1 = dnext;
-                        u2 = unext;
-                        sw = 1.0;
-                        dnext = fma(-f, dnext, ucur);
-                        unext = -f * unext;
-                        xi = bn;
-                        bc = fma(-f, bn, bc);
-                    }
-                    col.f(0, i) = f;
-                    col.f(1, i) = recip(u0);
-                    col.f(2, i) = u1;
-                    col.f(3, i) = u2;
-                    col.f(4, i) = sw;
-                    col.X(i) = xi;
-                    dcur = dnext;
-                    ucur = unext;
-                }
-                if (fabs(dcur) < ptol) dcur = (dcur < 0.0) ? -ptol : ptol;
-                col.f(1, d - 1) = recip(dcur);
-                col.f(2, d - 1) = 0.0;
-                col.f(3, d - 1) = 0.0;
-                col.X(d - 1) = bc;
-                double s2 = apply_u(col, d);
-                if (l > l0) s2 = orthogonalise(col, d, Sv, l0, l, xs);
-                apply_pl(col, d, 1.0 / sqrt(s2));
-                s2 = apply_u(col, d);
-                if (l > l0) s2 = orthogonalise(col, d, Sv, l0, l, xs);
-                if (!(s2 > 0.0) || !isfinite(s2)) {
-                    bad = true;
-                    s2 = INFINITY;  // drops the vector from T+
-                }
-                xs[l] = s2;
-            }
-        }
-    }
-    if (bad) atomicOr(&st.status, (int)kStatusJacobiNotConverged);
-    // eigenvalues are ascending: the first ps are <= 0 (a count, so that the
-    // range below stays inside [0, d] whatever the values)
-    const int ps = d - (int)block_sum<NT>((t < d && !(lamv[t] <= 0.0)) ? 1.0 : 0.0, red);  // also completes Sv, xs
+    double tnorm;
+    bool zero_block;
+    const int ps = tri_eigen<NT>(d, dd, ee, e2, lamv, xs, red, lprev, Sv, fac, &st.status, tnorm, zero_block);
 
     // ---------------------------------------------------------------- T+ in registers
     const bool use_pos = zero_block || (d - ps) <= ps;
@@ -932,6 +990,249 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     }
 }
 
+// ---- One wavefront per block (blocks of up to 64 rows).  Lane j holds column
+// j of the block in registers, so the products of the Householder reduction
+// and of the back-transform are lane-local and the rank-2 updates need no
+// exchange; the vectors they read (v, w, the column being reduced) are
+// broadcast through LDS.  Rows are visited in groups of eight, and groups
+// that the current step cannot change are skipped (uniform branches).
+constexpr int kW = 64;
+
+__global__ __launch_bounds__(kW) void k_project_wave(BatchDev B, const ProjItem* items, double* work, double* lamst) {
+    __shared__ double colb[kW];  // column k of the reduced matrix, rows >= k
+    __shared__ double vbuf[kW], wbuf[kW];
+    __shared__ double dd[kW], ee[kW], e2[kW], tau[kW], lamv[kW], xs[kW];
+    __shared__ double stage[kStageMax][kW];
+    __shared__ double red[1];
+
+    const ProjItem it = items[blockIdx.x];
+    InstState& st = B.st[it.inst];
+    if (!st.active) return;
+    const InstDev I = B.inst[it.inst];
+    const int d = it.d;
+    const int j = threadIdx.x;
+    const int jc = min(j, d - 1);
+    const int ng = (d + 7) >> 3;  // groups of eight rows that hold rows < d
+    const double mu = st.mu;
+    const double* U = B.u + I.poff + it.off;
+    double* hv = work + it.work_off;     // d x d, row k = reflector k
+    double* Sv = hv + (size_t)d * d;     // d x d, column l = eigenvector l (unnormalised)
+    double* fac = Sv + (size_t)d * d;    // 5 x d x d LU factors, [q][i][l]
+    double* lprev = lamst + it.lam_off;  // d eigenvalues + last largest change
+
+    double a[kW];
+#pragma unroll
+    for (int g = 0; g < kW / 8; g++) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[8 * g + u] = 0.0;
+        if (g < ng) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = 8 * g + u, ic = min(i, d - 1);
+                const double sv = 0.5 * (U[(size_t)ic * d + jc] + U[(size_t)jc * d + ic]);
+                a[i] = (i < d && j < d) ? sv : 0.0;
+            }
+        }
+        row_fence();
+    }
+
+    // lane k writes column k (rows >= k, by groups) to colb
+    auto publish = [&](int k) {
+        if (j == k) {
+#pragma unroll
+            for (int g = 0; g < kW / 8; g++)
+                if (g < ng && 8 * g + 7 >= k) {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) colb[8 * g + u] = a[8 * g + u];
+                }
+        }
+    };
+    // tk (M v)_j over the rows i > lo (v in vbuf, zero elsewhere)
+    auto product = [&](int lo, double tk) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int g = 0; g < kW / 8; g++) {
+            if (g < ng && 8 * g + 7 > lo) {
+                const int i = 8 * g;
+                s0 = fma(a[i + 0], vbuf[i + 0], s0);
+                s1 = fma(a[i + 1], vbuf[i + 1], s1);
+                s2 = fma(a[i + 2], vbuf[i + 2], s2);
+                s3 = fma(a[i + 3], vbuf[i + 3], s3);
+                s0 = fma(a[i + 4], vbuf[i + 4], s0);
+                s1 = fma(a[i + 5], vbuf[i + 5], s1);
+                s2 = fma(a[i + 6], vbuf[i + 6], s2);
+                s3 = fma(a[i + 7], vbuf[i + 7], s3);
+            }
+            row_fence();
+        }
+        return tk * ((s0 + s1) + (s2 + s3));
+    };
+    // M -= v w^T + w v^T on column j over the rows i > lo
+    auto update = [&](int lo, double vj, double wj) {
+#pragma unroll
+        for (int g = 0; g < kW / 8; g++) {
+            if (g < ng && 8 * g + 7 > lo) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int i = 8 * g + u;
+                    a[i] = fma(-wbuf[i], vj, fma(-vbuf[i], wj, a[i]));
+                }
+            }
+            row_fence();
+        }
+    };
+
+    // ---------------------------------------------------------------- U = Q T Q^T
+    for (int k = 0; k + 2 < d; k++) {
+        publish(k);
+        __syncthreads();
+        const double alpha = colb[k + 1];
+        const double xj = (j > k + 1 && j < d) ? colb[j] : 0.0;
+        const double sg = wsum(xj * xj);
+        double tk = 0.0, sc = 0.0, beta = alpha;
+        if (sg != 0.0) {
+            beta = -copysign(sqrt(fma(alpha, alpha, sg)), alpha);
+            tk = (beta - alpha) / beta;
+            sc = 1.0 / (alpha - beta);
+        }
+        if (j == 0) {
+            dd[k] = colb[k];
+            ee[k] = beta;
+            tau[k] = tk;
+        }
+        if (tk != 0.0) {  // uniform
+            const double vj = (j == k + 1) ? 1.0 : xj * sc;
+            if (j > k && j < d) hv[(size_t)k * d + j] = vj;
+            vbuf[j] = vj;
+            __syncthreads();
+            const double pj = product(k, tk);
+            const double kk = 0.5 * tk * wsum(vj * pj);
+            const double wj = (j > k && j < d) ? fma(-kk, vj, pj) : 0.0;
+            wbuf[j] = wj;
+            __syncthreads();
+            update(k, vj, wj);
+        }
+        __syncthreads();  // colb, vbuf, wbuf are rewritten by the next step
+    }
+    for (int k = d > 2 ? d - 2 : 0; k < d; k++) {  // the last two columns
+        publish(k);
+        __syncthreads();
+        if (j == 0) {
+            dd[k] = colb[k];
+            if (k + 1 < d) ee[k] = colb[k + 1];
+        }
+        __syncthreads();
+    }
+
+    // ---------------------------------------------------------------- T = S L S^T
+    double tnorm;
+    bool zero_block;
+    const int ps = tri_eigen<kW>(d, dd, ee, e2, lamv, xs, red, lprev, Sv, fac, &st.status, tnorm, zero_block);
+
+    // ---------------------------------------------------------------- T+ in registers
+    const bool use_pos = zero_block || (d - ps) <= ps;
+    const int lb = use_pos ? ps : 0, le = use_pos ? d : ps;
+    const double sgn = use_pos ? 1.0 : -1.0;
+    {
+        const double tjj = dd[jc], tj1 = (j + 1 < d) ? ee[j] : 0.0, tjm = (j > 0 && j < d) ? ee[j - 1] : 0.0;
+        const bool keep_t = !use_pos && j < d;
+#pragma unroll
+        for (int i = 0; i < kW; i++) {
+            double v = 0.0;
+            if (i == j) v = tjj;
+            else if (i == j + 1) v = tj1;
+            else if (i + 1 == j) v = tjm;
+            a[i] = (keep_t && i < d) ? v : 0.0;
+        }
+    }
+    if (!zero_block) {
+        for (int l0 = lb; l0 < le; l0 += kStageMax) {
+            __syncthreads();
+            for (int e = j; e < kStageMax * kW; e += kW) {
+                const int i = e / kStageMax, m = e - i * kStageMax;
+                stage[m][i] = (l0 + m < le && i < d) ? Sv[(size_t)i * d + l0 + m] : 0.0;
+            }
+            __syncthreads();
+            const int nm = (le - l0 < kStageMax) ? le - l0 : kStageMax;
+            for (int m = 0; m < nm; m++) {
+                const double cj = sgn * lamv[l0 + m] / xs[l0 + m] * stage[m][j];
+#pragma unroll
+                for (int g = 0; g < kW / 8; g++) {
+                    if (g < ng) {
+#pragma unroll
+                        for (int u = 0; u < 8; u++) a[8 * g + u] = fma(stage[m][8 * g + u], cj, a[8 * g + u]);
+                    }
+                    row_fence();
+                }
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- U+ = Q T+ Q^T
+    if (d >= 3) {
+        double vnext = (j > d - 3 && j < d) ? hv[(size_t)(d - 3) * d + j] : 0.0;
+        for (int k = d - 3; k >= 0; k--) {
+            const double tk = tau[k];
+            const double vj = vnext;
+            __syncthreads();  // the readers of the previous vbuf / wbuf are done
+            vbuf[j] = vj;
+            if (k > 0) vnext = (j > k - 1 && j < d) ? hv[(size_t)(k - 1) * d + j] : 0.0;
+            __syncthreads();
+            if (tk != 0.0) {
+                const double pj = product(k, tk);
+                const double kk = 0.5 * tk * wsum(vj * pj);
+                const double wj = j < d ? fma(-kk, vj, pj) : 0.0;
+                wbuf[j] = wj;
+                __syncthreads();
+                update(-1, vj, wj);
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- BPSDP block update
+    double ex = 0.0, dr = 0.0;
+    if (j < d) {
+        double* X = B.x + I.poff + it.off;
+        double* Z = B.z + I.poff + it.off;
+        double* V1 = B.v1 + I.poff + it.off;
+        const double* Cc = B.c + I.poff + it.off;
+        const double inv_mu = 1.0 / mu;
+#pragma unroll
+        for (int g = 0; g < kW / 8; g++) {
+            if (g < ng) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int i = 8 * g + u;
+                    if (i < d) {
+                        const size_t e = (size_t)i * d + j;
+                        const double up = a[i];
+                        const double uij = U[e];
+                        const double sym = 0.5 * (uij + U[(size_t)j * d + i]);
+                        const double xn = up * inv_mu;
+                        const double zn = up - sym;
+                        const double xo = X[e];
+                        const double res = (uij - mu * xo) + zn;
+                        X[e] = xn;
+                        Z[e] = zn;
+                        const double ce = Cc[e];
+                        V1[e] = ce - zn;
+                        ex += ce * xn;
+                        dr += res * res;
+                    }
+                }
+            }
+            row_fence();
+        }
+    }
+    ex = wsum(ex);
+    dr = wsum(dr);
+    if (j == 0) {
+        st.pending = 1;
+        B.proj_part[2 * (size_t)it.gblock] = ex;
+        B.proj_part[2 * (size_t)it.gblock + 1] = dr;
+    }
+}
+
 }  // namespace
 
 int project_tri_max_d(int cls) {
@@ -950,7 +1251,7 @@ void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int n
     if (nitems <= 0) return;
     switch (cls) {
         case 0:
-            hipLaunchKernelGGL((k_project_tri<256, 8, 2, kTileReg>), dim3(nitems), dim3(256), 0, s, B, items, work, lam);
+            hipLaunchKernelGGL(k_project_wave, dim3(nitems), dim3(kW), 0, s, B, items, work, lam);
             break;
         case 1:
             hipLaunchKernelGGL((k_project_tri<512, 8, 4, kTileReg>), dim3(nitems), dim3(512), 0, s, B, items, work, lam);

@@ -190,15 +190,21 @@ __global__ __launch_bounds__(kWG) void k_cg_at(BatchDev B, UnitsDev U) {
     const CgScalars cs = cg_scalars(st);
     const double mu = st.mu;
     const int first = st.cg_iter == 0;
-    const double *r = B.r + I.doff, *w = B.w + I.doff, *p = B.p + I.doff;
+    const CgRow* cg = B.cg + I.doff;
     const double *av1 = B.av1 + I.doff, *ax = B.ax + I.doff, *aaty = B.aaty + I.doff, *b = S.b;
     double acc = 0.0;
     bool lng;
-    int64_t row = gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) {
-        double pv = first ? r0_of(av1[col], b[col], ax[col], aaty[col], mu)
-                          : p_next(r_next(r[col], w[col], cs.al), p[col], cs.be);
-        acc = __fma_rn(cf, pv, acc);
-    }, lng);
+    int64_t row;
+    if (first) {
+        row = gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) {
+            acc = __fma_rn(cf, r0_of(av1[col], b[col], ax[col], aaty[col], mu), acc);
+        }, lng);
+    } else {
+        row = gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) {
+            const double2 rw = *reinterpret_cast<const double2*>(&cg[col].r);
+            acc = __fma_rn(cf, p_next(r_next(rw.x, rw.y, cs.al), cg[col].p, cs.be), acc);
+        }, lng);
+    }
     if (lng) {
         acc = wave_sum(acc);
         if (lane != 0) row = -1;
@@ -237,19 +243,20 @@ __global__ __launch_bounds__(kWG) void k_cg_a(BatchDev B, UnitsDev U) {
     double d4[4] = {0.0, 0.0, 0.0, 0.0};
     if (row >= 0) {
         const int64_t g = I.doff + row;
+        CgRow c = B.cg[g];
         double rn, pn;
         if (s == 0) {
             rn = r0_of(B.av1[g], S.b[row], B.ax[g], B.aaty[g], mu);
             pn = rn;
         } else {
-            double pold = B.p[g];
-            B.y[g] = __fma_rn(cs.al, pold, B.y[g]);
-            rn = r_next(B.r[g], B.w[g], cs.al);
-            pn = p_next(rn, pold, cs.be);
+            c.y = __fma_rn(cs.al, c.p, c.y);
+            rn = r_next(c.r, c.w, cs.al);
+            pn = p_next(rn, c.p, cs.be);
         }
-        B.r[g] = rn;
-        B.p[g] = pn;
-        B.w[g] = acc;
+        c.r = rn;
+        c.w = acc;
+        c.p = pn;
+        B.cg[g] = c;
         d4[0] = rn * rn;
         d4[1] = pn * acc;
         d4[2] = rn * acc;
@@ -330,7 +337,7 @@ __global__ __launch_bounds__(kWG) void k_pre(BatchDev B, UnitsDev U) {
         d4[0] = a0 * a0;
         d4[1] = a0 * e;
         d4[2] = e * e;
-        d4[3] = bv * B.y[g];
+        d4[3] = bv * B.cg[g].y;
     }
     block_partials<4>(d4, red, B.partials + (size_t)wg * 4);
 }
@@ -399,10 +406,11 @@ __global__ __launch_bounds__(kWG) void k_post(BatchDev B, UnitsDev U) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int unit = ((const int*)&U.units[wg])[wv];
     if (unit < 0) return;
-    const double* y = B.y + I.doff;
+    const CgRow* cg = B.cg + I.doff;
     double acc = 0.0;
     bool lng;
-    int64_t row = gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) { acc = __fma_rn(cf, y[col], acc); }, lng);
+    int64_t row =
+        gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) { acc = __fma_rn(cf, cg[col].y, acc); }, lng);
     if (lng) {
         acc = wave_sum(acc);
         if (lane != 0) row = -1;
@@ -698,6 +706,11 @@ __global__ void k_fill(double* p, int64_t n, double v) {
     for (; i < n; i += st) p[i] = v;
 }
 
+__global__ void k_copy_strided(double* dst, int64_t dst_stride, const double* src, int64_t src_stride, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i * dst_stride] = src[i * src_stride];
+}
+
 __global__ void k_copy(double* dst, const double* src, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t st = (int64_t)gridDim.x * blockDim.x;
@@ -806,6 +819,11 @@ static int grid_for(int64_t n) {
 
 void launch_fill(double* p, int64_t n, double v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_copy_strided(double* dst, int64_t dst_stride, const double* src, int64_t src_stride, int64_t n,
+                         hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_copy_strided, dim3(grid_for(n)), dim3(256), 0, s, dst, dst_stride, src, src_stride, n);
 }
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, dst, src, n);

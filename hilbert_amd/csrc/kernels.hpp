@@ -33,6 +33,9 @@ void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int n
 
 void launch_fill(double* p, int64_t n, double v, hipStream_t s);
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
+// dst[i * dst_stride] = src[i * src_stride], i < n (strides in doubles)
+void launch_copy_strided(double* dst, int64_t dst_stride, const double* src, int64_t src_stride, int64_t n,
+                         hipStream_t s);
 void launch_pack_cost(const int32_t* i1, const int32_t* i2, const double* src, double* c, int64_t n,
                       hipStream_t s);
 void launch_cost_adjoint(const int32_t* ptr, const int32_t* ent, const double* xbar, double* dsrc, int64_t nsrc,

@@ -11,13 +11,13 @@ namespace v2h {
 
 namespace {
 // Projection classes by block size.  Blocks of 17..256 take the tridiagonal
-// path (register tiles up to 192, registers + LDS up to 224, the tile in
-// global memory up to 256); the smallest blocks and those above 256 take
+// path (one wavefront per block up to 64, register tiles up to 192,
+// registers + LDS up to 224, the tile in global memory up to 256); the smallest blocks and those above 256 take
 // parallel Jacobi (packed matrix in LDS for d <= 16, in global memory beyond
 // 256).
 constexpr int kGlobalClass = 6;
 constexpr size_t kLdsLimit = 160 * 1024;
-constexpr int kClassThreads[Projector::kClasses] = {64, 256, 512, 768, 1024, 1024, 1024};
+constexpr int kClassThreads[Projector::kClasses] = {64, 64, 512, 768, 1024, 1024, 1024};
 constexpr int kClassTri[Projector::kClasses] = {-1, 0, 1, 2, 3, 4, -1};
 int proj_class(int d) {
     if (d <= 16) return 0;

@@ -140,14 +140,16 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     build_projection();
 
     for (auto* b : {&x_, &z_, &c_, &v1_, &u_, &aty_, &q_}) b->alloc(np_total_);
-    for (auto* b : {&y_, &r_, &p_, &w_, &av1_, &ax_, &aaty_}) b->alloc(nd_total_);
+    for (auto* b : {&av1_, &ax_, &aaty_}) b->alloc(nd_total_);
+    cg_.alloc(nd_total_);
     src_.alloc(ns_total_);
     partials_.alloc(4 * (size_t)std::max(nwgA_, nwgAT_) + 4);
     int64_t smax = 0;
     for (const auto& s : structs_) smax = std::max({smax, s->n_primal, s->n_dual, s->nsrc});
     scratch_.alloc(2 * smax);
     for (auto* b : {&x_, &z_, &c_, &v1_, &u_, &aty_, &q_}) launch_fill(b->p, np_total_, 0.0, stream_);
-    for (auto* b : {&y_, &r_, &p_, &w_, &av1_, &ax_, &aaty_}) launch_fill(b->p, nd_total_, 0.0, stream_);
+    for (auto* b : {&av1_, &ax_, &aaty_}) launch_fill(b->p, nd_total_, 0.0, stream_);
+    launch_fill(reinterpret_cast<double*>(cg_.p), 4 * nd_total_, 0.0, stream_);
     launch_fill(src_.p, ns_total_, 0.0, stream_);
 
     std::vector<StructDev> sd;
@@ -238,7 +240,7 @@ BatchDev Batch::dev() const {
     B.inst = d_inst_.p;
     B.st = d_state_.p;
     B.x = x_.p; B.z = z_.p; B.c = c_.p; B.v1 = v1_.p; B.u = u_.p; B.aty = aty_.p; B.q = q_.p;
-    B.y = y_.p; B.r = r_.p; B.p = p_.p; B.w = w_.p; B.av1 = av1_.p; B.ax = ax_.p; B.aaty = aaty_.p;
+    B.cg = cg_.p; B.av1 = av1_.p; B.ax = ax_.p; B.aaty = aaty_.p;
     B.partials = partials_.p;
     B.proj_part = proj_.proj_part();
     B.opt = opt_;
@@ -428,7 +430,7 @@ void Batch::set_guess_hf(int i) {
 void Batch::reset_state(int i) {
     const InstDev& I = inst_[i];
     const Structure& S = *structs_[inst_sid_[i]];
-    launch_fill(y_.p + I.doff, S.n_dual, 0.0, stream_);
+    launch_fill(reinterpret_cast<double*>(cg_.p + I.doff), 4 * S.n_dual, 0.0, stream_);
     launch_fill(z_.p + I.poff, S.n_primal, 0.0, stream_);
     launch_fill(aty_.p + I.poff, S.n_primal, 0.0, stream_);
     launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);
@@ -585,14 +587,26 @@ static void h2d(double* dst, const double* src, int64_t n, hipStream_t s) {
 }
 
 void Batch::get_primal(int i, double* x) { d2h(x, x_.p + inst_[i].poff, n_primal(i), stream_); }
-void Batch::get_dual(int i, double* y) { d2h(y, y_.p + inst_[i].doff, n_dual(i), stream_); }
+// y is the fourth field of the CG rows: gathered into a contiguous buffer
+void Batch::get_dual(int i, double* y) {
+    DBuf<double> t;
+    t.alloc(n_dual(i));
+    launch_copy_strided(t.p, 1, &cg_.p[inst_[i].doff].y, 4, n_dual(i), stream_);
+    d2h(y, t.p, n_dual(i), stream_);
+}
 void Batch::get_slack(int i, double* z) { d2h(z, z_.p + inst_[i].poff, n_primal(i), stream_); }
 void Batch::get_cost(int i, double* c) { d2h(c, c_.p + inst_[i].poff, n_primal(i), stream_); }
 void Batch::get_b(int i, double* b) const {
     const auto& v = structs_[inst_sid_[i]]->b;
     std::copy(v.begin(), v.end(), b);
 }
-void Batch::set_dual(int i, const double* y) { h2d(y_.p + inst_[i].doff, y, n_dual(i), stream_); }
+void Batch::set_dual(int i, const double* y) {
+    DBuf<double> t;
+    t.alloc(n_dual(i));
+    h2d(t.p, y, n_dual(i), stream_);
+    launch_copy_strided(&cg_.p[inst_[i].doff].y, 4, t.p, 1, n_dual(i), stream_);
+    V2H_HIP(hipStreamSynchronize(stream_));
+}
 void Batch::set_slack(int i, const double* z) {
     const InstDev& I = inst_[i];
     h2d(z_.p + I.poff, z, n_primal(i), stream_);

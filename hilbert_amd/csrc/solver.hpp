@@ -127,7 +127,8 @@ class Batch {
     DBuf<StructDev> d_structs_;
     DBuf<InstDev> d_inst_;
     DBuf<InstState> d_state_;
-    DBuf<double> x_, z_, c_, v1_, u_, aty_, q_, y_, r_, p_, w_, av1_, ax_, aaty_;
+    DBuf<double> x_, z_, c_, v1_, u_, aty_, q_, av1_, ax_, aaty_;
+    DBuf<CgRow> cg_;
     DBuf<double> src_, partials_, scratch_;
     DBuf<int4> unitsA_, unitsAT_;
     DBuf<int> wgA_inst_, wgAT_inst_, wgA_begin_, wgAT_begin_;

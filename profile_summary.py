@@ -49,6 +49,8 @@ def short_name(kernel: str) -> str:
         if nt == "1024":
             return "project_tri224" if ri == "7" else "project_tri256"
         return {"256": "project_tri64", "512": "project_tri128", "768": "project_tri192"}.get(nt, name)
+    if name == "k_project_wave":
+        return "project_tri64"
     return {"k_cg_at": "cg_at", "k_cg_a": "cg_a", "k_pre": "pre", "k_post": "post"}.get(name, name)
 
 

@@ -100,8 +100,12 @@ TIGHT = dict(e_convergence=1e-7, r_convergence=1e-6)
 # RDM elements are only determined to the stopping tolerance divided by the
 # conditioning of the optimal face: on c2v_n6 the oracle's own 1-RDM moves by
 # 1.8e-5 between its LAPACK and Jacobi projections at TIGHT, and by 1.7e-7 at
-# TIGHTER, so the 1e-5 RDM comparison runs at TIGHTER.
+# TIGHTER, so the 1e-5 RDM comparison runs at TIGHTER.  At TIGHTER the
+# oracle's own energies differ by up to 2.2e-8 Eh between those two
+# projections (doublet_n5; 8e-9 on c2v_n6, 2.6e-9 on c1_n4): the energy bound
+# there is 5e-8.
 TIGHTER = dict(e_convergence=1e-9, r_convergence=1e-8)
+TIGHTER_ENERGY = 5e-8
 
 
 def _oracle_solve(prob, h, g, **kw):
@@ -140,7 +144,7 @@ def test_rdms_match_oracle(name, make):
     B = _gpu_solve(prob, h, g, **TIGHTER)
     st = B.status(0)
     assert st["converged"] and st["status_bits"] == 0
-    assert abs(st["energy_primal"] - ref["energy_primal"]) < 1e-8
+    assert abs(st["energy_primal"] - ref["energy_primal"]) < TIGHTER_ENERGY
     x = B.x(0)
     assert np.abs(P.opdm(x) - P.opdm(ref["x"])).max() < 1e-5
     assert abs(P.spin_squared(x) - P.spin_squared(ref["x"])) < 1e-5
