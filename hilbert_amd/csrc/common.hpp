@@ -97,9 +97,21 @@ struct BatchDev {
     SolverOptions opt;
 };
 
+// One wavefront's share of a gather launch over one instance, with what the
+// wavefront needs to start loading.  Slices: width x 64 entries from ent,
+// k-major; meta = width | (valid lanes << 16); row = first row.  Long rows:
+// meta = n | kRecLong, n entries from ent, row = the row.  meta = 0: no work.
+constexpr uint32_t kRecLong = 1u << 31;
+struct alignas(16) UnitRec {
+    const uint32_t* ent;
+    const double* coef;     // [256] coefficient table of the operator
+    uint32_t meta;
+    int32_t row;
+};
+
 // One work unit list per operator direction.
 struct UnitsDev {
-    const int4* units;      // 4 wave units per workgroup (-1 = none)
+    const UnitRec* rec;     // 4 per workgroup
     const int* wg_inst;     // instance of each workgroup
     const int* wg_begin;    // [ninst + 1] first workgroup of each instance
     int nwg;

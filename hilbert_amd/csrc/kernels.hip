@@ -174,79 +174,145 @@ __device__ __forceinline__ void load_coef(double* dst, const double* src) {
 }
 
 // ------------------------------------------------------------------------
+// Gather launches.  Every wavefront starts from its UnitRec (one scalar load:
+// table entries, coefficient table, first row), so its table loads, the
+// workgroup's coefficient table and the instance state are in flight together;
+// the first kPre entries of each lane are loaded before the set-up barrier.
+
+constexpr int kPre = 4;
+
+struct WaveGather {
+    UnitRec R;
+    int lane;
+    bool is_long;
+    uint32_t pre[kPre];
+    __device__ __forceinline__ WaveGather(const UnitRec& r, int l) : R(r), lane(l) {
+        is_long = (R.meta & kRecLong) != 0u;
+        const int w = is_long ? 0 : (int)(R.meta & 0xFFFFu);
+#pragma unroll
+        for (int k = 0; k < kPre; k++) pre[k] = (k < w) ? R.ent[(size_t)k * 64 + lane] : 0u;
+    }
+    // first row of this lane (slices), before the entries are known
+    __device__ __forceinline__ int64_t lane_row() const {
+        const int nvalid = (int)((R.meta >> 16) & 0x7Fu);
+        return (!is_long && lane < nvalid) ? (int64_t)R.row + lane : -1;
+    }
+    // Load(col, coef) over the lane's entries; returns the row the lane
+    // writes (slices: -1 for padding lanes and long-row markers; long rows:
+    // the row, after the caller's wave reduction, on lane 0).
+    template <class Load>
+    __device__ __forceinline__ int64_t run(const double* coef, Load&& load) {
+        if (!is_long) {
+            const int w = (int)(R.meta & 0xFFFFu);
+            const bool skip = w > 0 && (pre[0] >> 24) == kMarker;
+#pragma unroll
+            for (int k = 0; k < kPre; k++)
+                if (k < w) load(pre[k] & 0xFFFFFFu, coef[pre[k] >> 24]);
+            const uint32_t* e = R.ent + lane;
+            int k = kPre;
+            for (; k + 4 <= w; k += 4) {
+                const uint32_t v0 = e[(size_t)(k + 0) * 64], v1 = e[(size_t)(k + 1) * 64];
+                const uint32_t v2 = e[(size_t)(k + 2) * 64], v3 = e[(size_t)(k + 3) * 64];
+                load(v0 & 0xFFFFFFu, coef[v0 >> 24]);
+                load(v1 & 0xFFFFFFu, coef[v1 >> 24]);
+                load(v2 & 0xFFFFFFu, coef[v2 >> 24]);
+                load(v3 & 0xFFFFFFu, coef[v3 >> 24]);
+            }
+            for (; k < w; k++) {
+                const uint32_t v = e[(size_t)k * 64];
+                load(v & 0xFFFFFFu, coef[v >> 24]);
+            }
+            return skip ? -1 : lane_row();
+        }
+        const int n = (int)(R.meta & ~kRecLong);
+        for (int k = lane; k < n; k += 64) {
+            const uint32_t v = R.ent[k];
+            load(v & 0xFFFFFFu, coef[v >> 24]);
+        }
+        return R.row;
+    }
+};
+
+// Coefficient table of the workgroup's structure into LDS (every wavefront of
+// a workgroup belongs to one instance).
+__device__ __forceinline__ void put_coef(double* coef, double v) {
+    coef[threadIdx.x] = v;
+    __syncthreads();
+}
+
 // CG step, part 1: q = A^T p
 __global__ __launch_bounds__(kWG) void k_cg_at(BatchDev B, UnitsDev U) {
+    static_assert(kWG == 256, "one coefficient per thread");
     const int wg = unit_wg(U);
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int inst = U.wg_inst[wg];
+    const UnitRec R = U.rec[(size_t)wg * 4 + wv];
+    const double cv = R.coef[threadIdx.x];
+    WaveGather G(R, lane);
     const InstState& st = B.st[inst];
-    if (!st.active || st.cg_done) return;
     const InstDev I = B.inst[inst];
-    const StructDev& S = B.structs[I.sid];
+    if (!st.active || st.cg_done) return;
     __shared__ double coef[256];
-    load_coef(coef, S.AT.coef);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int unit = ((const int*)&U.units[wg])[wv];
-    if (unit < 0) return;
+    put_coef(coef, cv);
     const CgScalars cs = cg_scalars(st);
     const double mu = st.mu;
-    const int first = st.cg_iter == 0;
-    const CgRow* cg = B.cg + I.doff;
-    const double *av1 = B.av1 + I.doff, *ax = B.ax + I.doff, *aaty = B.aaty + I.doff, *b = S.b;
     double acc = 0.0;
-    bool lng;
     int64_t row;
-    if (first) {
-        row = gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) {
+    if (st.cg_iter == 0) {
+        const double *av1 = B.av1 + I.doff, *ax = B.ax + I.doff, *aaty = B.aaty + I.doff;
+        const double* b = B.structs[I.sid].b;
+        row = G.run(coef, [&](uint32_t col, double cf) {
             acc = __fma_rn(cf, r0_of(av1[col], b[col], ax[col], aaty[col], mu), acc);
-        }, lng);
+        });
     } else {
-        row = gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) {
+        const CgRow* cg = B.cg + I.doff;
+        row = G.run(coef, [&](uint32_t col, double cf) {
             const double2 rw = *reinterpret_cast<const double2*>(&cg[col].r);
             acc = __fma_rn(cf, p_next(r_next(rw.x, rw.y, cs.al), cg[col].p, cs.be), acc);
-        }, lng);
+        });
     }
-    if (lng) {
+    if (G.is_long) {
         acc = wave_sum(acc);
         if (lane != 0) row = -1;
     }
     if (row >= 0) B.q[I.poff + row] = acc;
 }
 
-// CG step, part 2: w = A q, own-row updates and dots; last workgroup advances
-// the CG state of the instance.
+// CG step, part 2: w = A q, own-row updates of y, r, p and the dot partials.
 __global__ __launch_bounds__(kWG) void k_cg_a(BatchDev B, UnitsDev U) {
     const int wg = unit_wg(U);
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int inst = U.wg_inst[wg];
-    InstState& st = B.st[inst];
-    if (!st.active || st.cg_done) return;
+    const UnitRec R = U.rec[(size_t)wg * 4 + wv];
+    const double cv = R.coef[threadIdx.x];
+    WaveGather G(R, lane);
+    const InstState& st = B.st[inst];
     const InstDev I = B.inst[inst];
-    const StructDev& S = B.structs[I.sid];
+    if (!st.active || st.cg_done) return;
     __shared__ double coef[256];
     __shared__ double red[4 * 4];
-    load_coef(coef, S.A.coef);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int unit = ((const int*)&U.units[wg])[wv];
+    put_coef(coef, cv);
     const CgScalars cs = cg_scalars(st);
     const double mu = st.mu;
     const int s = st.cg_iter;
+    // the own row's state is loaded with the gather
+    const int64_t r0 = G.lane_row();
+    CgRow c{0.0, 0.0, 0.0, 0.0};
+    if (r0 >= 0) c = B.cg[I.doff + r0];
     const double* q = B.q + I.poff;
     double acc = 0.0;
-    int64_t row = -1;
-    if (unit >= 0) {
-        bool lng;
-        row = gather_unit(S.A, coef, unit, lane, [&](uint32_t col, double cf) { acc = __fma_rn(cf, q[col], acc); }, lng);
-        if (lng) {
-            acc = wave_sum(acc);
-            if (lane != 0) row = -1;
-        }
+    int64_t row = G.run(coef, [&](uint32_t col, double cf) { acc = __fma_rn(cf, q[col], acc); });
+    if (G.is_long) {
+        acc = wave_sum(acc);
+        if (lane != 0) row = -1;
+        if (row >= 0) c = B.cg[I.doff + row];
     }
     double d4[4] = {0.0, 0.0, 0.0, 0.0};
     if (row >= 0) {
         const int64_t g = I.doff + row;
-        CgRow c = B.cg[g];
         double rn, pn;
         if (s == 0) {
-            rn = r0_of(B.av1[g], S.b[row], B.ax[g], B.aaty[g], mu);
+            rn = r0_of(B.av1[g], B.structs[I.sid].b[row], B.ax[g], B.aaty[g], mu);
             pn = rn;
         } else {
             c.y = __fma_rn(cs.al, c.p, c.y);
@@ -299,32 +365,29 @@ __global__ __launch_bounds__(kWG) void k_cg_reduce(BatchDev B, UnitsDev U) {
 // previous iteration (errors, energies, mu, convergence) and sets up the CG.
 __global__ __launch_bounds__(kWG) void k_pre(BatchDev B, UnitsDev U) {
     const int wg = unit_wg(U);
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int inst = U.wg_inst[wg];
-    InstState& st = B.st[inst];
-    if (!st.active) return;
+    const UnitRec R = U.rec[(size_t)wg * 4 + wv];
+    const double cv = R.coef[threadIdx.x];
+    WaveGather G(R, lane);
+    const InstState& st = B.st[inst];
     const InstDev I = B.inst[inst];
-    const StructDev& S = B.structs[I.sid];
+    if (!st.active) return;
     __shared__ double coef[256];
     __shared__ double red[4 * 4];
-    load_coef(coef, S.A.coef);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int unit = ((const int*)&U.units[wg])[wv];
+    put_coef(coef, cv);
     const double *v1 = B.v1 + I.poff, *x = B.x + I.poff, *aty = B.aty + I.poff;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    int64_t row = -1;
-    if (unit >= 0) {
-        bool lng;
-        row = gather_unit(S.A, coef, unit, lane, [&](uint32_t col, double cf) {
-            a0 = __fma_rn(cf, v1[col], a0);
-            a1 = __fma_rn(cf, x[col], a1);
-            a2 = __fma_rn(cf, aty[col], a2);
-        }, lng);
-        if (lng) {
-            a0 = wave_sum(a0);
-            a1 = wave_sum(a1);
-            a2 = wave_sum(a2);
-            if (lane != 0) row = -1;
-        }
+    int64_t row = G.run(coef, [&](uint32_t col, double cf) {
+        a0 = __fma_rn(cf, v1[col], a0);
+        a1 = __fma_rn(cf, x[col], a1);
+        a2 = __fma_rn(cf, aty[col], a2);
+    });
+    if (G.is_long) {
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        a2 = wave_sum(a2);
+        if (lane != 0) row = -1;
     }
     double d4[4] = {0.0, 0.0, 0.0, 0.0};
     if (row >= 0) {
@@ -332,8 +395,8 @@ __global__ __launch_bounds__(kWG) void k_pre(BatchDev B, UnitsDev U) {
         B.av1[g] = a0;
         B.ax[g] = a1;
         B.aaty[g] = a2;
-        double bv = S.b[row];
-        double e = bv - a1;
+        const double bv = B.structs[I.sid].b[row];
+        const double e = bv - a1;
         d4[0] = a0 * a0;
         d4[1] = a0 * e;
         d4[2] = e * e;
@@ -396,22 +459,20 @@ __global__ __launch_bounds__(kWG) void k_pre_reduce(BatchDev B, UnitsDev U) {
 // After the CG: aty = A^T y, u = mu x + aty - c.
 __global__ __launch_bounds__(kWG) void k_post(BatchDev B, UnitsDev U) {
     const int wg = unit_wg(U);
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int inst = U.wg_inst[wg];
+    const UnitRec R = U.rec[(size_t)wg * 4 + wv];
+    const double cv = R.coef[threadIdx.x];
+    WaveGather G(R, lane);
     const InstState& st = B.st[inst];
-    if (!st.active) return;
     const InstDev I = B.inst[inst];
-    const StructDev& S = B.structs[I.sid];
+    if (!st.active) return;
     __shared__ double coef[256];
-    load_coef(coef, S.AT.coef);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int unit = ((const int*)&U.units[wg])[wv];
-    if (unit < 0) return;
+    put_coef(coef, cv);
     const CgRow* cg = B.cg + I.doff;
     double acc = 0.0;
-    bool lng;
-    int64_t row =
-        gather_unit(S.AT, coef, unit, lane, [&](uint32_t col, double cf) { acc = __fma_rn(cf, cg[col].y, acc); }, lng);
-    if (lng) {
+    int64_t row = G.run(coef, [&](uint32_t col, double cf) { acc = __fma_rn(cf, cg[col].y, acc); });
+    if (G.is_long) {
         acc = wave_sum(acc);
         if (lane != 0) row = -1;
     }

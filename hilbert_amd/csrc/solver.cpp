@@ -98,7 +98,7 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     if (descs.empty()) throw std::invalid_argument("empty batch");
     V2H_HIP(hipSetDevice(device_));
     V2H_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    for (int k = 0; k < kProjClasses; k++) {
+    for (int k = 0; k < kSideStreams; k++) {
         V2H_HIP(hipStreamCreateWithFlags(&pstream_[k], hipStreamNonBlocking));
         V2H_HIP(hipEventCreateWithFlags(&ev_proj_[k], hipEventDisableTiming));
     }
@@ -189,7 +189,7 @@ Batch::~Batch() {
         }
     if (pinned_) (void)hipHostFree(pinned_);
     if (plan_pinned_) (void)hipHostFree(plan_pinned_);
-    for (int k = 0; k < kProjClasses; k++) {
+    for (int k = 0; k < kSideStreams; k++) {
         if (pstream_[k]) (void)hipStreamDestroy(pstream_[k]);
         if (ev_proj_[k]) (void)hipEventDestroy(ev_proj_[k]);
     }
@@ -197,15 +197,43 @@ Batch::~Batch() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+namespace {
+// The UnitRec of one work-unit code (slice s, kLongFlag | long row l, or -1).
+UnitRec unit_rec(const SellHost& h, const uint32_t* ent, const uint32_t* lent, const double* coef, int code) {
+    UnitRec r{ent, coef, 0u, -1};
+    if (code < 0) return r;
+    if (code & kLongFlag) {
+        const int l = code & ~kLongFlag;
+        r.ent = lent + h.long_start[l];
+        r.meta = (h.long_start[l + 1] - h.long_start[l]) | kRecLong;
+        r.row = h.long_row[l];
+        return r;
+    }
+    const uint32_t w = h.slice_start[code + 1] - h.slice_start[code];
+    const int64_t valid = std::min<int64_t>(64, h.nrows - (int64_t)code * 64);
+    if (w > 0xFFFFu || valid <= 0) throw std::logic_error("SELL slice out of range");
+    r.ent = ent + (size_t)h.slice_start[code] * 64;
+    r.meta = w | ((uint32_t)valid << 16);
+    r.row = code * 64;
+    return r;
+}
+}  // namespace
+
 void Batch::build_units() {
     std::vector<int4> uA, uAT;
     std::vector<int> iA, iAT;
+    std::vector<UnitRec> rA, rAT;
     hwgA_begin_.assign(1, 0);
     hwgAT_begin_.assign(1, 0);
     for (int i = 0; i < count(); i++) {
         const Structure& S = *structs_[inst_sid_[i]];
         uA.insert(uA.end(), S.unitsA.begin(), S.unitsA.end());
         uAT.insert(uAT.end(), S.unitsAT.begin(), S.unitsAT.end());
+        for (const int4& g : S.unitsA)
+            for (int code : {g.x, g.y, g.z, g.w}) rA.push_back(unit_rec(S.hA, S.a_ent.p, S.a_lent.p, S.a_coef.p, code));
+        for (const int4& g : S.unitsAT)
+            for (int code : {g.x, g.y, g.z, g.w})
+                rAT.push_back(unit_rec(S.hAT, S.at_ent.p, S.at_lent.p, S.at_coef.p, code));
         iA.insert(iA.end(), S.unitsA.size(), i);
         iAT.insert(iAT.end(), S.unitsAT.size(), i);
         hwgA_begin_.push_back((int)uA.size());
@@ -215,6 +243,8 @@ void Batch::build_units() {
     nwgAT_ = (int)uAT.size();
     unitsA_.upload(uA);
     unitsAT_.upload(uAT);
+    recA_.upload(rA);
+    recAT_.upload(rAT);
     wgA_inst_.upload(iA);
     wgAT_inst_.upload(iAT);
     wgA_begin_.upload(hwgA_begin_);
@@ -249,13 +279,13 @@ BatchDev Batch::dev() const {
 
 UnitsDev Batch::units_a() const {
     const int N = count();
-    return UnitsDev{unitsA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_, N, plan_all_ ? nullptr : plan_dev_.p, planA_m_,
+    return UnitsDev{recA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_, N, plan_all_ ? nullptr : plan_dev_.p, planA_m_,
                     gridA_};
 }
 
 UnitsDev Batch::units_at() const {
     const int N = count();
-    return UnitsDev{unitsAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_, N,
+    return UnitsDev{recAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_, N,
                     plan_all_ ? nullptr : plan_dev_.p + 2 * N + 1, planAT_m_, gridAT_};
 }
 
@@ -529,19 +559,26 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         launch_post(B, units_at(), stream_);
         tstop(3);
         launches_++;
-        // the size classes are independent: largest first, each on its stream
+        // the size classes are independent: largest first, on the streams of
+        // kClassLane, joined back into stream_
         V2H_HIP(hipEventRecord(ev_post_, stream_));
+        bool used[kSideStreams + 1] = {};
         for (int k = kProjClasses - 1; k >= 0; k--) {
             if (!proj_.has(k)) continue;
-            const int kind = 4 + k;
-            V2H_HIP(hipStreamWaitEvent(pstream_[k], ev_post_, 0));
-            tstart(kind, pstream_[k]);
-            proj_.launch(k, B, pstream_[k]);
-            tstop(kind, pstream_[k]);
-            V2H_HIP(hipEventRecord(ev_proj_[k], pstream_[k]));
-            V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[k], 0));
+            const int kind = 4 + k, ln = kClassLane[k];
+            hipStream_t ps = ln == 0 ? stream_ : pstream_[ln - 1];
+            if (ln > 0 && !used[ln]) V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
+            used[ln] = true;
+            tstart(kind, ps);
+            proj_.launch(k, B, ps);
+            tstop(kind, ps);
             launches_++;
         }
+        for (int ln = 1; ln <= kSideStreams; ln++)
+            if (used[ln]) {
+                V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
+                V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[ln - 1], 0));
+            }
         V2H_HIP(hipGetLastError());
     }
     select(false);

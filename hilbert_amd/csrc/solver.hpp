@@ -91,6 +91,14 @@ class Batch {
     // global memory).
     static constexpr int kProjClasses = Projector::kClasses;
     static constexpr int kKinds = 4 + kProjClasses;
+    // The projection classes run concurrently on stream_ (lane 0) and three
+    // side streams: with the runtime's default of four hardware queues per
+    // process, every stream of a batch keeps a queue of its own, so classes
+    // on different lanes do not serialise behind each other.  Lanes by class
+    // (Jacobi <= 16, tri 64 / 128 / 192 / 224 / 256, Jacobi > 256), chosen
+    // to balance the per-iteration class times of the (16e,16o) workloads.
+    static constexpr int kSideStreams = 3;
+    static constexpr int kClassLane[kProjClasses] = {0, 0, 2, 3, 1, 1, 2};
     void set_profiling(bool on) { profile_ = on; }
     // ms/counts: all launches; full_ms/full_counts: launches made while every
     // instance of the batch was active (and, for CG kernels, still iterating)
@@ -130,16 +138,16 @@ class Batch {
     DBuf<double> x_, z_, c_, v1_, u_, aty_, q_, av1_, ax_, aaty_;
     DBuf<CgRow> cg_;
     DBuf<double> src_, partials_, scratch_;
-    DBuf<int4> unitsA_, unitsAT_;
+    DBuf<int4> unitsA_, unitsAT_;        // work-unit codes (per-instance maps)
+    DBuf<UnitRec> recA_, recAT_;          // the same units, resolved (batched gathers)
     DBuf<int> wgA_inst_, wgAT_inst_, wgA_begin_, wgAT_begin_;
     int nwgA_ = 0, nwgAT_ = 0;
     std::vector<int> hwgA_begin_, hwgAT_begin_;
 
     Projector proj_;
-    // the projection classes run concurrently, each on its own stream, joined
-    // back into stream_ by events
-    hipStream_t pstream_[kProjClasses] = {};
-    hipEvent_t ev_post_ = nullptr, ev_proj_[kProjClasses] = {};
+    // side streams of the projection, joined back into stream_ by events
+    hipStream_t pstream_[kSideStreams] = {};
+    hipEvent_t ev_post_ = nullptr, ev_proj_[kSideStreams] = {};
 
     struct KTimer {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
