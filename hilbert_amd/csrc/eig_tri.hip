@@ -43,6 +43,15 @@ __device__ __forceinline__ void lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Lane index recomputed where it is used (the asm keeps the compiler from
+// holding it, and values derived from it, in registers across the loops of
+// the projection, where they would otherwise be spilled and reloaded).
+__device__ __forceinline__ int lane_now() {
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
 // Compiler-only fence between the rows of a register tile: keeps the
 // scheduler from hoisting every row's memory operations at once, which would
 // exceed the register budget.
@@ -592,7 +601,10 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     const InstDev I = B.inst[it.inst];
     const int d = it.d;
     const int t = threadIdx.x, tr = t >> 5, tc = t & 31, lane = t & 63;
-    const int rr = ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);  // reduce_scatter8 entry
+    // the same indices, recomputed at their use inside the Householder steps
+    const int wv_s = __builtin_amdgcn_readfirstlane(t >> 6);
+    auto tr_now = [&]() { return 2 * wv_s + (lane_now() >> 5); };
+    auto tc_now = [&]() { return lane_now() & 31; };
     const double mu = st.mu;
     const double* U = B.u + I.poff + it.off;
     double* hv = work + it.work_off;        // d x d, row k = reflector k
@@ -648,7 +660,9 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             row_fence();
         }
         const double ps = reduce_scatter8(s8);
-        if ((lane & 3) == 0 && rr < RI) pbuf[tr + PR * rr] = tk * ps;
+        const int ln = lane_now();
+        const int rr = ((ln >> 4) & 1) * 4 + ((ln >> 3) & 1) * 2 + ((ln >> 2) & 1);  // reduce_scatter8 entry
+        if ((ln & 3) == 0 && rr < RI) pbuf[tr_now() + PR * rr] = tk * ps;
     };
     // A -= v w^T + w v^T, w = p - kk v, over rows/columns i with lo < i < d:
     // v_i = vbuf[i] * vscale except v_one = 1; p in pbuf.
@@ -671,6 +685,62 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             for (int c = 0; c < CJ; c++) A.at(r, c) = fma(-wi, vc[c], fma(-vi, wc[c], A.at(r, c)));
             row_fence();
         }
+    };
+
+    // Column k of the register matrix to colb[buf]: rows below the diagonal,
+    // zero elsewhere; the diagonal to dd[k].
+    // Row tests are made on the slot index r (a constant after unrolling)
+    // against per-step bounds, so that no row index has to live in a register.
+    auto slot_of = [&](int i, int trr) { return (i >= trr && (i - trr) % PR == 0) ? (i - trr) / PR : -1; };
+    auto put_col_z = [&](int k, int buf) {
+        if (tc_now() == (k & 31)) {
+            const int ck = k >> 5, trr = tr_now();
+            const int r_k = slot_of(k, trr), r_lo = k >= trr ? (k - trr) / PR + 1 : 0;  // rows > k from r_lo
+            const int r_e = d > trr ? (d - 1 - trr) / PR + 1 : 0;                     // rows < d below r_e
+#pragma unroll
+            for (int r = 0; r < RI; r++) {
+                const double v = A.col(r, ck);
+                colb[buf][trr + PR * r] = (r >= r_lo && r < r_e) ? v : 0.0;
+                if (r == r_k) dd[k] = v;
+            }
+        }
+    };
+    // A -= v w^T + w v^T with v_i = vsrc[i] * vscale (v_one = 1) and
+    // w = p - kk v, p in pbuf.  vsrc is zero outside the rows that the step
+    // reduces or transforms, so no row or column needs a mask: rows and
+    // columns outside the trailing matrix receive finite garbage that no later
+    // step reads (they only meet zero entries of v), and padding rows stay zero.
+    auto rank2_z = [&](const double* vsrc, const double (&vc)[CJ], double vscale, int one, double kk) {
+        double wc[CJ];
+        const int tcc = tc_now(), trr = tr_now();
+#pragma unroll
+        for (int c = 0; c < CJ; c++) wc[c] = fma(-kk, vc[c], pbuf[tcc + 32 * c]);
+        const int r_one = one >= 0 ? slot_of(one, trr) : -1;
+        // the LDS loads of row r + 1 are issued before the updates of row r
+        double vn = vsrc[trr], pn = pbuf[trr];
+#pragma unroll
+        for (int r = 0; r < RI; r++) {
+            const double vr = vn, pr = pn;
+            if (r + 1 < RI) {
+                vn = vsrc[trr + PR * (r + 1)];
+                pn = pbuf[trr + PR * (r + 1)];
+            }
+            const double vi = (r == r_one) ? 1.0 : vr * vscale;
+            const double wi = fma(-kk, vi, pr);
+#pragma unroll
+            for (int c = 0; c < CJ; c++) A.at(r, c) = fma(-wi, vc[c], fma(-vi, wc[c], A.at(r, c)));
+            row_fence();
+        }
+    };
+    // sum over the rows i < DMAX of f(i), on every lane
+    auto rows_sum = [&](auto&& f) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < (DMAX + 63) / 64; m++) {
+            const int i = lane + 64 * m;
+            if (DMAX % 64 == 0 || i < DMAX) s += f(i);
+        }
+        return wsum(s);
     };
 
     // One pass over the tile: apply the pending rank-2 update
@@ -701,7 +771,9 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             row_fence();
         }
         const double ps = reduce_scatter8(s8);
-        if ((lane & 3) == 0 && rr < RI) pbuf[tr + PR * rr] = tk * ps;
+        const int ln = lane_now();
+        const int rr = ((ln >> 4) & 1) * 4 + ((ln >> 3) & 1) * 2 + ((ln >> 2) & 1);
+        if ((ln & 3) == 0 && rr < RI) pbuf[tr_now() + PR * rr] = tk * ps;
     };
     // apply the pending update without a product
     auto apply_pending = [&]() {
@@ -723,13 +795,11 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
 
     // ---------------------------------------------------------------- U = Q T Q^T
     if constexpr (!GM) {
-    put_col(0, 0);
+    put_col_z(0, 0);
     for (int k = 0; k + 2 < d; k++) {
         lds_sync();
-        const double* col = colb[k & 1];
-        double sg = 0.0;
-        for (int i = k + 2 + lane; i < DMAX; i += 64) sg = fma(col[i], col[i], sg);  // zero beyond d
-        sg = wsum(sg);
+        const double* col = colb[k & 1];  // zero outside rows k+1 .. d-1
+        const double sg = rows_sum([&](int i) { const double ci = col[i]; return i > k + 1 ? ci * ci : 0.0; });
         const double alpha = col[k + 1];
         double tk = 0.0, sc = 0.0, beta = alpha;
         if (sg != 0.0) {
@@ -742,25 +812,23 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             tau[k] = tk;
         }
         if (tk != 0.0) {  // uniform: every wavefront computed the same sums
-            for (int i = k + 1 + t; i < d; i += NT) hv[(size_t)k * d + i] = (i == k + 1) ? 1.0 : col[i] * sc;
+            const int tn = 64 * wv_s + lane_now();
+            if (tn > k && tn < d) hv[(size_t)k * d + tn] = (tn == k + 1) ? 1.0 : col[tn] * sc;
             double vc[CJ];
+            const int tcc = tc_now();
+            const int c_one = (((k + 1 - tcc) & 31) == 0 && k + 1 >= tcc) ? (k + 1 - tcc) >> 5 : -1;
 #pragma unroll
-            for (int c = 0; c < CJ; c++) {
-                const int j = tc + 32 * c;
-                const double cj = col[j];
-                vc[c] = (j == k + 1) ? 1.0 : ((j > k + 1 && j < d) ? cj * sc : 0.0);
-            }
+            for (int c = 0; c < CJ; c++) vc[c] = (c == c_one) ? 1.0 : col[tcc + 32 * c] * sc;
             matvec(vc, tk);
             lds_sync();
-            double kk = 0.0;
-            for (int i = k + 1 + lane; i < d; i += 64) kk = fma((i == k + 1) ? 1.0 : col[i] * sc, pbuf[i], kk);
-            kk = 0.5 * tk * wsum(kk);
-            rank2(col, vc, sc, k, k + 1, kk);
+            const double kk =
+                0.5 * tk * rows_sum([&](int i) { return ((i == k + 1) ? 1.0 : col[i] * sc) * pbuf[i]; });
+            rank2_z(col, vc, sc, k + 1, kk);
         }
-        put_col(k + 1, (k + 1) & 1);
+        put_col_z(k + 1, (k + 1) & 1);
     }
-    if (d >= 2) put_col(d - 1, 0);  // writes dd[d-1] only
-    __syncthreads();                // also completes the reflector stores
+    if (d >= 2) put_col_z(d - 1, (d - 1) & 1);  // dd[d-1]; keeps column d-2 in the other buffer
+    __syncthreads();                  // also completes the reflector stores
     if (t == 0 && d >= 2) ee[d - 2] = colb[(d - 2) & 1][d - 1];
     __syncthreads();
 
@@ -892,7 +960,7 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
     if constexpr (!GM) {
     if (d >= 3) {
         auto load_v = [&](int k, int buf) {
-            for (int i = t; i < DMAX; i += NT) colb[buf][i] = (i > k && i < d) ? hv[(size_t)k * d + i] : 0.0;
+            if (t < DMAX) colb[buf][t] = (t > k && t < d) ? hv[(size_t)k * d + t] : 0.0;
         };
         load_v(d - 3, (d - 3) & 1);
         for (int k = d - 3; k >= 0; k--) {
@@ -900,16 +968,15 @@ __global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* 
             const double tk = tau[k];
             if (k > 0) load_v(k - 1, (k - 1) & 1);
             if (tk != 0.0) {
-                const double* v = colb[k & 1];
+                const double* v = colb[k & 1];  // zero outside rows k+1 .. d-1
                 double vc[CJ];
+                const int tcc = tc_now();
 #pragma unroll
-                for (int c = 0; c < CJ; c++) vc[c] = v[tc + 32 * c];
+                for (int c = 0; c < CJ; c++) vc[c] = v[tcc + 32 * c];
                 matvec(vc, tk);
                 lds_sync();
-                double kk = 0.0;
-                for (int i = k + 1 + lane; i < d; i += 64) kk = fma(v[i], pbuf[i], kk);
-                kk = 0.5 * tk * wsum(kk);
-                rank2(v, vc, 1.0, -1, -1, kk);
+                const double kk = 0.5 * tk * rows_sum([&](int i) { return v[i] * pbuf[i]; });
+                rank2_z(v, vc, 1.0, -1, kk);
             }
         }
     }
@@ -1047,6 +1114,7 @@ __global__ __launch_bounds__(kW) void k_project_wave(BatchDev B, const ProjItem*
                 }
         }
     };
+    // tk (M v)_j over the rows i > lo (v in vbuf, zero elsewhere)
     // tk (M v)_j over the rows i > lo (v in vbuf, zero elsewhere)
     auto product = [&](int lo, double tk) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
