@@ -202,6 +202,30 @@ def test_configs_against_oracle_golden(name):
     B.close()
 
 
+BATCH_GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "batch64_d2h_oracle.json")
+
+
+def test_mixed_batch_matches_oracle_golden():
+    """The first 16 instances of the mixed (6e,6o)-(12e,12o) D2h batch, solved
+    as one batch, against the oracle's energies of each instance (both at
+    1e-9 / 1e-8, where two solvers agree to the TIGHTER_ENERGY bound)."""
+    from hilbert_amd import configs
+    gold = json.load(open(BATCH_GOLDEN))
+    items = configs.batch64_d2h()[:gold["count"]]
+    B = Batch([p for p, _, _ in items], options(maxiter=200000, e_convergence=gold["e_convergence"],
+                                                 r_convergence=gold["r_convergence"]))
+    for i, (_, h, g) in enumerate(items):
+        B.set_integrals(i, h, g)
+    assert B.solve_to_convergence() == len(items)
+    for i, ((prob, _, _), ref) in enumerate(zip(items, gold["instances"])):
+        assert list(prob.irreps) == ref["irreps"] and prob.nmo == ref["nmo"]
+        st = B.status(i)
+        assert st["converged"] and st["status_bits"] == 0
+        assert abs(st["energy_primal"] - ref["energy_primal"]) < TIGHTER_ENERGY, \
+            (i, st["energy_primal"], ref["energy_primal"])
+    B.close()
+
+
 def _eigh_projection(U, dims):
     """x = U+ per block from LAPACK (numpy eigh) of the symmetrised block."""
     out = np.empty_like(U)
@@ -223,6 +247,10 @@ PROJ_CASES = {
     "c1_10": lambda: _synthetic(10, 30, 1, 11),
     "c2v_16": lambda: _synthetic(16, 48, 4, 12),
     "c1_11": lambda: _synthetic(11, 20, 1, 13),
+    # C1 n=12 (G2aa 288: Jacobi in global memory); first instance of the
+    # large D2h batch, (20e,20o)-(24e,24o) (blocks up to about 180)
+    "c1_12": lambda: _synthetic(12, 20, 1, 14),
+    "batch16_big_0": lambda: __import__("hilbert_amd.configs", fromlist=["x"]).batch16_big(1)[0],
 }
 
 
