@@ -576,8 +576,14 @@ __device__ __forceinline__ int tri_eigen(int d, const double* dd, const double* 
     return d - (int)block_sum<NT>((t < d && !(lamv[t] <= 0.0)) ? 1.0 : 0.0, red);  // also completes Sv, xs
 }
 
+// Waves per SIMD the compiler must leave room for: two workgroups of the
+// 512-thread class share a CU (its hot loops stay spill-free at 128 VGPRs).
+template <int NT>
+constexpr int kTriWavesPerEU = NT == 512 ? 4 : 1;
+
 template <int NT, int RI, int CJ, int MODE>
-__global__ __launch_bounds__(NT) void k_project_tri(BatchDev B, const ProjItem* items, double* work, double* lamst) {
+__global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev B, const ProjItem* items,
+                                                                      double* work, double* lamst) {
     constexpr bool GM = MODE == kTileGlobal;
     constexpr int PR = NT / 32;
     constexpr int DMAX = PR * RI;
