@@ -93,6 +93,8 @@ SOLVE_CASES = [
     ("doublet_n5", lambda: (Problem(5, 3, 2),) + synthetic_integrals(5, 10, 2)),
     ("c2v_n6", lambda: (Problem(6, 3, 3, [0, 0, 1, 2, 3, 3], 4),)
      + synthetic_integrals(6, 12, 3, irreps=[0, 0, 1, 2, 3, 3])),
+    # triplet (na - nb = 2): the non-singlet spin constraints of spin.cc:411-484
+    ("triplet_n6", lambda: (Problem(6, 4, 2),) + synthetic_integrals(6, 12, 7)),
 ]
 
 
