@@ -1,7 +1,7 @@
 """Write tests/golden/oracle_solutions.json: converged oracle solutions of the
 CPU-runnable workloads (energies, iteration counts, spin, 1-RDM and the C1
-spin-free 2-RDM), used by the GPU parity tests so the GPU box does not rerun
-the oracle.  Converged to e_convergence 1e-7, r_convergence 1e-6: at the
+spin-free 2-RDM), used by the GPU parity tests so that they do not rerun the
+oracle.  Converged to e_convergence 1e-7, r_convergence 1e-6: at the
 default 1e-6 / 1e-5 two correct implementations stop anywhere inside a band
 of a few 1e-6 Eh (finite-precision CG on the singular A A^T makes the
 iteration trajectory implementation dependent), at the tighter thresholds
