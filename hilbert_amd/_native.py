@@ -64,6 +64,7 @@ SIGNATURES = {
     "v2h_batch_block_dims": (C.c_int, [_vp, C.c_int, _ip]),
     "v2h_batch_orbital_order": (C.c_int, [_vp, C.c_int, _ip]),
     "v2h_batch_set_integrals": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "v2h_batch_set_df_integrals": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, _ip, _dp]),
     "v2h_batch_set_cost": (C.c_int, [_vp, C.c_int, _dp]),
     "v2h_batch_set_guess": (C.c_int, [_vp, C.c_int, _vp]),
     "v2h_batch_set_guess_hf": (C.c_int, [_vp, C.c_int]),

@@ -74,6 +74,27 @@ class Batch:
         g = np.ascontiguousarray(g, dtype=np.float64).reshape(n ** 4)
         N.check(N.lib().v2h_batch_set_integrals(self._h, i, h, g))
 
+    def set_df_integrals(self, i, h_full, qmo, core, active) -> float:
+        """Density-fitted Hamiltonian contracted on the device; returns E_fzc.
+
+        h_full: (nfull, nfull); qmo: (nq, nfull (nfull + 1) / 2), the packed
+        lower-triangle (Q|mn) layout of the reference's Qmo_; core: full-space
+        indices of the doubly occupied core; active: full-space index of each
+        active orbital, in this instance's orbital order."""
+        h_full = np.ascontiguousarray(h_full, dtype=np.float64)
+        nfull = h_full.shape[0]
+        qmo = np.ascontiguousarray(qmo, dtype=np.float64)
+        if h_full.shape != (nfull, nfull) or qmo.ndim != 2 or qmo.shape[1] != nfull * (nfull + 1) // 2:
+            raise ValueError("h_full must be (nfull, nfull) and qmo (nq, nfull (nfull + 1) / 2)")
+        core = np.ascontiguousarray(core, dtype=np.int32).reshape(-1)
+        active = np.ascontiguousarray(active, dtype=np.int32).reshape(-1)
+        if len(active) != self.problems[i].nmo:
+            raise ValueError("active must list one full-space index per active orbital")
+        e = np.zeros(1)
+        N.check(N.lib().v2h_batch_set_df_integrals(self._h, i, nfull, h_full.reshape(-1), qmo.shape[0],
+                                                   qmo.reshape(-1), len(core), core, active, e))
+        return float(e[0])
+
     def set_cost(self, i, c):
         N.check(N.lib().v2h_batch_set_cost(self._h, i, np.ascontiguousarray(c, dtype=np.float64)))
 

@@ -212,6 +212,19 @@ int v2h_batch_set_integrals(v2h_batch* b, int inst, const double* h, const doubl
     });
 }
 
+int v2h_batch_set_df_integrals(v2h_batch* b, int inst, int nfull, const double* h_full, int nq, const double* qmo,
+                               int ncore, const int* core, const int* active, double* efzc) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(h_full);
+        check_ptr(qmo);
+        check_ptr(active);
+        if (ncore > 0) check_ptr(core);
+        const double e = B(b).set_df_integrals(inst, nfull, h_full, nq, qmo, ncore, core, active);
+        if (efzc) *efzc = e;
+    });
+}
+
 int v2h_batch_set_cost(v2h_batch* b, int inst, const double* c) {
     return guarded([&] {
         check_inst(B(b), inst);

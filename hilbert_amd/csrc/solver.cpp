@@ -16,6 +16,7 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "df.hpp"
 #include "kernels.hpp"
 
 namespace v2h {
@@ -425,6 +426,77 @@ void Batch::set_integrals(int i, const double* h, const double* g) {
     // v1 = c - z
     launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);
     refresh_norms(i);
+}
+
+// The density-fitted intake (df.hip): h' and (pq|rs) of the active space are
+// formed on the device in the source array, then packed like host integrals.
+double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, const double* qmo, int ncore,
+                               const int* core, const int* active) {
+    const InstDev& I = inst_[i];
+    const Structure& S = *structs_[inst_sid_[i]];
+    const int n = S.L.n;
+    if (nfull < 1 || nq < 1 || ncore < 0 || ncore + n > nfull)
+        throw std::invalid_argument("set_df_integrals: need nfull >= ncore + nmo, nq >= 1");
+    std::vector<char> seen(nfull, 0);
+    auto take = [&](int k) {
+        if (k < 0 || k >= nfull) throw std::invalid_argument("set_df_integrals: orbital index out of range");
+        if (seen[k]) throw std::invalid_argument("set_df_integrals: orbital listed twice (core and active are disjoint)");
+        seen[k] = 1;
+    };
+    for (int c = 0; c < ncore; c++) take(core[c]);
+    for (int p = 0; p < n; p++) take(active[p]);
+    auto tri = [](int64_t a, int64_t b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
+    std::vector<int64_t> idx;
+    idx.reserve((size_t)n * n + ncore + (size_t)n * ncore + (size_t)ncore * ncore);
+    for (int p = 0; p < n; p++)
+        for (int q = 0; q < n; q++) idx.push_back(tri(active[p], active[q]));
+    for (int c = 0; c < ncore; c++) idx.push_back(tri(core[c], core[c]));
+    for (int p = 0; p < n; p++)
+        for (int c = 0; c < ncore; c++) idx.push_back(tri(active[p], core[c]));
+    for (int c = 0; c < ncore; c++)
+        for (int c2 = 0; c2 < ncore; c2++) idx.push_back(tri(core[c], core[c2]));
+    const int64_t ntri = (int64_t)nfull * (nfull + 1) / 2;
+    DBuf<double> dq, dh, rows, dvec, kmat, e;
+    DBuf<int64_t> didx;
+    DBuf<int> dact, dcore;
+    dq.alloc((size_t)nq * ntri);
+    V2H_HIP(hipMemcpyAsync(dq.p, qmo, sizeof(double) * nq * ntri, hipMemcpyHostToDevice, stream_));
+    dh.alloc((size_t)nfull * nfull);
+    V2H_HIP(hipMemcpyAsync(dh.p, h_full, sizeof(double) * nfull * nfull, hipMemcpyHostToDevice, stream_));
+    didx.upload(idx);
+    dact.upload(std::vector<int>(active, active + n));
+    dcore.upload(ncore > 0 ? std::vector<int>(core, core + ncore) : std::vector<int>{0});
+    rows.alloc(idx.size() * (size_t)nq);
+    dvec.alloc(nq);
+    kmat.alloc((size_t)n * n);
+    e.alloc(1);
+    DfArgs a{};
+    a.qmo = dq.p;
+    a.h_full = dh.p;
+    a.idx = didx.p;
+    a.active = dact.p;
+    a.core = dcore.p;
+    a.ntri = ntri;
+    a.m_rows = (int64_t)idx.size();
+    a.nfull = nfull;
+    a.nq = nq;
+    a.n = n;
+    a.ncore = ncore;
+    a.rows = rows.p;
+    a.dvec = dvec.p;
+    a.kmat = kmat.p;
+    a.h_out = src_.p + I.soff;
+    a.g_out = src_.p + I.soff + (int64_t)n * n;
+    a.efzc = e.p;
+    launch_df_intake(a, stream_);
+    launch_pack_cost(S.cost1.p, S.cost2.p, src_.p + I.soff, c_.p + I.poff, S.n_primal, stream_);
+    launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);  // v1 = c - z
+    V2H_HIP(hipGetLastError());
+    double efzc = 0.0;
+    V2H_HIP(hipMemcpyAsync(&efzc, e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));  // before the scratch buffers go
+    refresh_norms(i);
+    return efzc;
 }
 
 void Batch::set_cost(int i, const double* c) {

@@ -58,6 +58,11 @@ class Batch {
     // Hamiltonian in the user's orbital order: h[n*n], g[n^4] chemists' (pq|rs).
     void set_integrals(int i, const double* h, const double* g);
     void set_cost(int i, const double* c);        // primal-layout cost vector
+    // Density-fitted Hamiltonian (qmo[nq][nfull (nfull + 1) / 2], h_full
+    // [nfull][nfull]) with core[ncore] doubly occupied; active[nmo] are the
+    // full-space indices of the caller's active orbitals.  Returns E_fzc.
+    double set_df_integrals(int i, int nfull, const double* h_full, int nq, const double* qmo, int ncore,
+                            const int* core, const int* active);
     void set_guess(int i, const double* x);       // nullptr: hilbert's random guess
     void set_guess_hf(int i);                     // hilbert's TPDM_GUESS HF
     void reset_state(int i);                      // y = z = 0, mu = mu0, counters cleared

@@ -10,6 +10,11 @@
   unless sym(p) = sym(q), every auxiliary function Q carries an irrep and
   L^Q_pq = 0 unless sym(p) x sym(q) = sym(Q).
 
+* ``synthetic_df`` / ``pack_qmo`` -- the same construction over a full
+  orbital space, returned as density-fitted three-index integrals in the
+  packed (Q|mn) layout of the reference's Qmo_ (tei.cc:280-283): row Q holds
+  L^Q_mn at INDEX(m, n) = m (m + 1) / 2 + n for m >= n.
+
 Integral layout everywhere: h[p, q], g[p, q, r, s] = (pq|rs) (chemists'
 notation), the layout of hilbert's external-Hamiltonian constructor
 (v2rdm_solver.cc:147-195).
@@ -60,3 +65,34 @@ def synthetic_integrals(n: int, naux: int, rng, irreps=None):
     Lm = L.reshape(naux, n * n)
     g = (Lm.T @ Lm).reshape(n, n, n, n)
     return h, g
+
+
+def pack_qmo(L):
+    """(nq, n, n) symmetric three-index integrals -> (nq, n (n + 1) / 2),
+    packed lower triangle, row-major: (0,0), (1,0), (1,1), (2,0), ..."""
+    L = np.asarray(L, dtype=np.float64)
+    r, c = np.tril_indices(L.shape[1])
+    return np.ascontiguousarray(L[:, r, c])
+
+
+def synthetic_df(nfull: int, naux: int, rng, irreps=None):
+    """(h_full, qmo, L): synthetic_integrals' h and L over nfull orbitals, with
+    L also packed as qmo."""
+    rng = np.random.default_rng(rng)
+    h = rng.normal(0.0, 0.1, (nfull, nfull))
+    h = np.triu(h, 1)
+    h = h + h.T
+    h[np.diag_indices(nfull)] = -rng.uniform(0.5, 2.5, nfull)
+    L = rng.normal(0.0, 0.15, (naux, nfull, nfull))
+    L = np.triu(L) + np.transpose(np.triu(L, 1), (0, 2, 1))
+    if irreps is not None:
+        irr = np.asarray(irreps)
+        nirrep = 1
+        while nirrep <= int(irr.max()):
+            nirrep *= 2
+        h[irr[:, None] != irr[None, :]] = 0.0
+        qirr = rng.integers(0, nirrep, naux)
+        pair = irr[:, None] ^ irr[None, :]
+        for Q in range(naux):
+            L[Q][pair != qirr[Q]] = 0.0
+    return h, pack_qmo(L), L

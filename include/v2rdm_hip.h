@@ -105,6 +105,24 @@ int v2h_batch_orbital_order(const v2h_batch *b, int inst, int *to_user);
  * the caller's orbital order; packed on the device into c. */
 int v2h_batch_set_integrals(v2h_batch *b, int inst, const double *h, const double *g);
 int v2h_batch_set_cost(v2h_batch *b, int inst, const double *c);
+/* Density-fitted molecular Hamiltonian, contracted and packed on the device:
+ * the DF branch of GetIntegrals / RepackIntegrals / FrozenCoreEnergy / TEI
+ * (v2rdm_casscf/tei.cc:58-67, 139-275, 280-283), whose integrals are
+ * TEI(i,j,k,l) = sum_Q qmo[Q*ntri + INDEX(i,j)] qmo[Q*ntri + INDEX(k,l)].
+ *   nfull   orbitals of the integral space (nmo_ - nfrzv_ in the reference)
+ *   h_full  [nfull*nfull] one-electron integrals, read on and above the
+ *           diagonal (oei_full_sym_, tei.cc:124-132)
+ *   qmo     [nq * ntri], ntri = nfull (nfull + 1) / 2: (Q|mn) in the layout
+ *           of the reference's Qmo_ (INDEX(m,n) = packed lower triangle)
+ *   core    [ncore] full-space indices of the doubly occupied (frozen and
+ *           restricted) core orbitals; NULL when ncore = 0
+ *   active  [nmo] full-space index of each active orbital, caller order
+ *   efzc    out (may be NULL): the frozen-core energy, which the reference
+ *           adds to c.x (v2rdm_solver.cc:1385-1386)
+ * The one-electron part of c receives the core Fock terms of
+ * FrozenCoreEnergy (tei.cc:229-268). */
+int v2h_batch_set_df_integrals(v2h_batch *b, int inst, int nfull, const double *h_full, int nq, const double *qmo,
+                               int ncore, const int *core, const int *active, double *efzc);
 /* Primal guess; NULL = hilbert's TPDM_GUESS RANDOM (v2rdm_solver.cc:1799-1806). */
 int v2h_batch_set_guess(v2h_batch *b, int inst, const double *x);
 /* TPDM_GUESS HF (Guess, v2rdm_solver.cc:1711-1798; Q2_constraints_guess,

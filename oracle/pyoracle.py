@@ -64,6 +64,7 @@ def lib():
         L.orc_pack_c.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_pack_c_adjoint.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_hubbard_integrals.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.orc_df_integrals.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _dp, _dp, _dp]
         L.orc_Au.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_ATu.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_guess_random.argtypes = [C.c_long, _dp]
@@ -113,6 +114,28 @@ def dsyevd_pointer():
         except Exception:  # pragma: no cover - scipy is part of the image
             _dsyevd = (None, None)
     return _dsyevd[1]
+
+
+def df_integrals(h_full, qmo, core, active):
+    """Active-space (h with core Fock terms, (pq|rs), E_fzc) from DF integrals.
+
+    h_full: (nfull, nfull); qmo: (nq, nfull (nfull + 1) / 2) in the packed
+    lower-triangle pair order; core / active: full-space orbital indices.
+    Returns (h_act, g_act, efzc) in the order of ``active``."""
+    h_full = np.ascontiguousarray(h_full, dtype=np.float64)
+    nfull = h_full.shape[0]
+    qmo = np.ascontiguousarray(qmo, dtype=np.float64)
+    nq = qmo.shape[0]
+    assert qmo.shape == (nq, nfull * (nfull + 1) // 2)
+    core = np.ascontiguousarray(core, dtype=np.int32)
+    active = np.ascontiguousarray(active, dtype=np.int32)
+    n = len(active)
+    h_act = np.zeros(n * n)
+    g_act = np.zeros(n ** 4)
+    e = np.zeros(1)
+    lib().orc_df_integrals(nfull, h_full.reshape(-1), nq, qmo.reshape(-1), len(core), core, n, active, h_act,
+                           g_act, e)
+    return h_act.reshape(n, n), g_act.reshape(n, n, n, n), float(e[0])
 
 
 class OracleProblem:

@@ -331,6 +331,62 @@ void orc_hubbard_integrals(int n, double t, double U, double *hm, double *g) {
     for (int i = 0; i < n; i++) g[g4(n, i, i, i, i)] = U;
 }
 
+/* psi4's INDEX: packed lower triangle (the same formula as INDEX2,
+ * misc/integraltransform_sort_so_tpdm.cc:41). */
+static long tri_index(long i, long j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+/* tei.cc:280-283: C_DDOT over Q with stride ntri. */
+static double df_tei(const double *qmo, int nq, long ntri, int i, int j, int k, int l) {
+    const double *a = qmo + tri_index(i, j), *b = qmo + tri_index(k, l);
+    double s = 0.0;
+    for (int Q = 0; Q < nq; Q++) s += a[(long)Q * ntri] * b[(long)Q * ntri];
+    return s;
+}
+
+/* oei_full_sym_ keeps i <= j (tei.cc:124-132) and is read through INDEX. */
+static double oei_up(const double *h, int n, int i, int j) {
+    return i <= j ? h[(long)i * n + j] : h[(long)j * n + i];
+}
+
+void orc_df_integrals(int nfull, const double *h_full, int nq, const double *qmo, int ncore, const int *core,
+                      int nact, const int *active, double *h_act, double *g_act, double *efzc) {
+    const long ntri = (long)nfull * (nfull + 1) / 2;
+    /* FrozenCoreEnergy, tei.cc:204-227 */
+    double e = 0.0;
+    for (int a = 0; a < ncore; a++) {
+        const int i = core[a];
+        e += 2.0 * oei_up(h_full, nfull, i, i);
+        for (int b = 0; b < ncore; b++) {
+            const int j = core[b];
+            const double dum1 = df_tei(qmo, nq, ntri, i, i, j, j);
+            const double dum2 = df_tei(qmo, nq, ntri, i, j, i, j);
+            e += 2.0 * dum1 - dum2;
+        }
+    }
+    *efzc = e;
+    /* core repulsion in the one-electron integrals, tei.cc:229-268 */
+    for (int p = 0; p < nact; p++)
+        for (int q = 0; q < nact; q++) {
+            const int i = active[p], j = active[q];
+            double dum = 0.0;
+            for (int c = 0; c < ncore; c++) {
+                const int k = core[c];
+                const double dum1 = df_tei(qmo, nq, ntri, i, j, k, k);
+                const double dum2 = df_tei(qmo, nq, ntri, i, k, j, k);
+                dum += 2.0 * dum1 - dum2;
+            }
+            h_act[(long)p * nact + q] = oei_up(h_full, nfull, i, j) + dum;
+        }
+    /* the (pq|rs) that RepackIntegrals packs, tei.cc:155-197 */
+    const long n = nact;
+    for (int p = 0; p < nact; p++)
+        for (int q = 0; q < nact; q++)
+            for (int r = 0; r < nact; r++)
+                for (int s = 0; s < nact; s++)
+                    g_act[((p * n + q) * n + r) * n + s] =
+                        df_tei(qmo, nq, ntri, active[p], active[q], active[r], active[s]);
+}
+
 /* ---- A u ---------------------------------------------------------------- */
 /* v2rdm_solver.cc:2995-3050 (dispatch order D2, spin, Q2, G2). */
 void orc_Au(const orc_problem *p, const double *u, double *A) {

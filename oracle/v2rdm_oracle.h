@@ -83,6 +83,18 @@ void orc_pack_c_adjoint(const orc_problem *p, const double *xbar, double *hbar, 
 /* Hubbard ring integrals (oei.cc:76-95, tei.cc:285-291). */
 void orc_hubbard_integrals(int n, double t, double U, double *h, double *g);
 
+/* Density-fitted Hamiltonian of the active space with the frozen-core terms:
+ * the DF branch of TEI (tei.cc:280-283, TEI(i,j,k,l) = sum_Q
+ * qmo[Q*ntri + INDEX(i,j)] qmo[Q*ntri + INDEX(k,l)], ntri = nfull(nfull+1)/2,
+ * INDEX the packed lower triangle), FrozenCoreEnergy (tei.cc:201-275) and the
+ * integrals RepackIntegrals reads (tei.cc:139-199).  h_full[nfull*nfull] is
+ * read on and above the diagonal, as oei_full_sym_ holds it (tei.cc:124-132).
+ * core[ncore] and active[nact] are full-space orbital indices; the outputs
+ * are in the order of active[]: h_act[nact*nact] (with the core Fock terms),
+ * g_act[nact^4] chemists' (pq|rs), *efzc the frozen-core energy. */
+void orc_df_integrals(int nfull, const double *h_full, int nq, const double *qmo, int ncore, const int *core,
+                      int nact, const int *active, double *h_act, double *g_act, double *efzc);
+
 /* Constraint maps (v2rdm_solver.cc:2995-3108; d2.cc, spin.cc, q2.cc, g2.cc). */
 void orc_Au(const orc_problem *p, const double *u, double *out);
 void orc_ATu(const orc_problem *p, const double *u, double *out);
