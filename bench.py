@@ -13,7 +13,15 @@ launch processes the instances still iterating) over the mean duration of its
 launches, timed with HIP events on the library's stream inside the timed
 region.
 
-Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1]
+With --workload df the line measures the density-fitted integral intake
+instead (v2h_batch_set_df_integrals): (Q|mn) of a 100-orbital space with 400
+auxiliary functions contracted into the (16e,16o) active-space Hamiltonian
+with 10 frozen-core orbitals and packed into c.  value = intakes/s of the
+device part (HIP events from the first contraction kernel to the packed c,
+(Q|mn) already in HBM); the call rate including the host-to-device copy of
+(Q|mn) is reported beside it.
+
+Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1] [--workload sdp|df]
 """
 from __future__ import annotations
 
@@ -170,6 +178,76 @@ def solve_with_progress(B, label, chunk=1000):
             return done
 
 
+DF_NFULL, DF_NQ, DF_NCORE, DF_NACT = 100, 400, 10, 16
+
+
+def df_flops(n, nq, ncore):
+    """FP64 flops of one intake: (pq|rs) = Pa Pa^T over n^2 active pairs, the
+    exchange M M^T, the Coulomb dots and E_fzc."""
+    return 2.0 * (n ** 4 * nq + n * n * ncore * nq + n * n * nq + ncore * ncore * nq)
+
+
+def bench_df(args):
+    import numpy as np
+    from hilbert_amd.batch import Batch, Problem, options
+    from hilbert_amd.integrals import synthetic_df
+    rng = np.random.default_rng(2024)
+    irr = np.sort(rng.integers(0, 4, DF_NFULL)).astype(np.int32)
+    h, qmo, _ = synthetic_df(DF_NFULL, DF_NQ, rng, irreps=irr)
+    # core: the first DF_NCORE orbitals; active: the next DF_NACT (Pitzer order)
+    core = np.arange(DF_NCORE, dtype=np.int32)
+    active = np.arange(DF_NCORE, DF_NCORE + DF_NACT, dtype=np.int32)
+    prob = Problem(DF_NACT, DF_NACT // 2, DF_NACT // 2, irr[active].tolist(), 4)
+    B = Batch([prob], options())
+    for _ in range(args.warmup):
+        B.set_df_integrals(0, h, qmo, core, active)
+    B.set_profiling(True)
+    B.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        efzc = B.set_df_integrals(0, h, qmo, core, active)
+    call_s = (time.perf_counter() - t0) / args.steps
+    dev_ms, launches = B.kernel_times()["df_intake"]
+    dev_ms /= max(1, launches)
+    c_gpu = B.c(0)
+    B.close()
+    flops = df_flops(DF_NACT, DF_NQ, DF_NCORE)
+    achieved = flops / (dev_ms * 1e-3) / 1e12
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle.pyoracle import OracleProblem, df_integrals
+        P = OracleProblem(DF_NACT, DF_NACT // 2, DF_NACT // 2, irr[active].tolist(), 4)
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            ha, ga, e_ref = df_integrals(h, qmo, core, active)
+            c_ref = P.pack_c(ha, ga)
+            reps += 1
+            if time.perf_counter() - t0 > min(args.cpu_seconds, 10.0):
+                break
+        dt = (time.perf_counter() - t0) / reps
+        err = float(np.abs(c_gpu - c_ref).max())
+        cpu = {"value": 1.0 / dt, "unit": "intakes/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} full intakes (oracle orc_df_integrals + orc_pack_c, one thread); "
+                         f"max |c_gpu - c_oracle| = {err:.1e}, |dE_fzc| = {abs(efzc - e_ref):.1e}"}
+    out = {
+        "metric": "DF integral intakes/s (device part), 100 orbitals, 400 aux, 10 core, (16e,16o) active",
+        "value": 1e3 / dev_ms, "unit": "intakes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dev_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "density-fitted intake: (Q|mn) -> active (pq|rs), core Fock h', E_fzc, packed c",
+                   "nfull": DF_NFULL, "naux": DF_NQ, "ncore": DF_NCORE, "nmo": DF_NACT, "nirrep": 4},
+        "call_rate_with_h2d": {"value": 1.0 / call_s, "unit": "intakes/s",
+                               "note": "whole API call: host-to-device copy of (Q|mn) and c norms included"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "whole device part of one intake (5-6 kernels) against the FP64 peak"},
+        "cpu_baseline": cpu,
+        "host": host_info(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def host_info():
     try:
         model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
@@ -181,7 +259,7 @@ def host_info():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 1; df: 200 intakes)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--maxiter", type=int, default=20000)
@@ -191,6 +269,8 @@ def main():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="after the solve, three plain copies of this many MiB (k_copy), which calibrate "
                          "the traffic counters of a profile (0 = none)")
+    ap.add_argument("--workload", choices=("sdp", "df"), default="sdp",
+                    help="sdp: the batched (16e,16o) solve; df: the density-fitted integral intake")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
@@ -201,6 +281,14 @@ def main():
     if _native.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+
+    if args.workload == "df":
+        if args.steps is None:
+            args.steps = 200
+        bench_df(args)
+        return
+    if args.steps is None:
+        args.steps = 1
 
     items = configs.c2v_16(args.batch)
     B = make_batch(items, args.maxiter)

@@ -144,7 +144,7 @@ class Batch:
         return N.lib().v2h_batch_last_solve_seconds(self._h)
 
     KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project_jacobi16", "project_tri64", "project_tri128",
-                    "project_tri192", "project_tri224", "project_tri256", "project_jacobi_big")
+                    "project_tri192", "project_tri224", "project_tri256", "project_jacobi_big", "df_intake")
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))

@@ -7,12 +7,15 @@
 // RepackIntegrals (tei.cc:139-199) and FrozenCoreEnergy (tei.cc:201-275).
 // Here the same sums are products of pair-major row blocks:
 //   rows   R[r][Q] = qmo[Q ntri + INDEX(i_r, j_r)] for a list of orbital pairs
-//   g      G = Pa Pa^T        Pa: rows of the active pairs (p, q), p, q < n
-//   J      d_Q = sum_k R(k,k)[Q],  J_pq = Pa[pq] . d
-//   K      K = M M^T          M[p]: rows (a_p, k) for every core k, contiguous
+//   g      Gu = Pu Pu^T       Pu: rows of the active pairs p >= q, row
+//                             p (p + 1) / 2 + q (INDEX(a_p, a_q) does not
+//                             depend on the order of p and q)
+//          (pq|rs) = Gu[u(p,q)][u(r,s)], expanded to all n^4
+//   J      d_Q = sum_k R(k,k)[Q],  J_pq = Pu[u(p,q)] . d
+//   K      K_pq = M[p] . M[q]     M[p]: rows (a_p, k) for every core k, contiguous
 //   h'     h_pq + (2 J_pq - K_pq)
 //   E_fzc  2 sum_k h_kk + 2 d.d - sum_(k,l) |R(k,l)|^2
-// G lands directly in the (pq|rs) part of the instance's integral source
+// (pq|rs) lands in the two-electron part of the instance's integral source
 // array and h' in its one-electron part, the layout the cost-vector gather
 // (k_pack_cost) reads, so the packing is the same as for host integrals.
 #include <hip/hip_runtime.h>
@@ -26,7 +29,6 @@ namespace v2h {
 
 namespace {
 
-constexpr int kT = 64;   // output tile of the row-product kernel
 constexpr int kK = 16;   // auxiliary-index chunk staged in LDS
 
 // R[r][Q] = qmo[Q ntri + idx[r]]; Q fastest, so the stores coalesce and the
@@ -42,46 +44,53 @@ __global__ __launch_bounds__(256) void k_df_rows(const double* qmo, int64_t ntri
     }
 }
 
-// C[a][b] = sum_k X[a][k] Y[b][k] for a < ma, b < mb, k < kk: 64 x 64 output
-// tiles, 4 x 4 per thread, k in LDS-staged chunks of 16.  Every output is one
-// sequential fused multiply-add chain over k.
+// C_z[a][b] = sum_{k in split z} X[a][k] Y[b][k] for a < ma, b < mb: (16 TM)^2
+// output tiles, TM x TM per thread of a 16 x 16 workgroup, k in LDS-staged
+// chunks of 16; blockIdx.z = z picks the k range [z kc, (z + 1) kc) and the
+// output plane C + z cplane.  Every output of a plane is one sequential fused
+// multiply-add chain over its k range, whatever TM.
+template <int TM>
 __global__ __launch_bounds__(256) void k_df_gemm_nt(const double* X, int64_t ldx, const double* Y, int64_t ldy,
-                                                    int ma, int mb, int64_t kk, double* C, int64_t ldc) {
+                                                    int ma, int mb, int64_t kk, int64_t kc, double* C, int64_t ldc,
+                                                    int64_t cplane) {
+    constexpr int kT = 16 * TM;
     __shared__ double xs[kK][kT + 1], ys[kK][kT + 1];
     const int a0 = blockIdx.y * kT, b0 = blockIdx.x * kT;
+    const int64_t klo = (int64_t)blockIdx.z * kc, khi = klo + kc < kk ? klo + kc : kk;
+    C += (int64_t)blockIdx.z * cplane;
     const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
-    double acc[4][4];
+    double acc[TM][TM];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-    for (int64_t q0 = 0; q0 < kk; q0 += kK) {
+        for (int j = 0; j < TM; j++) acc[i][j] = 0.0;
+    for (int64_t q0 = klo; q0 < khi; q0 += kK) {
         for (int e = t; e < kT * kK; e += 256) {
             const int r = e / kK, k = e % kK;
             const int64_t q = q0 + k;
-            xs[k][r] = (a0 + r < ma && q < kk) ? X[(int64_t)(a0 + r) * ldx + q] : 0.0;
-            ys[k][r] = (b0 + r < mb && q < kk) ? Y[(int64_t)(b0 + r) * ldy + q] : 0.0;
+            xs[k][r] = (a0 + r < ma && q < khi) ? X[(int64_t)(a0 + r) * ldx + q] : 0.0;
+            ys[k][r] = (b0 + r < mb && q < khi) ? Y[(int64_t)(b0 + r) * ldy + q] : 0.0;
         }
         __syncthreads();
-        const int kn = (int)((kk - q0) < kK ? (kk - q0) : kK);
+        const int kn = (int)((khi - q0) < kK ? (khi - q0) : kK);
         for (int k = 0; k < kn; k++) {
-            double xv[4], yv[4];
+            double xv[TM], yv[TM];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+            for (int i = 0; i < TM; i++) {
                 xv[i] = xs[k][ty + 16 * i];
                 yv[i] = ys[k][tx + 16 * i];
             }
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < TM; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = fma(xv[i], yv[j], acc[i][j]);
+                for (int j = 0; j < TM; j++) acc[i][j] = fma(xv[i], yv[j], acc[i][j]);
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < TM; j++) {
             const int a = a0 + ty + 16 * i, b = b0 + tx + 16 * j;
             if (a < ma && b < mb) C[(int64_t)a * ldc + b] = acc[i][j];
         }
@@ -106,31 +115,102 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
     return s;
 }
 
-// One workgroup: h'[pq] = h_up(a_p, a_q) + (2 Pa[pq] . d - K[pq]) and E_fzc.
-__global__ __launch_bounds__(256) void k_df_finish(const double* h_full, int nfull, const int* active, int n,
-                                                   const int* core, int ncore, const double* Pa, const double* d,
-                                                   const double* Kc, const double* Rcc, int nq, double* h_out,
-                                                   double* efzc) {
+__device__ __forceinline__ int upair(int p, int q) { return p >= q ? p * (p + 1) / 2 + q : q * (q + 1) / 2 + p; }
+
+// (pq|rs) = sum over the planes z, in order, of Gu_z[u(p,q)][u(r,s)], for
+// all p, q, r, s < n
+__global__ __launch_bounds__(256) void k_df_expand(const double* Gu, int nu, int planes, int n, double* g) {
+    const int64_t n2 = (int64_t)n * n, total = n2 * n2, plane = (int64_t)nu * nu;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int pq = (int)(e / n2), rs = (int)(e - (int64_t)pq * n2);
+        const int64_t u = (int64_t)upair(pq / n, pq % n) * nu + upair(rs / n, rs % n);
+        double v = Gu[u];
+        for (int z = 1; z < planes; z++) v += Gu[z * plane + u];
+        g[e] = v;
+    }
+}
+
+// One workgroup per active pair (p, q): h'_pq = h_up(a_p, a_q) + (2 J - K)
+// with J = Pu[u(p,q)] . d and K = M[p] . M[q] (fixed-order block sums).
+__global__ __launch_bounds__(256) void k_df_fock(const double* h_full, int nfull, const int* active, int n, int ncore,
+                                                 const double* Pu, const double* d, const double* M, int nq,
+                                                 double* h_out) {
+    __shared__ double red[4];
+    const int pq = blockIdx.x, p = pq / n, q = pq % n;
+    const int i = active[p], j = active[q];
+    const double h = i <= j ? h_full[(int64_t)i * nfull + j] : h_full[(int64_t)j * nfull + i];
+    if (ncore == 0) {
+        if (threadIdx.x == 0) h_out[pq] = h;
+        return;
+    }
+    const double* pu = Pu + (int64_t)upair(p, q) * nq;
+    double jv = 0.0;
+    for (int Q = threadIdx.x; Q < nq; Q += 256) jv = fma(pu[Q], d[Q], jv);
+    const int64_t kk = (int64_t)ncore * nq;
+    const double *mp = M + p * kk, *mq = M + q * kk;
+    double kv = 0.0;
+    for (int64_t e = threadIdx.x; e < kk; e += 256) kv = fma(mp[e], mq[e], kv);
+    jv = block_sum256(jv, red);
+    kv = block_sum256(kv, red);
+    if (threadIdx.x == 0) h_out[pq] = h + (2.0 * jv - kv);
+}
+
+// One workgroup per core pair row (k, l): sq[row] = |R(k,l)|^2
+__global__ __launch_bounds__(256) void k_df_rownorm(const double* Rcc, int nq, double* sq) {
+    __shared__ double red[4];
+    const double* r = Rcc + (int64_t)blockIdx.x * nq;
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nq; q += 256) s = fma(r[q], r[q], s);
+    s = block_sum256(s, red);
+    if (threadIdx.x == 0) sq[blockIdx.x] = s;
+}
+
+// One workgroup: E_fzc = 2 sum_k h_kk + 2 d.d - sum_(k,l) sq[(k,l)]
+__global__ __launch_bounds__(256) void k_df_efzc(const double* h_full, int nfull, const int* core, int ncore,
+                                                 const double* d, const double* sq, int nq, double* efzc) {
     __shared__ double red[4];
     const int t = threadIdx.x;
-    for (int pq = t; pq < n * n; pq += 256) {
-        const int i = active[pq / n], j = active[pq % n];
-        const double h = i <= j ? h_full[(int64_t)i * nfull + j] : h_full[(int64_t)j * nfull + i];
-        double jv = 0.0;
-        if (ncore > 0)
-            for (int q = 0; q < nq; q++) jv = fma(Pa[(int64_t)pq * nq + q], d[q], jv);
-        h_out[pq] = ncore > 0 ? h + (2.0 * jv - Kc[pq]) : h;
-    }
-    // E_fzc = 2 sum_k h_kk + 2 d.d - sum_(k,l) |R(k,l)|^2
     double s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (int c = t; c < ncore; c += 256) s1 += h_full[(int64_t)core[c] * nfull + core[c]];
     for (int q = t; q < nq && ncore > 0; q += 256) s2 = fma(d[q], d[q], s2);
-    const int64_t nrc = (int64_t)ncore * ncore * nq;
-    for (int64_t e = t; e < nrc; e += 256) s3 = fma(Rcc[e], Rcc[e], s3);
+    for (int r = t; r < ncore * ncore; r += 256) s3 += sq[r];
     s1 = block_sum256(s1, red);
     s2 = block_sum256(s2, red);
     s3 = block_sum256(s3, red);
     if (t == 0) *efzc = 2.0 * s1 + (2.0 * s2 - s3);
+}
+
+// Tiling of the pair-product Gu = Pu Pu^T (nu x nu, k = nq): the widest
+// per-thread tile that still gives at least 256 workgroups (one per CU), and
+// below that, splits of k (each at least 32 long) up to 256 workgroups.
+struct GemmPlan {
+    int tm, splits;
+    int64_t kc;
+};
+GemmPlan gemm_plan(int nu, int64_t kk) {
+    auto tiles = [&](int tm) { return (int64_t)((nu + 16 * tm - 1) / (16 * tm)) * ((nu + 16 * tm - 1) / (16 * tm)); };
+    GemmPlan p{};
+    p.tm = tiles(4) >= 256 ? 4 : (tiles(2) >= 256 ? 2 : 1);
+    const int64_t want = (256 + tiles(p.tm) - 1) / tiles(p.tm), most = kk / 32 > 1 ? kk / 32 : 1;
+    p.splits = (int)(want < most ? want : most);
+    if (p.splits > kDfMaxSplits) p.splits = kDfMaxSplits;
+    p.kc = (kk + p.splits - 1) / p.splits;
+    p.splits = (int)((kk + p.kc - 1) / p.kc);  // no empty split
+    return p;
+}
+
+void launch_gemm_nt(const double* X, int64_t ldx, int m, int64_t kk, double* C, hipStream_t s) {
+    const GemmPlan p = gemm_plan(m, kk);
+    const int tw = 16 * p.tm;
+    const dim3 grid((m + tw - 1) / tw, (m + tw - 1) / tw, p.splits);
+    const int64_t plane = (int64_t)m * m;
+    if (p.tm == 4)
+        hipLaunchKernelGGL(k_df_gemm_nt<4>, grid, dim3(256), 0, s, X, ldx, X, ldx, m, m, kk, p.kc, C, (int64_t)m, plane);
+    else if (p.tm == 2)
+        hipLaunchKernelGGL(k_df_gemm_nt<2>, grid, dim3(256), 0, s, X, ldx, X, ldx, m, m, kk, p.kc, C, (int64_t)m, plane);
+    else
+        hipLaunchKernelGGL(k_df_gemm_nt<1>, grid, dim3(256), 0, s, X, ldx, X, ldx, m, m, kk, p.kc, C, (int64_t)m, plane);
 }
 
 int grid_of(int64_t n) {
@@ -143,22 +223,23 @@ int grid_of(int64_t n) {
 void launch_df_intake(const DfArgs& a, hipStream_t s) {
     const int64_t m = a.m_rows;
     hipLaunchKernelGGL(k_df_rows, dim3(grid_of(m * a.nq)), dim3(256), 0, s, a.qmo, a.ntri, a.nq, a.idx, m, a.rows);
-    const int n = a.n, n2 = n * n;
-    const double* Pa = a.rows;  // rows [0, n^2): active pairs
-    // (pq|rs) = Pa Pa^T straight into the source array, row pq = p n + q
-    hipLaunchKernelGGL(k_df_gemm_nt, dim3((n2 + kT - 1) / kT, (n2 + kT - 1) / kT), dim3(256), 0, s, Pa,
-                       (int64_t)a.nq, Pa, (int64_t)a.nq, n2, n2, (int64_t)a.nq, a.g_out, (int64_t)n2);
-    const double* Rkk = a.rows + (int64_t)n2 * a.nq;                 // ncore rows (k, k)
+    const int n = a.n, nu = n * (n + 1) / 2;
+    const double* Pu = a.rows;                                       // nu rows (a_p, a_q), p >= q
+    const double* Rkk = Pu + (int64_t)nu * a.nq;                     // ncore rows (k, k)
     const double* M = Rkk + (int64_t)a.ncore * a.nq;                 // n x ncore rows (a_p, k)
     const double* Rcc = M + (int64_t)n * a.ncore * a.nq;             // ncore^2 rows (k, l)
-    if (a.ncore > 0) {
+    launch_gemm_nt(Pu, a.nq, nu, a.nq, a.gu, s);
+    const int64_t n4 = (int64_t)n * n * n * n;
+    hipLaunchKernelGGL(k_df_expand, dim3(grid_of(n4)), dim3(256), 0, s, a.gu, nu, gemm_plan(nu, a.nq).splits, n,
+                       a.g_out);
+    if (a.ncore > 0)
         hipLaunchKernelGGL(k_df_dvec, dim3(grid_of(a.nq)), dim3(256), 0, s, Rkk, a.ncore, a.nq, a.dvec);
-        const int64_t kk = (int64_t)a.ncore * a.nq;
-        hipLaunchKernelGGL(k_df_gemm_nt, dim3((n + kT - 1) / kT, (n + kT - 1) / kT), dim3(256), 0, s, M, kk, M, kk,
-                           n, n, kk, a.kmat, (int64_t)n);
-    }
-    hipLaunchKernelGGL(k_df_finish, dim3(1), dim3(256), 0, s, a.h_full, a.nfull, a.active, n, a.core, a.ncore, Pa,
-                       a.dvec, a.kmat, Rcc, a.nq, a.h_out, a.efzc);
+    hipLaunchKernelGGL(k_df_fock, dim3(n * n), dim3(256), 0, s, a.h_full, a.nfull, a.active, n, a.ncore, Pu, a.dvec,
+                       M, a.nq, a.h_out);
+    if (a.ncore > 0)
+        hipLaunchKernelGGL(k_df_rownorm, dim3(a.ncore * a.ncore), dim3(256), 0, s, Rcc, a.nq, a.sq);
+    hipLaunchKernelGGL(k_df_efzc, dim3(1), dim3(256), 0, s, a.h_full, a.nfull, a.core, a.ncore, a.dvec, a.sq, a.nq,
+                       a.efzc);
 }
 
 }  // namespace v2h

@@ -8,9 +8,13 @@
 namespace v2h {
 
 // Device pointers and sizes of one intake.  The pair list idx (INDEX of each
-// row's orbital pair) holds, in order: the n^2 active pairs (a_p, a_q), row
-// p n + q; the ncore pairs (k, k); the n ncore pairs (a_p, k), row p ncore + c;
-// the ncore^2 pairs (k, l), row c ncore + c'.
+// row's orbital pair) holds, in order: the n (n + 1) / 2 active pairs
+// (a_p, a_q) with p >= q, row p (p + 1) / 2 + q; the ncore pairs (k, k); the
+// n ncore pairs (a_p, k), row p ncore + c; the ncore^2 pairs (k, l), row
+// c ncore + c'.
+// Most partial planes of the pair product (splits of the auxiliary index).
+constexpr int kDfMaxSplits = 8;
+
 struct DfArgs {
     const double* qmo;      // [nq][ntri]
     const double* h_full;   // [nfull][nfull], read on and above the diagonal
@@ -21,7 +25,8 @@ struct DfArgs {
     int nfull, nq, n, ncore;
     double* rows;           // [m_rows][nq] scratch
     double* dvec;           // [nq] scratch
-    double* kmat;           // [n][n] scratch
+    double* gu;             // [kDfMaxSplits][nu][nu] scratch, nu = n (n + 1) / 2
+    double* sq;             // [max(ncore^2, 1)] scratch
     double* h_out;          // [n][n]: one-electron part of the source array
     double* g_out;          // [n^2][n^2]: (pq|rs) part of the source array
     double* efzc;           // one double

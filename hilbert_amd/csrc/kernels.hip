@@ -806,12 +806,23 @@ __global__ void k_cost_adjoint(const int32_t* ptr, const int32_t* ent, const dou
 }
 
 // out[0] = sum_i a[i] b[i] in one workgroup (fixed order)
-__global__ __launch_bounds__(1024) void k_dot1(const double* a, const double* b, int64_t n, double* out) {
-    __shared__ double red[16];
+// a . b in two launches with a fixed summation order: kDotParts workgroups
+// each reduce one contiguous chunk into out[1 + part], then one workgroup
+// sums the parts in order into out[0].
+constexpr int kDotParts = 256;
+__global__ __launch_bounds__(256) void k_dot_part(const double* a, const double* b, int64_t n, double* out) {
+    __shared__ double red[4];
+    const int64_t chunk = (n + kDotParts - 1) / kDotParts;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 1024) s += a[i] * b[i];
-    s = block_sum<1024>(s, red);
-    if (threadIdx.x == 0) *out = s;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256) s = fma(a[i], b[i], s);
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) out[1 + blockIdx.x] = s;
+}
+__global__ __launch_bounds__(256) void k_dot_final(double* out) {
+    __shared__ double red[4];
+    const double s = block_sum<256>(out[1 + threadIdx.x], red);
+    if (threadIdx.x == 0) out[0] = s;
 }
 
 }  // namespace
@@ -897,7 +908,9 @@ void launch_cost_adjoint(const int32_t* ptr, const int32_t* ent, const double* x
     if (nsrc > 0) hipLaunchKernelGGL(k_cost_adjoint, dim3(grid_for(nsrc)), dim3(256), 0, s, ptr, ent, xbar, dsrc, nsrc);
 }
 void launch_dot1(const double* a, const double* b, int64_t n, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_dot1, dim3(1), dim3(1024), 0, s, a, b, n, out);
+    static_assert(kDotParts == 256, "k_dot_final sums one part per thread");
+    hipLaunchKernelGGL(k_dot_part, dim3(kDotParts), dim3(256), 0, s, a, b, n, out);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, out);
 }
 
 }  // namespace v2h

@@ -40,6 +40,7 @@ void launch_pack_cost(const int32_t* i1, const int32_t* i2, const double* src, d
                       hipStream_t s);
 void launch_cost_adjoint(const int32_t* ptr, const int32_t* ent, const double* xbar, double* dsrc, int64_t nsrc,
                          hipStream_t s);
+// out[0] = a . b; out[1..256] receive the partial sums
 void launch_dot1(const double* a, const double* b, int64_t n, double* out, hipStream_t s);
 
 }  // namespace v2h

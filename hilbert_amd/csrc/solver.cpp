@@ -147,7 +147,7 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     partials_.alloc(4 * (size_t)std::max(nwgA_, nwgAT_) + 4);
     int64_t smax = 0;
     for (const auto& s : structs_) smax = std::max({smax, s->n_primal, s->n_dual, s->nsrc});
-    scratch_.alloc(2 * smax);
+    scratch_.alloc(std::max<int64_t>(2 * smax, 257));  // launch_dot1 needs 1 + 256
     for (auto* b : {&x_, &z_, &c_, &v1_, &u_, &aty_, &q_}) launch_fill(b->p, np_total_, 0.0, stream_);
     for (auto* b : {&av1_, &ax_, &aaty_}) launch_fill(b->p, nd_total_, 0.0, stream_);
     launch_fill(reinterpret_cast<double*>(cg_.p), 4 * nd_total_, 0.0, stream_);
@@ -447,16 +447,17 @@ double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, c
     for (int p = 0; p < n; p++) take(active[p]);
     auto tri = [](int64_t a, int64_t b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
     std::vector<int64_t> idx;
-    idx.reserve((size_t)n * n + ncore + (size_t)n * ncore + (size_t)ncore * ncore);
+    const int64_t nu = (int64_t)n * (n + 1) / 2;
+    idx.reserve(nu + ncore + (size_t)n * ncore + (size_t)ncore * ncore);
     for (int p = 0; p < n; p++)
-        for (int q = 0; q < n; q++) idx.push_back(tri(active[p], active[q]));
+        for (int q = 0; q <= p; q++) idx.push_back(tri(active[p], active[q]));
     for (int c = 0; c < ncore; c++) idx.push_back(tri(core[c], core[c]));
     for (int p = 0; p < n; p++)
         for (int c = 0; c < ncore; c++) idx.push_back(tri(active[p], core[c]));
     for (int c = 0; c < ncore; c++)
         for (int c2 = 0; c2 < ncore; c2++) idx.push_back(tri(core[c], core[c2]));
     const int64_t ntri = (int64_t)nfull * (nfull + 1) / 2;
-    DBuf<double> dq, dh, rows, dvec, kmat, e;
+    DBuf<double> dq, dh, rows, dvec, gu, sq, e;
     DBuf<int64_t> didx;
     DBuf<int> dact, dcore;
     dq.alloc((size_t)nq * ntri);
@@ -468,7 +469,8 @@ double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, c
     dcore.upload(ncore > 0 ? std::vector<int>(core, core + ncore) : std::vector<int>{0});
     rows.alloc(idx.size() * (size_t)nq);
     dvec.alloc(nq);
-    kmat.alloc((size_t)n * n);
+    gu.alloc((size_t)kDfMaxSplits * nu * nu);
+    sq.alloc(std::max<size_t>(1, (size_t)ncore * ncore));
     e.alloc(1);
     DfArgs a{};
     a.qmo = dq.p;
@@ -484,13 +486,16 @@ double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, c
     a.ncore = ncore;
     a.rows = rows.p;
     a.dvec = dvec.p;
-    a.kmat = kmat.p;
+    a.gu = gu.p;
+    a.sq = sq.p;
     a.h_out = src_.p + I.soff;
     a.g_out = src_.p + I.soff + (int64_t)n * n;
     a.efzc = e.p;
+    tstart(kKindDf);
     launch_df_intake(a, stream_);
     launch_pack_cost(S.cost1.p, S.cost2.p, src_.p + I.soff, c_.p + I.poff, S.n_primal, stream_);
     launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);  // v1 = c - z
+    tstop(kKindDf);
     V2H_HIP(hipGetLastError());
     double efzc = 0.0;
     V2H_HIP(hipMemcpyAsync(&efzc, e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));

@@ -93,9 +93,11 @@ class Batch {
     // HIP-event timing of every kernel launch of solve(), per kernel kind:
     // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..10 projection classes: Jacobi d <= 16,
     // tridiagonal d <= 64 / 128 / 192 / 224 / 256, Jacobi d > 256 (matrix in
-    // global memory).
+    // global memory); 11: the device part of one set_df_integrals (its
+    // contraction kernels and the cost packing).
     static constexpr int kProjClasses = Projector::kClasses;
-    static constexpr int kKinds = 4 + kProjClasses;
+    static constexpr int kKindDf = 4 + kProjClasses;
+    static constexpr int kKinds = kKindDf + 1;
     // The projection classes run concurrently on stream_ (lane 0) and three
     // side streams: with the runtime's default of four hardware queues per
     // process, every stream of a batch keeps a queue of its own, so classes

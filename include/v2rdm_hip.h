@@ -172,11 +172,12 @@ int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
 
 /* HIP-event timing of the kernels launched by v2h_batch_solve, recorded on
- * the library's stream.  ms[11] and counts[11] per kernel kind: 0 iteration
+ * the library's stream.  ms[12] and counts[12] per kernel kind: 0 iteration
  * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..10 PSD
  * projection classes: Jacobi for blocks up to 16, tridiagonal for blocks up to
- * 64 / 128 / 192 / 224 / 256, Jacobi beyond 256 (matrix in global memory).
- * Cumulative since the
+ * 64 / 128 / 192 / 224 / 256, Jacobi beyond 256 (matrix in global memory);
+ * 11 the device part of v2h_batch_set_df_integrals (from the first
+ * contraction kernel to the packed c).  Cumulative since the
  * last reset.  full_ms / full_counts (may be NULL) cover only the launches
  * made while every instance was running (for CG kernels: still iterating)
  * at the previous host poll, i.e. launches that processed the whole batch. */
