@@ -21,7 +21,14 @@ device part (HIP events from the first contraction kernel to the packed c,
 (Q|mn) already in HBM); the call rate including the host-to-device copy of
 (Q|mn) is reported beside it.
 
-Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1] [--workload sdp|df]
+With --workload fcidump it measures the FCIDUMP writer
+(v2h_batch_write_fcidump) on the same 100-orbital space: value = integral
+tables/s of its device part, the pair Gram matrix (Q|mn)^T (Q|mn) over all
+5050 orbital pairs (HIP events around k_pair_gram, (Q|mn) in HBM); the rate
+of whole writes (device pass, copy back, 3.2 M integral records and the RDM
+records written to files) is reported beside it.
+
+Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1] [--workload sdp|df|fcidump]
 """
 from __future__ import annotations
 
@@ -248,6 +255,72 @@ def bench_df(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_fcidump(args):
+    import tempfile
+    import numpy as np
+    from hilbert_amd.batch import Batch, Problem, options
+    from hilbert_amd.integrals import synthetic_df
+    rng = np.random.default_rng(2024)
+    irr = np.sort(rng.integers(0, 4, DF_NFULL)).astype(np.int32)
+    h, qmo, _ = synthetic_df(DF_NFULL, DF_NQ, rng, irreps=irr)
+    core = np.arange(DF_NCORE, dtype=np.int32)
+    active = np.arange(DF_NCORE, DF_NCORE + DF_NACT, dtype=np.int32)
+    prob = Problem(DF_NACT, DF_NACT // 2, DF_NACT // 2, irr[active].tolist(), 4)
+    B = Batch([prob], options())
+    B.set_guess(0, None)
+    ntri = DF_NFULL * (DF_NFULL + 1) // 2
+    with tempfile.TemporaryDirectory() as d:
+        fi, fr = os.path.join(d, "bench.int"), os.path.join(d, "bench.rdm")
+        for _ in range(args.warmup):
+            B.write_fcidump(0, fi, fr, h, qmo, core, active, enuc=1.0, full_irreps=irr)
+        B.set_profiling(True)
+        B.reset_kernel_times()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            n_int, n_rdm = B.write_fcidump(0, fi, fr, h, qmo, core, active, enuc=1.0, full_irreps=irr)
+        call_s = (time.perf_counter() - t0) / args.steps
+        x = B.x(0)
+    dev_ms, launches = B.kernel_times()["pair_gram"]
+    dev_ms /= max(1, launches)
+    B.close()
+    flops = 2.0 * DF_NQ * ntri * (ntri + 1) / 2       # every pair product a <= b
+    achieved = flops / (dev_ms * 1e-3) / 1e12
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle.pyoracle import OracleProblem, fcidump_integrals, fcidump_map
+        space = np.full(DF_NFULL, 2, dtype=np.int32)
+        space[core] = 0
+        space[active] = 1
+        lab = fcidump_map(space, irr, 4)
+        P = OracleProblem(DF_NACT, DF_NACT // 2, DF_NACT // 2, irr[active].tolist(), 4)
+        t0 = time.perf_counter()
+        fcidump_integrals(h, qmo, 1.0, lab, irr)
+        P.fcidump_rdms(x, active, lab, DF_NCORE)
+        dt = time.perf_counter() - t0
+        cpu = {"value": 1.0 / dt, "unit": "FCIDUMP record sets/s", "cores": 1, "kind": "port",
+               "sample": "one full set of integral and RDM records in memory (oracle orc_fcidump_*, one thread; "
+                         "the oracle counts the records in a first pass and fills them in a second)"}
+    out = {
+        "metric": "FCIDUMP integral tables/s (device pair Gram), 100 orbitals, 400 aux",
+        "value": 1e3 / dev_ms, "unit": "tables/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dev_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "FCIDUMP writer: (pq|rs) for all 5050 x 5050 pairs from (Q|mn), records of "
+                               "integrals and of the (16e,16o) RDMs written to files",
+                   "nfull": DF_NFULL, "naux": DF_NQ, "ncore": DF_NCORE, "nmo": DF_NACT, "nirrep": 4,
+                   "integral_records": n_int, "rdm_records": n_rdm},
+        "whole_write": {"value": 1.0 / call_s, "unit": "writes/s",
+                        "note": "copy-in of (Q|mn), device Gram, 204 MB copy back, record loops, file writes"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "k_pair_gram, algorithmic flops of the pair products a <= b (FP64 FMA on VALU; "
+                             "MI355X's FP64 vector and matrix peaks are equal)"},
+        "cpu_baseline": cpu,
+        "host": host_info(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def host_info():
     try:
         model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
@@ -269,8 +342,9 @@ def main():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="after the solve, three plain copies of this many MiB (k_copy), which calibrate "
                          "the traffic counters of a profile (0 = none)")
-    ap.add_argument("--workload", choices=("sdp", "df"), default="sdp",
-                    help="sdp: the batched (16e,16o) solve; df: the density-fitted integral intake")
+    ap.add_argument("--workload", choices=("sdp", "df", "fcidump"), default="sdp",
+                    help="sdp: the batched (16e,16o) solve; df: the density-fitted integral intake; "
+                         "fcidump: the FCIDUMP writer")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
@@ -286,6 +360,11 @@ def main():
         if args.steps is None:
             args.steps = 200
         bench_df(args)
+        return
+    if args.workload == "fcidump":
+        if args.steps is None:
+            args.steps = 5
+        bench_fcidump(args)
         return
     if args.steps is None:
         args.steps = 1

@@ -65,6 +65,8 @@ SIGNATURES = {
     "v2h_batch_orbital_order": (C.c_int, [_vp, C.c_int, _ip]),
     "v2h_batch_set_integrals": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_set_df_integrals": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, _ip, _dp]),
+    "v2h_batch_write_fcidump": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, _dp, C.c_int, _ip, _ip,
+                                          C.c_double, C.c_char_p, C.c_char_p, _vp, _vp]),
     "v2h_batch_set_cost": (C.c_int, [_vp, C.c_int, _dp]),
     "v2h_batch_set_guess": (C.c_int, [_vp, C.c_int, _vp]),
     "v2h_batch_set_guess_hf": (C.c_int, [_vp, C.c_int]),

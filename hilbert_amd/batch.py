@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -29,6 +30,10 @@ def options(e_convergence=1e-6, r_convergence=1e-5, cg_convergence=1e-9, cg_maxi
             mu_update_frequency=1000, mu0=1.0) -> N.SdpOptions:
     return N.SdpOptions(e_convergence, r_convergence, cg_convergence, cg_maxiter, maxiter,
                         mu_update_frequency, mu0)
+
+
+# one record of the FCIDUMP files (v2h_batch_write_fcidump)
+FCIDUMP_RECORD = np.dtype([("value", "<f8"), ("labels", "<i4", (4,))])
 
 
 class Batch:
@@ -95,6 +100,32 @@ class Batch:
                                                    qmo.reshape(-1), len(core), core, active, e))
         return float(e[0])
 
+    def write_fcidump(self, i, int_path, rdm_path, h_full, qmo, core, active, enuc=0.0, full_irreps=None):
+        """hilbert's FCIDUMP files (integrals, RDMs of the current x) for
+        instance i from DF integrals; returns the two record counts.  Each
+        record: float64 value, four int32 labels (numpy dtype
+        ``FCIDUMP_RECORD``)."""
+        h_full = np.ascontiguousarray(h_full, dtype=np.float64)
+        nfull = h_full.shape[0]
+        qmo = np.ascontiguousarray(qmo, dtype=np.float64)
+        if h_full.shape != (nfull, nfull) or qmo.ndim != 2 or qmo.shape[1] != nfull * (nfull + 1) // 2:
+            raise ValueError("h_full must be (nfull, nfull) and qmo (nq, nfull (nfull + 1) / 2)")
+        core = np.ascontiguousarray(core, dtype=np.int32).reshape(-1)
+        active = np.ascontiguousarray(active, dtype=np.int32).reshape(-1)
+        if len(active) != self.problems[i].nmo:
+            raise ValueError("active must list one full-space index per active orbital")
+        irr = None
+        if full_irreps is not None:
+            irr = np.ascontiguousarray(full_irreps, dtype=np.int32).reshape(-1)
+            if len(irr) != nfull:
+                raise ValueError("full_irreps must have one label per orbital")
+        n_int, n_rdm = C.c_int64(), C.c_int64()
+        N.check(N.lib().v2h_batch_write_fcidump(
+            self._h, i, nfull, None if irr is None else irr.ctypes.data_as(C.c_void_p), h_full.reshape(-1),
+            qmo.shape[0], qmo.reshape(-1), len(core), core, active, float(enuc), os.fsencode(int_path),
+            os.fsencode(rdm_path), C.byref(n_int), C.byref(n_rdm)))
+        return n_int.value, n_rdm.value
+
     def set_cost(self, i, c):
         N.check(N.lib().v2h_batch_set_cost(self._h, i, np.ascontiguousarray(c, dtype=np.float64)))
 
@@ -144,7 +175,8 @@ class Batch:
         return N.lib().v2h_batch_last_solve_seconds(self._h)
 
     KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project_jacobi16", "project_tri64", "project_tri128",
-                    "project_tri192", "project_tri224", "project_tri256", "project_jacobi_big", "df_intake")
+                    "project_tri192", "project_tri224", "project_tri256", "project_jacobi_big", "df_intake",
+                    "pair_gram")
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))

@@ -225,6 +225,33 @@ int v2h_batch_set_df_integrals(v2h_batch* b, int inst, int nfull, const double* 
     });
 }
 
+int v2h_batch_write_fcidump(v2h_batch* b, int inst, int nfull, const int* full_irreps, const double* h_full, int nq,
+                            const double* qmo, int ncore, const int* core, const int* active, double enuc,
+                            const char* int_path, const char* rdm_path, int64_t* n_int, int64_t* n_rdm) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(h_full);
+        check_ptr(qmo);
+        check_ptr(active);
+        check_ptr(int_path);
+        check_ptr(rdm_path);
+        if (ncore > 0) check_ptr(core);
+        v2h::FcidumpArgs a;
+        a.nfull = nfull;
+        a.full_irreps = full_irreps;
+        a.h_full = h_full;
+        a.nq = nq;
+        a.qmo = qmo;
+        a.ncore = ncore;
+        a.core = core;
+        a.active = active;
+        a.enuc = enuc;
+        a.int_path = int_path;
+        a.rdm_path = rdm_path;
+        B(b).write_fcidump(inst, a, n_int, n_rdm);
+    });
+}
+
 int v2h_batch_set_cost(v2h_batch* b, int inst, const double* c) {
     return guarded([&] {
         check_inst(B(b), inst);

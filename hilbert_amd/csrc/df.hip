@@ -213,12 +213,70 @@ void launch_gemm_nt(const double* X, int64_t ldx, int m, int64_t kk, double* C, 
         hipLaunchKernelGGL(k_df_gemm_nt<1>, grid, dim3(256), 0, s, X, ldx, X, ldx, m, m, kk, p.kc, C, (int64_t)m, plane);
 }
 
+// G[a][b] = sum_Q qmo[Q][a] qmo[Q][b] over the pair index of every orbital
+// pair (the full two-electron integral table of the FCIDUMP writer): one
+// workgroup per 64 x 64 tile on or above the diagonal (tile rows ta <= tile
+// columns tb), 4 x 4 outputs per thread, Q staged 16 rows at a time with the
+// loads running along the pair index (coalesced).  Each output is one
+// sequential fused multiply-add chain over Q.
+__global__ __launch_bounds__(256) void k_pair_gram(const double* qmo, int64_t ntri, int nq, double* G) {
+    __shared__ double xs[kK][64], ys[kK][64];
+    const int64_t k = blockIdx.x;
+    int64_t tb = (int64_t)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+    while (tb * (tb + 1) / 2 > k) tb--;
+    while ((tb + 1) * (tb + 2) / 2 <= k) tb++;
+    const int64_t ta = k - tb * (tb + 1) / 2;
+    const int64_t a0 = ta * 64, b0 = tb * 64;
+    const int t = threadIdx.x, ty = t >> 4, tx = t & 15;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int q0 = 0; q0 < nq; q0 += kK) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int e = t + 256 * i, r = e & 63, kq = e >> 6;
+            const int q = q0 + kq;
+            xs[kq][r] = (a0 + r < ntri && q < nq) ? qmo[(int64_t)q * ntri + a0 + r] : 0.0;
+            ys[kq][r] = (b0 + r < ntri && q < nq) ? qmo[(int64_t)q * ntri + b0 + r] : 0.0;
+        }
+        __syncthreads();
+        const int kn = nq - q0 < kK ? nq - q0 : kK;
+        for (int kq = 0; kq < kn; kq++) {
+            double xv[4], yv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                xv[i] = xs[kq][ty + 16 * i];
+                yv[i] = ys[kq][tx + 16 * i];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = fma(xv[i], yv[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t a = a0 + ty + 16 * i, b = b0 + tx + 16 * j;
+            if (a < ntri && b < ntri) G[a * ntri + b] = acc[i][j];
+        }
+}
+
 int grid_of(int64_t n) {
     int64_t g = (n + 255) / 256;
     return (int)(g < 2048 ? (g > 0 ? g : 1) : 2048);
 }
 
 }  // namespace
+
+void launch_pair_gram(const double* qmo, int64_t ntri, int nq, double* G, hipStream_t s) {
+    const int64_t T = (ntri + 63) / 64;
+    hipLaunchKernelGGL(k_pair_gram, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, qmo, ntri, nq, G);
+}
 
 void launch_df_intake(const DfArgs& a, hipStream_t s) {
     const int64_t m = a.m_rows;

@@ -34,4 +34,8 @@ struct DfArgs {
 
 void launch_df_intake(const DfArgs& a, hipStream_t s);
 
+// G[a][b] = sum_Q qmo[Q][a] qmo[Q][b] for a, b < ntri in the 64 x 64 tiles on
+// and above the diagonal (so every a <= b); G is ntri x ntri, row-major.
+void launch_pair_gram(const double* qmo, int64_t ntri, int nq, double* G, hipStream_t s);
+
 }  // namespace v2h

@@ -43,6 +43,22 @@ struct InstStatus {
 
 using Monitor = std::function<void(int inst, const InstStatus&)>;
 
+// Inputs of Batch::write_fcidump: the density-fitted integrals of the full
+// orbital space and the placement of the instance's orbitals in it.
+struct FcidumpArgs {
+    int nfull = 0;
+    const int* full_irreps = nullptr;  // [nfull], nullptr = C1
+    const double* h_full = nullptr;    // [nfull][nfull]
+    int nq = 0;
+    const double* qmo = nullptr;       // [nq][nfull (nfull + 1) / 2]
+    int ncore = 0;
+    const int* core = nullptr;         // [ncore] doubly occupied
+    const int* active = nullptr;       // [nmo] full index of each orbital of the instance
+    double enuc = 0.0;
+    const char* int_path = nullptr;
+    const char* rdm_path = nullptr;
+};
+
 class Batch {
   public:
     Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, int device);
@@ -63,6 +79,9 @@ class Batch {
     // full-space indices of the caller's active orbitals.  Returns E_fzc.
     double set_df_integrals(int i, int nfull, const double* h_full, int nq, const double* qmo, int ncore,
                             const int* core, const int* active);
+    // The integral and RDM files of hilbert's FCIDUMP option for instance i
+    // (its current primal vector); record counts out (may be null).
+    void write_fcidump(int i, const FcidumpArgs& a, int64_t* n_int, int64_t* n_rdm);
     void set_guess(int i, const double* x);       // nullptr: hilbert's random guess
     void set_guess_hf(int i);                     // hilbert's TPDM_GUESS HF
     void reset_state(int i);                      // y = z = 0, mu = mu0, counters cleared
@@ -94,10 +113,12 @@ class Batch {
     // 0 pre, 1 cg_at, 2 cg_a, 3 post, 4..10 projection classes: Jacobi d <= 16,
     // tridiagonal d <= 64 / 128 / 192 / 224 / 256, Jacobi d > 256 (matrix in
     // global memory); 11: the device part of one set_df_integrals (its
-    // contraction kernels and the cost packing).
+    // contraction kernels and the cost packing); 12: the pair Gram matrix of
+    // write_fcidump.
     static constexpr int kProjClasses = Projector::kClasses;
     static constexpr int kKindDf = 4 + kProjClasses;
-    static constexpr int kKinds = kKindDf + 1;
+    static constexpr int kKindGram = kKindDf + 1;
+    static constexpr int kKinds = kKindGram + 1;
     // The projection classes run concurrently on stream_ (lane 0) and three
     // side streams: with the runtime's default of four hardware queues per
     // process, every stream of a batch keeps a queue of its own, so classes

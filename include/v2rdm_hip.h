@@ -123,6 +123,25 @@ int v2h_batch_set_cost(v2h_batch *b, int inst, const double *c);
  * FrozenCoreEnergy (tei.cc:229-268). */
 int v2h_batch_set_df_integrals(v2h_batch *b, int inst, int nfull, const double *h_full, int nq, const double *qmo,
                                int ncore, const int *core, const int *active, double *efzc);
+/* The binary integral and RDM files of the FCIDUMP option (v2RDMSolver::FCIDUMP,
+ * v2rdm_casscf/fcidump.cc:60-243) for one instance, from density-fitted
+ * integrals (the reference's FCIDUMP works with DF integrals only).  Each
+ * file is a sequence of 24-byte records: a double and four int32 labels.
+ *   int_path: every (pq|rs) with p <= q, r <= s, INDEX(p,q) <= INDEX(r,s) and
+ *     equal pair symmetry, labels (p,q,r,s); then h_pq for p, q of one irrep,
+ *     labels (p,q,0,0); then (enuc,0,0,0,0).  Labels are 1-based and order
+ *     the orbitals core, active, virtual, each by irrep, then index.
+ *   rdm_path: the spin-free 2-RDM D2ab(ij,kl) + D2ab(ji,lk) +
+ *     s (D2aa + D2bb)(ij,kl) over the ab geminals (labels i,j,k,l relative
+ *     to the core), then the spin-summed 1-RDM (labels i,j,0,0), of the
+ *     instance's current primal vector.
+ * full_irreps may be NULL (C1); core may be NULL when ncore = 0; active[u] is
+ * the full-space index of the instance's orbital u.  The reference writes
+ * after transforming to natural orbitals; pass integrals and x in the basis
+ * wanted.  n_int / n_rdm (may be NULL) receive the record counts. */
+int v2h_batch_write_fcidump(v2h_batch *b, int inst, int nfull, const int *full_irreps, const double *h_full, int nq,
+                            const double *qmo, int ncore, const int *core, const int *active, double enuc,
+                            const char *int_path, const char *rdm_path, int64_t *n_int, int64_t *n_rdm);
 /* Primal guess; NULL = hilbert's TPDM_GUESS RANDOM (v2rdm_solver.cc:1799-1806). */
 int v2h_batch_set_guess(v2h_batch *b, int inst, const double *x);
 /* TPDM_GUESS HF (Guess, v2rdm_solver.cc:1711-1798; Q2_constraints_guess,
@@ -172,12 +191,13 @@ int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
 
 /* HIP-event timing of the kernels launched by v2h_batch_solve, recorded on
- * the library's stream.  ms[12] and counts[12] per kernel kind: 0 iteration
+ * the library's stream.  ms[13] and counts[13] per kernel kind: 0 iteration
  * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..10 PSD
  * projection classes: Jacobi for blocks up to 16, tridiagonal for blocks up to
  * 64 / 128 / 192 / 224 / 256, Jacobi beyond 256 (matrix in global memory);
  * 11 the device part of v2h_batch_set_df_integrals (from the first
- * contraction kernel to the packed c).  Cumulative since the
+ * contraction kernel to the packed c), 12 the pair Gram matrix of
+ * v2h_batch_write_fcidump.  Cumulative since the
  * last reset.  full_ms / full_counts (may be NULL) cover only the launches
  * made while every instance was running (for CG kernels: still iterating)
  * at the previous host poll, i.e. launches that processed the whole batch. */

@@ -65,6 +65,12 @@ def lib():
         L.orc_pack_c_adjoint.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_hubbard_integrals.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.orc_df_integrals.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _dp, _dp, _dp]
+        L.orc_fcidump_map.argtypes = [C.c_int, C.c_int, C.c_void_p, _ip, _ip]
+        L.orc_fcidump_integrals.restype = C.c_long
+        L.orc_fcidump_integrals.argtypes = [C.c_int, C.c_void_p, _ip, _dp, C.c_int, _dp, C.c_double, C.c_void_p,
+                                            C.c_void_p]
+        L.orc_fcidump_rdms.restype = C.c_long
+        L.orc_fcidump_rdms.argtypes = [C.c_void_p, _dp, _ip, _ip, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_Au.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_ATu.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_guess_random.argtypes = [C.c_long, _dp]
@@ -136,6 +142,35 @@ def df_integrals(h_full, qmo, core, active):
     lib().orc_df_integrals(nfull, h_full.reshape(-1), nq, qmo.reshape(-1), len(core), core, n, active, h_act,
                            g_act, e)
     return h_act.reshape(n, n), g_act.reshape(n, n, n, n), float(e[0])
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def fcidump_map(space, full_irreps=None, nirrep=1):
+    """1-based FCIDUMP label of every orbital (core, active, virtual; then irrep)."""
+    space = np.ascontiguousarray(space, dtype=np.int32)
+    irr = None if full_irreps is None else np.ascontiguousarray(full_irreps, dtype=np.int32)
+    out = np.zeros(len(space), dtype=np.int32)
+    lib().orc_fcidump_map(len(space), nirrep, _ptr(irr), space, out)
+    return out
+
+
+def fcidump_integrals(h_full, qmo, enuc, lab_map, full_irreps=None):
+    """(values, labels[:, 4]) of the integral file, in writing order."""
+    h_full = np.ascontiguousarray(h_full, dtype=np.float64)
+    nfull = h_full.shape[0]
+    qmo = np.ascontiguousarray(qmo, dtype=np.float64)
+    irr = None if full_irreps is None else np.ascontiguousarray(full_irreps, dtype=np.int32)
+    lab_map = np.ascontiguousarray(lab_map, dtype=np.int32)
+    n = lib().orc_fcidump_integrals(nfull, _ptr(irr), lab_map, h_full.reshape(-1), qmo.shape[0], qmo.reshape(-1),
+                                    float(enuc), None, None)
+    val = np.zeros(n)
+    lab = np.zeros((n, 4), dtype=np.int32)
+    lib().orc_fcidump_integrals(nfull, _ptr(irr), lab_map, h_full.reshape(-1), qmo.shape[0], qmo.reshape(-1),
+                                float(enuc), _ptr(val), _ptr(lab))
+    return val, lab
 
 
 class OracleProblem:
@@ -213,6 +248,17 @@ class OracleProblem:
         out = np.zeros(self.n_primal)
         lib().orc_pack_c(self._p, h, g, out)
         return out
+
+    def fcidump_rdms(self, x, act_full, lab_map, ninact):
+        """(values, labels[:, 4]) of the RDM file, in writing order."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        act_full = np.ascontiguousarray(act_full, dtype=np.int32)
+        lab_map = np.ascontiguousarray(lab_map, dtype=np.int32)
+        n = lib().orc_fcidump_rdms(self._p, x, act_full, lab_map, int(ninact), None, None)
+        val = np.zeros(n)
+        lab = np.zeros((n, 4), dtype=np.int32)
+        lib().orc_fcidump_rdms(self._p, x, act_full, lab_map, int(ninact), _ptr(val), _ptr(lab))
+        return val, lab
 
     def pack_c_adjoint(self, xbar):
         n = self.nmo

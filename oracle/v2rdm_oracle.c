@@ -387,6 +387,94 @@ void orc_df_integrals(int nfull, const double *h_full, int nq, const double *qmo
                         df_tei(qmo, nq, ntri, active[p], active[q], active[r], active[s]);
 }
 
+/* ---- FCIDUMP records (fcidump.cc:60-243) ------------------------------------ */
+
+void orc_fcidump_map(int nfull, int nirrep, const int *full_irreps, const int *space, int *map) {
+    int count = 0;
+    for (int sp = 0; sp < 3; sp++)
+        for (int h = 0; h < nirrep; h++)
+            for (int p = 0; p < nfull; p++)
+                if (space[p] == sp && (full_irreps ? full_irreps[p] : 0) == h) map[p] = ++count;
+}
+
+static void put_rec(double *val, int *lab, long k, double v, int a, int b, int c, int d) {
+    if (!val) return;
+    val[k] = v;
+    lab[4 * k] = a;
+    lab[4 * k + 1] = b;
+    lab[4 * k + 2] = c;
+    lab[4 * k + 3] = d;
+}
+
+long orc_fcidump_integrals(int nfull, const int *full_irreps, const int *map, const double *h_full, int nq,
+                           const double *qmo, double enuc, double *val, int *lab) {
+    const long ntri = (long)nfull * (nfull + 1) / 2;
+#define FIRR(p) (full_irreps ? full_irreps[(p)] : 0)
+    long k = 0;
+    for (int p = 0; p < nfull; p++)
+        for (int q = p; q < nfull; q++) {
+            const int hpq = FIRR(p) ^ FIRR(q);
+            const long pq = tri_index(p, q);
+            for (int r = 0; r < nfull; r++)
+                for (int s = r; s < nfull; s++) {
+                    if ((FIRR(r) ^ FIRR(s)) != hpq) continue;
+                    if (pq > tri_index(r, s)) continue;
+                    if (val) put_rec(val, lab, k, df_tei(qmo, nq, ntri, p, q, r, s), map[p], map[q], map[r], map[s]);
+                    k++;
+                }
+        }
+    int nirrep = 1;
+    for (int p = 0; p < nfull; p++)
+        while (FIRR(p) >= nirrep) nirrep *= 2;
+    for (int h = 0; h < nirrep; h++)
+        for (int p = 0; p < nfull; p++) {
+            if (FIRR(p) != h) continue;
+            for (int q = 0; q < nfull; q++) {
+                if (FIRR(q) != h) continue;
+                put_rec(val, lab, k, h_full[(long)p * nfull + q], map[p], map[q], 0, 0);
+                k++;
+            }
+        }
+#undef FIRR
+    put_rec(val, lab, k, enuc, 0, 0, 0, 0);
+    return k + 1;
+}
+
+long orc_fcidump_rdms(const orc_problem *p, const double *x, const int *act_full, const int *map, int ninact,
+                      double *val, int *lab) {
+    long k = 0;
+    for (int h = 0; h < p->nirrep; h++)
+        for (int ij = 0; ij < GAB(h); ij++) {
+            const int i = BAB0(h, ij), j = BAB1(h, ij), ji = IAB(h, j, i);
+            for (int kl = 0; kl < GAB(h); kl++) {
+                const int kk = BAB0(h, kl), l = BAB1(h, kl), lk = IAB(h, l, kk);
+                double dum = x[p->d2aboff[h] + (long)ij * GAB(h) + kl] + x[p->d2aboff[h] + (long)ji * GAB(h) + lk];
+                if (i != j && kk != l) {
+                    const int ija = IAA(h, i, j), kla = IAA(h, kk, l);
+                    int sg = 1;
+                    if (i > j) sg = -sg;
+                    if (kk > l) sg = -sg;
+                    dum += sg * x[p->d2aaoff[h] + (long)ija * GAA(h) + kla];
+                    dum += sg * x[p->d2bboff[h] + (long)ija * GAA(h) + kla];
+                }
+                put_rec(val, lab, k, dum, map[act_full[i]] - ninact, map[act_full[j]] - ninact,
+                        map[act_full[kk]] - ninact, map[act_full[l]] - ninact);
+                k++;
+            }
+        }
+    for (int h = 0; h < p->nirrep; h++)
+        for (int ij = 0; ij < GAB(h); ij++) {
+            const int i = BAB0(h, ij), j = BAB1(h, ij);
+            const int hi = SYM(i), hj = SYM(j);
+            if (hi != hj) continue;
+            const int ii = i - p->pitzer_offset[hi], jj = j - p->pitzer_offset[hj];
+            const double dum = x[p->d1aoff[hi] + (long)ii * p->amopi[hi] + jj] + x[p->d1boff[hi] + (long)ii * p->amopi[hi] + jj];
+            put_rec(val, lab, k, dum, map[act_full[i]] - ninact, map[act_full[j]] - ninact, 0, 0);
+            k++;
+        }
+    return k;
+}
+
 /* ---- A u ---------------------------------------------------------------- */
 /* v2rdm_solver.cc:2995-3050 (dispatch order D2, spin, Q2, G2). */
 void orc_Au(const orc_problem *p, const double *u, double *A) {

@@ -95,6 +95,24 @@ void orc_hubbard_integrals(int n, double t, double U, double *h, double *g);
 void orc_df_integrals(int nfull, const double *h_full, int nq, const double *qmo, int ncore, const int *core,
                       int nact, const int *active, double *h_act, double *g_act, double *efzc);
 
+/* FCIDUMP records (fcidump.cc:60-243), in memory: one record is a value and
+ * four orbital labels, in the order the reference fwrite()s them.
+ * orc_fcidump_map: the label of every orbital, 1-based, ordered by space
+ * (space[p]: 0 core, 1 active, 2 virtual), then irrep, then index
+ * (fcidump.cc:83-107); full_irreps may be NULL (C1).
+ * orc_fcidump_integrals: the (pq|rs) with p <= q, r <= s, INDEX(p,q) <=
+ * INDEX(r,s) of equal pair symmetry from the DF integrals, then h[p][q] over
+ * each irrep block, then (enuc, 0, 0, 0, 0) (fcidump.cc:109-168).
+ * orc_fcidump_rdms: the spin-free 2-RDM over the ab geminals, then the
+ * spin-summed 1-RDM (fcidump.cc:170-243); act_full[i] is the full-space index
+ * of Pitzer active orbital i, labels are map[] - ninact.
+ * Both return the record count; with val == NULL they only count. */
+void orc_fcidump_map(int nfull, int nirrep, const int *full_irreps, const int *space, int *map);
+long orc_fcidump_integrals(int nfull, const int *full_irreps, const int *map, const double *h_full, int nq,
+                           const double *qmo, double enuc, double *val, int *lab);
+long orc_fcidump_rdms(const orc_problem *p, const double *x, const int *act_full, const int *map, int ninact,
+                      double *val, int *lab);
+
 /* Constraint maps (v2rdm_solver.cc:2995-3108; d2.cc, spin.cc, q2.cc, g2.cc). */
 void orc_Au(const orc_problem *p, const double *u, double *out);
 void orc_ATu(const orc_problem *p, const double *u, double *out);
