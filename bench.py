@@ -105,15 +105,17 @@ def executions(kind, oiter, iiter):
     return oiter  # post and the projection classes
 
 
-def roofline(B, kt, counts, workload):
-    """Dominant kernel (largest total time in the timed region) -> roofline
-    dict.  Work per launch is the mean over the kernel's launches of the work
-    of the instances each launch processed (instances leave a launch when
-    their CG, or their solve, has stopped); time per launch is the mean HIP
-    event time of the kernel's launches."""
+def roofline(B, kt, counts, workload, dom=None):
+    """Roofline dict of kernel kind `dom` (default: the dominant kernel, the
+    largest total time in the timed region).  Work per launch is the mean
+    over the kernel's launches of the work of the instances each launch
+    processed (instances leave a launch when their CG, or their solve, has
+    stopped); time per launch is the mean HIP event time of the kernel's
+    launches."""
     n_p, n_d = B.n_primal, B.n_dual
     totals = {k: v[0] for k, v in kt.items() if v[1]}
-    dom = max(totals, key=totals.get)
+    if dom is None:
+        dom = max(totals, key=totals.get)
     ms, cnt = kt[dom]
     per_launch_ms = ms / max(cnt, 1)
     traffic = load_traffic(dom, workload)
@@ -336,7 +338,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--maxiter", type=int, default=20000)
-    ap.add_argument("--single", type=int, default=1, help="also time one instance alone (0 = skip)")
+    ap.add_argument("--single", type=int, default=4,
+                    help="also solve the first K instances one at a time, each alone on the GPU (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=60.0,
                     help="CPU oracle budget: a full solve of instance 0 if it fits, else a sample (0 = skip)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -396,6 +399,8 @@ def main():
     st = [B.status(i) for i in range(B.count)]
     workload = f"c2v16x{args.batch}"
     dom, roof = roofline(B, kt, counts, workload)
+    # the HBM-bound gather kernel of the CG step, whichever kernel dominates
+    roof_cg = roofline(B, kt, counts, workload, "cg_a")[1] if dom != "cg_a" else None
     energy0 = B.status(0)["energy_primal"]
     B.close()
 
@@ -407,13 +412,20 @@ def main():
 
     single = None
     if args.single:
-        S = make_batch(items[:1], args.maxiter)
-        reset_all(S)
-        t0 = time.perf_counter()
-        solve_with_progress(S, "single instance")
-        dt = time.perf_counter() - t0
-        single = {"value": 1.0 / dt, "unit": "SDPs/s", "seconds": dt, "oiter": S.status(0)["oiter"]}
-        S.close()
+        # one instance at a time: the single-SDP drop-in use; the outer
+        # iteration count of one instance varies a lot from instance to
+        # instance, so the rate is taken over several
+        secs, oits = [], []
+        for k in range(min(args.single, len(items))):
+            S = make_batch(items[k:k + 1], args.maxiter)
+            reset_all(S)
+            t0 = time.perf_counter()
+            solve_with_progress(S, f"single instance {k}")
+            secs.append(time.perf_counter() - t0)
+            oits.append(S.status(0)["oiter"])
+            S.close()
+        single = {"value": len(secs) / sum(secs), "unit": "SDPs/s", "instances": len(secs),
+                  "seconds": secs, "oiter": oits}
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -438,6 +450,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
         "roofline": roof,
+        "roofline_cg_a": roof_cg,
         "kernel_ms": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
         "kernel_launches": {k: v[1] for k, v in kt.items() if v[1]},
         "host": host_info(),

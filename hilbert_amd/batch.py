@@ -24,6 +24,7 @@ class Problem:
     g2: bool = True
     spin: bool = True
     sz: bool = True
+    t1: bool = False          # POSITIVITY DQGT1
 
 
 def options(e_convergence=1e-6, r_convergence=1e-5, cg_convergence=1e-9, cg_maxiter=10000, maxiter=50,
@@ -43,7 +44,7 @@ class Batch:
         keep = []
         for k, p in enumerate(problems):
             d, arr = N.make_desc(p.nmo, p.nalpha, p.nbeta, p.irreps, p.nirrep, p.multiplicity,
-                                 p.q2, p.g2, p.spin, p.sz)
+                                 p.q2, p.g2, p.spin, p.sz, getattr(p, "t1", False))
             descs[k] = d
             keep.append(arr)
         self.problems = problems

@@ -3,8 +3,9 @@
 // Row families and their order follow hilbert's bpsdp_Au
 // (v2rdm_solver.cc:2995-3050): D2 (traces, hermiticity, D1/Q1, contractions;
 // d2.cc:314-594), spin (spin.cc:276-508), Q2 (q2.cc:170-317), G2
-// (g2.cc:245-441).  Each row is written here as the linear functional it
-// imposes on the primal vector, with its b value.
+// (g2.cc:245-441), then T1 when asked for (t1.cc:419-793, dispatched at
+// v2rdm_solver.cc:3015-3016).  Each row is written here as the linear
+// functional it imposes on the primal vector, with its b value.
 #include "constraints.hpp"
 
 #include <algorithm>
@@ -338,6 +339,94 @@ Csr build_constraints(const Layout& L) {
         }
     }
 
+    // ---- T1: rows T1aab, T1bba, T1aaa, T1bbb (the order T1_constraints_Au
+    // writes them), each -T1(ijk,lmn) plus the Q2, D2, G2 and D1 elements
+    // t1.cc names under the same Kronecker conditions; b = 0
+    // (v2rdm_solver.cc:2076-2112)
+    if (d.t1) {
+        auto sy = [&](int p) { return L.sym[p]; };
+        for (int s = 0; s < 2; s++)  // T1aab, T1bba
+            for (int h = 0; h < H; h++) {
+                const auto& T = L.aab_trips[h];
+                const int t = (int)T.size();
+                for (int a = 0; a < t; a++) {
+                    const auto [i, j, k] = T[a];
+                    for (int b = 0; b < t; b++) {
+                        const auto [l, m, nn] = T[b];
+                        R.add(gem(s ? T1BBA : T1AAB, h, a, b, t), -1.0);
+                        if (k == nn) {
+                            const int hij = sy(i) ^ sy(j);
+                            R.add(gem(s ? Q2BB : Q2AA, hij, L.aa(hij, i, j), L.aa(hij, l, m), L.gaa[hij]), 1.0);
+                        }
+                        if (j == l) {
+                            const int hki = sy(k) ^ sy(i);
+                            const int nm = s ? L.ab(hki, nn, m) : L.ab(hki, m, nn);
+                            const int ki = s ? L.ab(hki, k, i) : L.ab(hki, i, k);
+                            R.add(gem(D2AB, hki, nm, ki, L.gab[hki]), -1.0);
+                        }
+                        if (l == i) {
+                            const int hkj = sy(k) ^ sy(j);
+                            const int nm = s ? L.ab(hkj, nn, m) : L.ab(hkj, m, nn);
+                            const int kj = s ? L.ab(hkj, k, j) : L.ab(hkj, j, k);
+                            R.add(gem(D2AB, hkj, nm, kj, L.gab[hkj]), 1.0);
+                        }
+                        if (j == m) {
+                            const int hni = sy(nn) ^ sy(i);
+                            R.add(gem(s ? G2AB : G2BA, hni, L.ab(hni, nn, i), L.ab(hni, k, l), L.gab[hni]), -1.0);
+                        }
+                        if (i == m) {
+                            const int hkl = sy(k) ^ sy(l);
+                            R.add(gem(s ? G2AB : G2BA, hkl, L.ab(hkl, nn, j), L.ab(hkl, k, l), L.gab[hkl]), 1.0);
+                        }
+                        R.end(0.0);
+                    }
+                }
+            }
+        for (int s = 0; s < 2; s++)  // T1aaa, T1bbb
+            for (int h = 0; h < H; h++) {
+                const auto& T = L.aaa_trips[h];
+                const int t = (int)T.size();
+                const int q2 = s ? Q2BB : Q2AA, d2 = s ? D2BB : D2AA, d1 = s ? D1B : D1A;
+                // G2aa element (a, b) of the aa (s = 0) or bb (s = 1) quadrant
+                auto g2q = [&](int hh, int a, int b) {
+                    const int g = L.gab[hh];
+                    return gem(G2AA, hh, a + s * g, b + s * g, 2 * g);
+                };
+                auto same = [&](int f, int hh, int p, int q, int r, int u) {  // pair indices, no sign
+                    return gem(f, hh, L.aa(hh, p, q), L.aa(hh, r, u), L.gaa[hh]);
+                };
+                for (int a = 0; a < t; a++) {
+                    const auto [i, j, k] = T[a];
+                    for (int b = 0; b < t; b++) {
+                        const auto [l, m, nn] = T[b];
+                        R.add(gem(s ? T1BBB : T1AAA, h, a, b, t), -1.0);
+                        if (k == nn) R.add(same(q2, sy(i) ^ sy(j), i, j, l, m), 1.0);
+                        if (j == nn && (sy(i) ^ sy(k)) == (sy(l) ^ sy(m))) R.add(same(q2, sy(i) ^ sy(k), i, k, l, m), -1.0);
+                        if (i == nn && (sy(j) ^ sy(k)) == (sy(l) ^ sy(m))) R.add(same(q2, sy(j) ^ sy(k), j, k, l, m), 1.0);
+                        if (l == k && (sy(j) ^ sy(i)) == (sy(nn) ^ sy(m))) R.add(same(d2, sy(j) ^ sy(i), nn, m, j, i), 1.0);
+                        if (j == l) R.add(same(d2, sy(k) ^ sy(i), nn, m, k, i), -1.0);
+                        if (l == i) R.add(same(d2, sy(k) ^ sy(j), nn, m, k, j), 1.0);
+                        if (k == m) {
+                            if (j == l) R.add(L.one_body(d1, nn, i), -1.0);
+                            const int hni = sy(nn) ^ sy(i);
+                            R.add(g2q(hni, L.ab(hni, nn, i), L.ab(hni, j, l)), 1.0);
+                        }
+                        if (j == m) {
+                            if (k == l) R.add(L.one_body(d1, nn, i), 1.0);
+                            const int hni = sy(nn) ^ sy(i);
+                            R.add(g2q(hni, L.ab(hni, nn, i), L.ab(hni, k, l)), -1.0);
+                        }
+                        if (i == m) {
+                            if (k == l) R.add(L.one_body(d1, nn, j), -1.0);
+                            const int hkl = sy(k) ^ sy(l);
+                            R.add(g2q(hkl, L.ab(hkl, nn, j), L.ab(hkl, k, l)), 1.0);
+                        }
+                        R.end(0.0);
+                    }
+                }
+            }
+    }
+
     A.nrows = (int64_t)A.rhs.size();
     return A;
 }
@@ -390,13 +479,17 @@ std::vector<double> hf_guess(const Layout& L, const Csr& A, int nocc_alpha, int 
     std::vector<int8_t> fam(L.n_primal, -1);
     for (const Block& b : L.blocks)
         for (int64_t e = 0; e < (int64_t)b.dim * b.dim; e++) fam[b.off + e] = (int8_t)b.fam;
-    auto derived = [&](int32_t c) { return fam[c] >= Q2AB && fam[c] <= G2AA; };
-    // every Q2 / G2 row holds exactly one Q2 / G2 element; solve the row for it
+    // Q2, G2 and T1 elements are derived: every Q2 / G2 row holds exactly one
+    // Q2 / G2 element, and every T1 row one T1 element besides Q2 / G2
+    // elements that earlier rows have fixed; solve each row for its one
+    // unknown, in row order
+    std::vector<char> known(L.n_primal, 1);
+    for (int64_t c = 0; c < L.n_primal; c++) known[c] = fam[c] < Q2AB;
     for (int64_t r = 0; r < A.nrows; r++) {
         int64_t e = -1;
         int nd = 0;
         for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; k++)
-            if (derived(A.col[k])) {
+            if (!known[A.col[k]]) {
                 e = k;
                 nd++;
             }
@@ -405,6 +498,7 @@ std::vector<double> hf_guess(const Layout& L, const Csr& A, int nocc_alpha, int 
         for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; k++)
             if (k != e) s -= A.val[k] * x[A.col[k]];
         x[A.col[e]] = s / A.val[e];
+        known[A.col[e]] = 1;
     }
     return x;
 }

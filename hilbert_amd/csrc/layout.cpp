@@ -8,6 +8,9 @@
 //                       aa pairs keep first < second in the same order
 //   v2rdm_solver.cc:3729-3846  primal blocks D2ab, D2aa, D2bb, D200, D1a, D1b,
 //                       Q1a, Q1b, Q2ab, Q2aa, Q2bb, G2ab, G2ba, G2aa(2g)
+//   basis.cc:642-744    triplets (T1): loop order i, j, k within an irrep;
+//                       aab keeps i < j, aaa keeps i < j < k
+//   v2rdm_solver.cc:3848-3873  T1aaa, T1bbb, T1aab, T1bba after G2
 #include "layout.hpp"
 
 #include <algorithm>
@@ -20,7 +23,7 @@ namespace v2h {
 std::string ProblemDesc::key() const {
     std::ostringstream k;
     k << nmo << ':' << nalpha << ':' << nbeta << ':' << (multiplicity ? multiplicity : nalpha - nbeta + 1)
-      << ':' << nirrep << ':' << q2 << g2 << spin << sz << ':';
+      << ':' << nirrep << ':' << q2 << g2 << spin << sz << t1 << ':';
     std::vector<int> s = irreps.empty() ? std::vector<int>(nmo, 0) : irreps;
     std::sort(s.begin(), s.end());  // the tables depend only on the irrep counts
     for (int v : s) k << v << ',';
@@ -36,6 +39,7 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
     if (n <= 0) throw std::invalid_argument("nmo must be positive");
     if (H != 1 && H != 2 && H != 4 && H != 8) throw std::invalid_argument("nirrep must be 1, 2, 4 or 8");
     if (na < 0 || nb < 0 || na > n || nb > n) throw std::invalid_argument("electron counts out of range");
+    if (d.t1 && !(d.q2 && d.g2)) throw std::invalid_argument("the T1 condition needs Q2 and G2 (POSITIVITY DQGT1)");
     std::vector<int> irr = d.irreps.empty() ? std::vector<int>(n, 0) : d.irreps;
     if ((int)irr.size() != n) throw std::invalid_argument("orbital_irreps has the wrong length");
     for (int v : irr)
@@ -72,6 +76,27 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
         gaa[h] = (int)aa_pairs[h].size();
     }
 
+    if (d.t1)
+        for (int h = 0; h < H; h++) {
+            aab_index[h].assign((size_t)n * n * n, -1);
+            aaa_index[h].assign((size_t)n * n * n, -1);
+            auto at = [&](std::vector<int>& t, int i, int j, int k) -> int& { return t[((size_t)i * n + j) * n + k]; };
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++)
+                    for (int k = 0; k < n; k++) {
+                        if ((sym[i] ^ sym[j] ^ sym[k]) != h || i >= j) continue;
+                        const int c = (int)aab_trips[h].size();
+                        at(aab_index[h], i, j, k) = at(aab_index[h], j, i, k) = c;
+                        aab_trips[h].push_back({i, j, k});
+                        if (j >= k) continue;
+                        const int e = (int)aaa_trips[h].size();
+                        for (auto [a, b, c2] : {std::array<int, 3>{i, j, k}, {i, k, j}, {j, i, k}, {j, k, i}, {k, i, j},
+                                                {k, j, i}})
+                            at(aaa_index[h], a, b, c2) = e;
+                        aaa_trips[h].push_back({i, j, k});
+                    }
+        }
+
     int64_t o = 0;
     auto push = [&](int f, int h, int dim) {
         off[f][h] = o;
@@ -94,6 +119,12 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
         for (int h = 0; h < H; h++) push(G2AB, h, gab[h]);
         for (int h = 0; h < H; h++) push(G2BA, h, gab[h]);
         for (int h = 0; h < H; h++) push(G2AA, h, 2 * gab[h]);
+    }
+    if (d.t1) {
+        for (int h = 0; h < H; h++) push(T1AAA, h, taaa(h));
+        for (int h = 0; h < H; h++) push(T1BBB, h, taaa(h));
+        for (int h = 0; h < H; h++) push(T1AAB, h, taab(h));
+        for (int h = 0; h < H; h++) push(T1BBA, h, taab(h));
     }
     n_primal = o;
     if (n_primal >= (int64_t)1 << 24)

@@ -59,6 +59,8 @@ typedef struct v2h_problem_desc {
     int constrain_q2, constrain_g2;   /* POSITIVITY (plugin.cc:220, v2rdm_solver.cc:4081-4104) */
     int constrain_spin;               /* CONSTRAIN_SPIN (plugin.cc:306) */
     int constrain_sz;                 /* CONSTRAIN_SZ (plugin.cc:309) */
+    int constrain_t1;                 /* POSITIVITY DQGT1 (v2rdm_solver.cc:4101-4104; needs q2 and g2):
+                                         T1 blocks after G2, rows t1.cc:419-793 */
 } v2h_problem_desc;
 
 typedef struct v2h_status {

@@ -162,9 +162,84 @@ void orc_problem_free(orc_problem *p) {
     for (int h = 0; h < p->nirrep; h++) {
         free(p->ibas_ab[h]); free(p->ibas_aa[h]);
         free(p->bas_ab[h]); free(p->bas_aa[h]);
+        free(p->ibas_aab[h]); free(p->ibas_aaa[h]);
+        free(p->bas_aab[h]); free(p->bas_aaa[h]);
     }
     free(p->symmetry);
     memset(p, 0, sizeof(*p));
+}
+
+/* basis.cc:642-744: triplets (i,j,k) of irrep h in loop order i, j, k; aab
+ * keeps i < j, aaa keeps i < j < k; the index tables are filled for the
+ * swapped first two (aab) and for all six orders (aaa). */
+static void build_triplets(orc_problem *p) {
+    const int n = NMO;
+    const long n3 = (long)n * n * n;
+    for (int h = 0; h < p->nirrep; h++) {
+        p->ibas_aab[h] = (int *)malloc(sizeof(int) * n3);
+        p->ibas_aaa[h] = (int *)malloc(sizeof(int) * n3);
+        p->bas_aab[h] = (int *)malloc(sizeof(int) * 3 * n3);
+        p->bas_aaa[h] = (int *)malloc(sizeof(int) * 3 * n3);
+        for (long e = 0; e < n3; e++) p->ibas_aab[h][e] = p->ibas_aaa[h][e] = -999;
+        int caab = 0, caaa = 0;
+#define I3(a, b, c) (((long)(a) * n + (b)) * n + (c))
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++)
+                for (int k = 0; k < n; k++) {
+                    if ((SYM(i) ^ SYM(j) ^ SYM(k)) != h) continue;
+                    if (i >= j) continue;
+                    p->ibas_aab[h][I3(i, j, k)] = p->ibas_aab[h][I3(j, i, k)] = caab;
+                    p->bas_aab[h][3 * caab] = i;
+                    p->bas_aab[h][3 * caab + 1] = j;
+                    p->bas_aab[h][3 * caab + 2] = k;
+                    caab++;
+                    if (j >= k) continue;
+                    p->ibas_aaa[h][I3(i, j, k)] = p->ibas_aaa[h][I3(i, k, j)] = p->ibas_aaa[h][I3(j, i, k)] = caaa;
+                    p->ibas_aaa[h][I3(j, k, i)] = p->ibas_aaa[h][I3(k, i, j)] = p->ibas_aaa[h][I3(k, j, i)] = caaa;
+                    p->bas_aaa[h][3 * caaa] = i;
+                    p->bas_aaa[h][3 * caaa + 1] = j;
+                    p->bas_aaa[h][3 * caaa + 2] = k;
+                    caaa++;
+                }
+#undef I3
+        p->trip_aab[h] = caab;
+        p->trip_aaa[h] = caaa;
+    }
+}
+
+int orc_problem_enable_t1(orc_problem *p) {
+    if (!p->constrain_q2 || !p->constrain_g2 || p->constrain_t1) return -1;
+    p->constrain_t1 = 1;
+    build_triplets(p);
+    const int H = p->nirrep;
+    long off = p->n_primal;
+    int nb = p->nblocks;
+    /* v2rdm_solver.cc:3848-3873 */
+    for (int h = 0; h < H; h++) { p->t1aaaoff[h] = off; off += (long)p->trip_aaa[h] * p->trip_aaa[h]; p->dims[nb++] = p->trip_aaa[h]; }
+    for (int h = 0; h < H; h++) { p->t1bbboff[h] = off; off += (long)p->trip_aaa[h] * p->trip_aaa[h]; p->dims[nb++] = p->trip_aaa[h]; }
+    for (int h = 0; h < H; h++) { p->t1aaboff[h] = off; off += (long)p->trip_aab[h] * p->trip_aab[h]; p->dims[nb++] = p->trip_aab[h]; }
+    for (int h = 0; h < H; h++) { p->t1bbaoff[h] = off; off += (long)p->trip_aab[h] * p->trip_aab[h]; p->dims[nb++] = p->trip_aab[h]; }
+    p->n_primal = off;
+    p->nblocks = nb;
+    /* v2rdm_solver.cc:3554-3567 */
+    for (int h = 0; h < H; h++) p->n_dual += 2L * p->trip_aaa[h] * p->trip_aaa[h] + 2L * p->trip_aab[h] * p->trip_aab[h];
+    return 0;
+}
+
+void orc_get_t1_offsets(const orc_problem *p, long *off) {
+    for (int h = 0; h < p->nirrep; h++) {
+        off[h] = p->t1aaaoff[h];
+        off[p->nirrep + h] = p->t1bbboff[h];
+        off[2 * p->nirrep + h] = p->t1aaboff[h];
+        off[3 * p->nirrep + h] = p->t1bbaoff[h];
+    }
+}
+
+void orc_get_trip_counts(const orc_problem *p, int *taab, int *taaa) {
+    for (int h = 0; h < p->nirrep; h++) {
+        taab[h] = p->trip_aab[h];
+        taaa[h] = p->trip_aaa[h];
+    }
 }
 
 long orc_n_primal(const orc_problem *p) { return p->n_primal; }
@@ -475,8 +550,136 @@ long orc_fcidump_rdms(const orc_problem *p, const double *x, const int *act_full
     return k;
 }
 
+/* ---- T1 rows (t1.cc:419-793) ------------------------------------------------- */
+/* Row order T1aab, T1bba, T1aaa, T1bbb (the order T1_constraints_Au writes
+ * them with its running offset); every row is - T1(ijk,lmn) plus the Q2, D2,
+ * G2 and D1 elements t1.cc names, with the same Kronecker conditions.  The
+ * terms are handed to `term` in the reference's order, so that Au sums them
+ * in that order and A^T u scatters the same terms. */
+typedef void (*t1_term_fn)(long row, long col, double coef, void *ctx);
+
+static long t1_rows(const orc_problem *p, long r, t1_term_fn term, void *ctx) {
+    const int H = p->nirrep;
+#define T3(arr, h, a, c) (p->arr[(h)][3 * (a) + (c)])
+#define D1(off, a, b) ((off)[SYM(a)] + (long)((a) - p->pitzer_offset[SYM(a)]) * p->amopi[SYM(a)] + ((b) - p->pitzer_offset[SYM(a)]))
+    for (int s = 0; s < 2; s++) /* 0: T1aab, 1: T1bba */
+        for (int h = 0; h < H; h++) {
+            const int t = p->trip_aab[h];
+            const long toff = s == 0 ? p->t1aaboff[h] : p->t1bbaoff[h];
+            for (int ijk = 0; ijk < t; ijk++) {
+                const int i = T3(bas_aab, h, ijk, 0), j = T3(bas_aab, h, ijk, 1), k = T3(bas_aab, h, ijk, 2);
+                for (int lmn = 0; lmn < t; lmn++) {
+                    const int l = T3(bas_aab, h, lmn, 0), m = T3(bas_aab, h, lmn, 1), n = T3(bas_aab, h, lmn, 2);
+                    const long row = r + (long)ijk * t + lmn;
+                    term(row, toff + (long)ijk * t + lmn, -1.0, ctx);
+                    if (k == n) {
+                        const int hij = SYM(i) ^ SYM(j);
+                        const long q2 = s == 0 ? p->q2aaoff[hij] : p->q2bboff[hij];
+                        term(row, q2 + (long)IAA(hij, i, j) * GAA(hij) + IAA(hij, l, m), 1.0, ctx);
+                    }
+                    if (j == l) {
+                        const int hki = SYM(k) ^ SYM(i);
+                        const long nm = s == 0 ? IAB(hki, m, n) : IAB(hki, n, m);
+                        const long ki = s == 0 ? IAB(hki, i, k) : IAB(hki, k, i);
+                        term(row, p->d2aboff[hki] + nm * GAB(hki) + ki, -1.0, ctx);
+                    }
+                    if (l == i) {
+                        const int hkj = SYM(k) ^ SYM(j);
+                        const long nm = s == 0 ? IAB(hkj, m, n) : IAB(hkj, n, m);
+                        const long kj = s == 0 ? IAB(hkj, j, k) : IAB(hkj, k, j);
+                        term(row, p->d2aboff[hkj] + nm * GAB(hkj) + kj, 1.0, ctx);
+                    }
+                    if (j == m) {
+                        const int hni = SYM(n) ^ SYM(i);
+                        const long g2 = s == 0 ? p->g2baoff[hni] : p->g2aboff[hni];
+                        term(row, g2 + (long)IAB(hni, n, i) * GAB(hni) + IAB(hni, k, l), -1.0, ctx);
+                    }
+                    if (i == m) {
+                        const int hkl = SYM(k) ^ SYM(l);
+                        const long g2 = s == 0 ? p->g2baoff[hkl] : p->g2aboff[hkl];
+                        term(row, g2 + (long)IAB(hkl, n, j) * GAB(hkl) + IAB(hkl, k, l), 1.0, ctx);
+                    }
+                }
+            }
+            r += (long)t * t;
+        }
+    for (int s = 0; s < 2; s++) /* 0: T1aaa, 1: T1bbb */
+        for (int h = 0; h < H; h++) {
+            const int t = p->trip_aaa[h];
+            const long toff = s == 0 ? p->t1aaaoff[h] : p->t1bbboff[h];
+            const long *q2o = s == 0 ? p->q2aaoff : p->q2bboff;
+            const long *d2o = s == 0 ? p->d2aaoff : p->d2bboff;
+            const long *d1o = s == 0 ? p->d1aoff : p->d1boff;
+            for (int ijk = 0; ijk < t; ijk++) {
+                const int i = T3(bas_aaa, h, ijk, 0), j = T3(bas_aaa, h, ijk, 1), k = T3(bas_aaa, h, ijk, 2);
+                for (int lmn = 0; lmn < t; lmn++) {
+                    const int l = T3(bas_aaa, h, lmn, 0), m = T3(bas_aaa, h, lmn, 1), n = T3(bas_aaa, h, lmn, 2);
+                    const long row = r + (long)ijk * t + lmn;
+                    /* G2aa element (a, b) of the aa quadrant (s = 0) or bb quadrant (s = 1) */
+#define G2Q(hh, a, b) (p->g2aaoff[(hh)] + (long)((a) + s * GAB(hh)) * 2 * GAB(hh) + ((b) + s * GAB(hh)))
+                    term(row, toff + (long)ijk * t + lmn, -1.0, ctx);
+                    if (k == n) {
+                        const int hij = SYM(i) ^ SYM(j);
+                        term(row, q2o[hij] + (long)IAA(hij, i, j) * GAA(hij) + IAA(hij, l, m), 1.0, ctx);
+                    }
+                    if (j == n) {
+                        const int hik = SYM(i) ^ SYM(k), hlm = SYM(l) ^ SYM(m);
+                        if (hik == hlm) term(row, q2o[hik] + (long)IAA(hik, i, k) * GAA(hik) + IAA(hik, l, m), -1.0, ctx);
+                    }
+                    if (i == n) {
+                        const int hjk = SYM(j) ^ SYM(k), hlm = SYM(l) ^ SYM(m);
+                        if (hjk == hlm) term(row, q2o[hjk] + (long)IAA(hjk, j, k) * GAA(hjk) + IAA(hjk, l, m), 1.0, ctx);
+                    }
+                    if (l == k) {
+                        const int hnm = SYM(n) ^ SYM(m), hji = SYM(j) ^ SYM(i);
+                        if (hji == hnm) term(row, d2o[hji] + (long)IAA(hji, n, m) * GAA(hji) + IAA(hji, j, i), 1.0, ctx);
+                    }
+                    if (j == l) {
+                        const int hki = SYM(k) ^ SYM(i);
+                        term(row, d2o[hki] + (long)IAA(hki, n, m) * GAA(hki) + IAA(hki, k, i), -1.0, ctx);
+                    }
+                    if (l == i) {
+                        const int hkj = SYM(k) ^ SYM(j);
+                        term(row, d2o[hkj] + (long)IAA(hkj, n, m) * GAA(hkj) + IAA(hkj, k, j), 1.0, ctx);
+                    }
+                    if (k == m) {
+                        if (j == l) term(row, D1(d1o, n, i), -1.0, ctx);
+                        const int hni = SYM(n) ^ SYM(i);
+                        term(row, G2Q(hni, IAB(hni, n, i), IAB(hni, j, l)), 1.0, ctx);
+                    }
+                    if (j == m) {
+                        if (k == l) term(row, D1(d1o, n, i), 1.0, ctx);
+                        const int hni = SYM(n) ^ SYM(i);
+                        term(row, G2Q(hni, IAB(hni, n, i), IAB(hni, k, l)), -1.0, ctx);
+                    }
+                    if (i == m) {
+                        if (k == l) term(row, D1(d1o, n, j), -1.0, ctx);
+                        const int hkl = SYM(k) ^ SYM(l);
+                        term(row, G2Q(hkl, IAB(hkl, n, j), IAB(hkl, k, l)), 1.0, ctx);
+                    }
+#undef G2Q
+                }
+            }
+            r += (long)t * t;
+        }
+#undef T3
+#undef D1
+    return r;
+}
+
+struct t1_ctx { const double *in; double *out; };
+static void t1_gather(long row, long col, double coef, void *c) {
+    struct t1_ctx *x = (struct t1_ctx *)c;
+    x->out[row] += coef * x->in[col];
+}
+static void t1_scatter(long row, long col, double coef, void *c) {
+    struct t1_ctx *x = (struct t1_ctx *)c;
+    x->out[col] += coef * x->in[row];
+}
+
 /* ---- A u ---------------------------------------------------------------- */
-/* v2rdm_solver.cc:2995-3050 (dispatch order D2, spin, Q2, G2). */
+/* v2rdm_solver.cc:2995-3050 (dispatch order D2, spin, Q2, G2, then T1 at
+ * :3015-3016). */
 void orc_Au(const orc_problem *p, const double *u, double *A) {
     const int n = NMO, H = p->nirrep;
     const double na = p->nalpha, nb = p->nbeta;
@@ -818,6 +1021,10 @@ void orc_Au(const orc_problem *p, const double *u, double *A) {
             }
             r += (long)G * G;
         }
+    }
+    if (p->constrain_t1) {
+        struct t1_ctx c = {u, A};
+        r = t1_rows(p, r, t1_gather, &c);
     }
 }
 
@@ -1170,6 +1377,10 @@ void orc_ATu(const orc_problem *p, const double *u, double *A) {
             }
             r += (long)G * G;
         }
+    }
+    if (p->constrain_t1) {
+        struct t1_ctx c = {u, A};
+        r = t1_rows(p, r, t1_scatter, &c);
     }
 }
 

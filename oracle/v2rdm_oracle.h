@@ -51,7 +51,15 @@ typedef struct orc_problem {
     long g2aboff[ORC_MAX_IRREP], g2baoff[ORC_MAX_IRREP], g2aaoff[ORC_MAX_IRREP];
     long n_primal, n_dual;
     int nblocks;
-    int dims[14 * ORC_MAX_IRREP];
+    int dims[18 * ORC_MAX_IRREP];
+
+    /* T1 (POSITIVITY DQGT1), set by orc_problem_enable_t1: triplet tables
+     * (basis.cc:642-744) and the T1 blocks after G2 (v2rdm_solver.cc:3848-3873) */
+    int constrain_t1;
+    int trip_aab[ORC_MAX_IRREP], trip_aaa[ORC_MAX_IRREP];
+    int *bas_aab[ORC_MAX_IRREP], *bas_aaa[ORC_MAX_IRREP];    /* [trip*3] */
+    int *ibas_aab[ORC_MAX_IRREP], *ibas_aaa[ORC_MAX_IRREP];  /* [nmo^3], -999 when absent */
+    long t1aaaoff[ORC_MAX_IRREP], t1bbboff[ORC_MAX_IRREP], t1aaboff[ORC_MAX_IRREP], t1bbaoff[ORC_MAX_IRREP];
 } orc_problem;
 
 /* Build tables/offsets.  irreps: [nmo] orbital irreps in Pitzer (non-decreasing)
@@ -60,6 +68,13 @@ int  orc_problem_init(orc_problem *p, int nmo, int nalpha, int nbeta, int multip
                       const int *irreps, int nirrep,
                       int constrain_q2, int constrain_g2, int constrain_spin, int constrain_sz);
 void orc_problem_free(orc_problem *p);
+/* Add the T1 condition (t1.cc) to an initialised problem with Q2 and G2:
+ * triplet tables, the T1aaa, T1bbb, T1aab, T1bba blocks and their rows
+ * (b = 0, v2rdm_solver.cc:2076-2112).  Returns 0, or -1 without Q2 and G2. */
+int  orc_problem_enable_t1(orc_problem *p);
+/* [4][nirrep] offsets of the T1aaa, T1bbb, T1aab, T1bba blocks */
+void orc_get_t1_offsets(const orc_problem *p, long *off);
+void orc_get_trip_counts(const orc_problem *p, int *trip_aab, int *trip_aaa);
 
 /* Accessors for ctypes. */
 long orc_n_primal(const orc_problem *p);

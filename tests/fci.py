@@ -216,3 +216,33 @@ def pack_primal(P, R):
                         x[off["d200"][h] + (a + g) * G2 + b_] = 0.5 / dkl * (d2(a, b_) + d2(a, lk) - d2(ji, b_) - d2(ji, lk))
                         x[off["d200"][h] + (a + g) * G2 + b_ + g] = 0.5 * (d2(a, b_) - d2(ji, b_) - d2(a, lk) + d2(ji, lk))
     return x
+
+
+def pack_t1(P, F, psi, x):
+    """Fill the T1 blocks of x (oracle layout, P built with t1=True) with the
+    exact T1(ijk,lmn) = <a+_i a+_j a+_k a_n a_m a_l> + <a_i a_j a_k a+_n a+_m a+_l>
+    of psi; spins aaa, bbb, aab (i, j alpha, k beta), bba.  Triplets of irrep
+    h in loop order i, j, k (aab: i < j; aaa: i < j < k), Pitzer orbitals."""
+    import itertools as it
+    n, M = F.n, F.M
+    irr = np.asarray(P.irreps)
+    ann = [F._ann[k] for k in range(M)]
+    cre = [F._ann[k].T.tocsr() for k in range(M)]
+    A3, C3 = {}, {}
+    for a, b, c in it.product(range(M), repeat=3):
+        A3[a, b, c] = ann[a] @ (ann[b] @ (ann[c] @ psi))
+        C3[a, b, c] = cre[a] @ (cre[b] @ (cre[c] @ psi))
+    off = P.t1_offsets()
+    for h in range(P.nirrep):
+        trips = [(i, j, k) for i in range(n) for j in range(n) for k in range(n) if irr[i] ^ irr[j] ^ irr[k] == h]
+        aab = [t for t in trips if t[0] < t[1]]
+        aaa = [t for t in aab if t[1] < t[2]]
+        for name, tl, sp in (("t1aaa", aaa, (0, 0, 0)), ("t1bbb", aaa, (1, 1, 1)),
+                             ("t1aab", aab, (0, 0, 1)), ("t1bba", aab, (1, 1, 0))):
+            t = len(tl)
+            for a, (i, j, k) in enumerate(tl):
+                I, J, K = i + sp[0] * n, j + sp[1] * n, k + sp[2] * n
+                for b, (l, m, q) in enumerate(tl):
+                    L_, M_, N_ = l + sp[0] * n, m + sp[1] * n, q + sp[2] * n
+                    x[off[name][h] + a * t + b] = A3[K, J, I] @ A3[N_, M_, L_] + C3[K, J, I] @ C3[N_, M_, L_]
+    return x

@@ -1,0 +1,75 @@
+"""GPU parity of the T1 condition (POSITIVITY DQGT1) through the C ABI.
+
+* maps: the device's A u and A^T y with the T1 blocks and rows equal the
+  oracle's (t1.cc restated, pinned in test_oracle_t1.py) to 1e-13, for C1,
+  2 and 4 irreps, singlet and doublet;
+* a DQGT1 solve matches the oracle's DQGT1 solve within 5e-8 Eh at
+  1e-9 / 1e-8, and lies above the DQG energy of the same Hamiltonian;
+* the HF guess with T1 equals the product's host guess (which the CPU tests
+  pin to the exact T1 of the determinant).
+"""
+import numpy as np
+import pytest
+
+from hilbert_amd.batch import Batch, Problem, options
+from hilbert_amd.integrals import synthetic_integrals
+from hilbert_amd.structure import Structure
+from oracle.pyoracle import OracleProblem
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    (4, 2, 2, None, 1),
+    (5, 3, 2, None, 1),
+    (5, 2, 2, [0, 0, 1, 1, 1], 2),
+    (6, 3, 3, [0, 0, 1, 2, 3, 3], 4),
+]
+
+
+def _oracle(n, na, nb, irr, H):
+    return OracleProblem(n, na, nb, irr if irr is not None else [0] * n, H, t1=True)
+
+
+def test_t1_maps_match_oracle():
+    B = Batch([Problem(n, na, nb, irr, H, t1=True) for n, na, nb, irr, H in CASES], options())
+    rng = np.random.default_rng(8)
+    for i, case in enumerate(CASES):
+        P = _oracle(*case)
+        assert (B.n_primal[i], B.n_dual[i]) == (P.n_primal, P.n_dual)
+        assert B.block_dims(i).tolist() == P.dims.tolist()
+        u = rng.standard_normal(P.n_primal)
+        ref = P.Au(u)
+        assert np.abs(B.Au(i, u) - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), case
+        y = rng.standard_normal(P.n_dual)
+        ref = P.ATu(y)
+        assert np.abs(B.ATu(i, y) - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), case
+    B.close()
+
+
+def test_t1_solve_matches_oracle():
+    n, na, nb = 4, 2, 2
+    h, g = synthetic_integrals(n, 8, 5)
+    tight = dict(e_convergence=1e-9, r_convergence=1e-8)
+    P = OracleProblem(n, na, nb, t1=True)
+    ref = P.solve(P.pack_c(h, g), maxiter=50000, nthreads=4, **tight)
+    assert ref["converged"]
+    B = Batch([Problem(n, na, nb, t1=True), Problem(n, na, nb)], options(maxiter=50000, **tight))
+    for i in range(2):
+        B.set_integrals(i, h, g)
+    assert B.solve_to_convergence() == 2
+    st, st0 = B.status(0), B.status(1)
+    assert st["converged"] and st["status_bits"] == 0
+    print("DQGT1", st["energy_primal"], "oracle", ref["energy_primal"], "DQG", st0["energy_primal"], flush=True)
+    assert abs(st["energy_primal"] - ref["energy_primal"]) < 5e-8
+    assert st["energy_primal"] > st0["energy_primal"] + 1e-3
+    B.close()
+
+
+def test_t1_hf_guess_on_the_device_equals_host_guess():
+    n, na, irr, H = 6, 3, [0, 0, 1, 1, 2, 3], 4
+    B = Batch([Problem(n, na, na, irr, H, t1=True)], options())
+    B.set_guess_hf(0)
+    x = B.x(0)
+    B.close()
+    S = Structure(n, na, na, irr, H, t1=True)
+    assert np.array_equal(x, S.hf_guess())

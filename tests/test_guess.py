@@ -5,7 +5,7 @@ quantisation), with the D200 blocks left zero as in the reference."""
 import numpy as np
 import pytest
 
-from fci import Fock, RDMs, pack_primal
+from fci import Fock, RDMs, pack_primal, pack_t1
 from hilbert_amd.structure import Structure
 from oracle.pyoracle import OracleProblem
 
@@ -52,3 +52,26 @@ def test_hf_guess_open_shell_follows_reference_occupations():
     d1b = x[off["d1b"][0]:off["d1b"][0] + n * n].reshape(n, n)
     d1a = x[off["d1a"][0]:off["d1a"][0] + n * n].reshape(n, n)
     assert np.trace(d1a) == na and np.trace(d1b) == na
+
+
+@pytest.mark.parametrize("n,na,irr,H", CASES[:3])
+def test_hf_guess_with_t1_is_the_determinant(n, na, irr, H):
+    """POSITIVITY DQGT1: the T1 blocks of the guess (each T1 row solved for
+    its T1 element once the Q2 and G2 rows have fixed theirs) are the exact
+    T1 of the determinant."""
+    irreps = irr if irr is not None else [0] * n
+    P = OracleProblem(n, na, na, irreps, H, t1=True)
+    P0 = OracleProblem(n, na, na, irreps, H)
+    S = Structure(n, na, na, irr, H, t1=True)
+    F = Fock(n)
+    psi = _determinant(F, n, na, na)
+    x_ref = np.zeros(P.n_primal)
+    x_ref[:P0.n_primal] = pack_primal(P0, RDMs(F, psi))
+    pack_t1(P, F, psi, x_ref)
+    off = P.offsets()
+    dims = S.block_dims()
+    for h in range(H):
+        d = dims[3 * H + h]
+        x_ref[off["d200"][h]:off["d200"][h] + d * d] = 0.0
+    x = S.hf_guess()
+    assert np.abs(x - x_ref).max() < 1e-13

@@ -59,6 +59,35 @@ def test_constraints_match_oracle(case, flags):
         assert np.abs(A.T @ y - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
 
 
+T1_CASES = CASES[:5]
+
+
+@pytest.mark.parametrize("spin_sz", [(True, True), (False, True), (True, False)])
+@pytest.mark.parametrize("case", T1_CASES)
+def test_t1_constraints_match_oracle(case, spin_sz):
+    """POSITIVITY DQGT1: T1 blocks after G2 and the T1 rows of t1.cc."""
+    n, na, nb, irr, H = case
+    spin, sz = spin_sz
+    S = Structure(n, na, nb, irr, H, spin=spin, sz=sz, t1=True)
+    P = OracleProblem(n, na, nb, irr, H, spin=spin, sz=sz, t1=True)
+    assert (S.n_primal, S.n_dual) == (P.n_primal, P.n_dual)
+    assert S.block_dims().tolist() == P.dims.tolist()
+    A, b = S.matrix()
+    assert np.array_equal(b, P.b())
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(P.n_primal)
+    ref = P.Au(u)
+    assert np.abs(A @ u - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    y = rng.standard_normal(P.n_dual)
+    ref = P.ATu(y)
+    assert np.abs(A.T @ y - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_t1_needs_q2_and_g2():
+    with pytest.raises(Exception, match="Q2 and G2"):
+        Structure(4, 2, 2, q2=False, t1=True)
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_tables_match_oracle(case):
     n, na, nb, irr, H = case
