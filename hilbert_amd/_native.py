@@ -30,7 +30,8 @@ class ProblemDesc(C.Structure):
     _fields_ = [("nmo", C.c_int), ("nalpha", C.c_int), ("nbeta", C.c_int), ("multiplicity", C.c_int),
                 ("nirrep", C.c_int), ("orbital_irreps", C.POINTER(C.c_int)),
                 ("constrain_q2", C.c_int), ("constrain_g2", C.c_int), ("constrain_spin", C.c_int),
-                ("constrain_sz", C.c_int), ("constrain_t1", C.c_int)]
+                ("constrain_sz", C.c_int), ("constrain_t1", C.c_int),
+                ("constrain_t2", C.c_int)]
 
 
 class Status(C.Structure):
@@ -148,7 +149,7 @@ def check(rc: int):
 
 
 def make_desc(nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=0, q2=True, g2=True, spin=True,
-              sz=True, t1=False):
+              sz=True, t1=False, t2=False):
     """ProblemDesc plus the array that keeps its irrep pointer alive."""
     arr = None
     ptr = None
@@ -160,7 +161,7 @@ def make_desc(nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=0, q2=T
             while nirrep <= max(int(v) for v in irreps):
                 nirrep *= 2
     d = ProblemDesc(int(nmo), int(nalpha), int(nbeta), int(multiplicity), int(nirrep or 1), ptr,
-                    int(q2), int(g2), int(spin), int(sz), int(t1))
+                    int(q2), int(g2), int(spin), int(sz), int(t1), int(t2))
     return d, arr
 
 

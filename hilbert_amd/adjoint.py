@@ -26,13 +26,13 @@ from .v2rdm import _POSITIVITY, resolve_options
 
 
 def _problem(nmo, nalpha, nbeta, o):
-    q2, g2, t1 = _POSITIVITY[o["POSITIVITY"]]
+    q2, g2, t1, t2 = _POSITIVITY[o["POSITIVITY"]]
     irreps = o["IRREPS"]
     nirrep = o["NIRREP"]
     if irreps is not None and nirrep is None:
         nirrep = 1 << int(np.ceil(np.log2(max(int(np.max(irreps)) + 1, 1))))
     return Problem(nmo, nalpha, nbeta, None if irreps is None else list(irreps), nirrep, 0, q2, g2,
-                   bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1)
+                   bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1, t2)
 
 
 class _Energies(torch.autograd.Function):

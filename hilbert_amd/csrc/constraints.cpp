@@ -3,8 +3,8 @@
 // Row families and their order follow hilbert's bpsdp_Au
 // (v2rdm_solver.cc:2995-3050): D2 (traces, hermiticity, D1/Q1, contractions;
 // d2.cc:314-594), spin (spin.cc:276-508), Q2 (q2.cc:170-317), G2
-// (g2.cc:245-441), then T1 when asked for (t1.cc:419-793, dispatched at
-// v2rdm_solver.cc:3015-3016).  Each row is written here as the linear
+// (g2.cc:245-441), then T1 and T2 when asked for (t1.cc:419-793 and the slow
+// path t2.cc:1236-1843, dispatched at v2rdm_solver.cc:3015-3021).  Each row is written here as the linear
 // functional it imposes on the primal vector, with its b value.
 #include "constraints.hpp"
 
@@ -420,6 +420,119 @@ Csr build_constraints(const Layout& L) {
                             if (k == l) R.add(L.one_body(d1, nn, j), -1.0);
                             const int hkl = sy(k) ^ sy(l);
                             R.add(g2q(hkl, L.ab(hkl, nn, j), L.ab(hkl, k, l)), 1.0);
+                        }
+                        R.end(0.0);
+                    }
+                }
+            }
+    }
+
+    // ---- T2: rows T2aab, T2bba, T2aaa, T2bbb (T2_constraints_Au_slow's
+    // order); T2aaa / T2bbb are one block each over the aab triplets then all
+    // triplets.  Each row is -T2(ijk,lmn) plus the D2 and D1 elements t2.cc
+    // names; b = 0 (v2rdm_solver.cc:2113-2150)
+    if (d.t2) {
+        auto sy = [&](int p) { return L.sym[p]; };
+        auto dab = [&](int hh, int a, int b) { return gem(D2AB, hh, a, b, L.gab[hh]); };
+        auto dss = [&](int f, int hh, int a, int b) { return gem(f, hh, a, b, L.gaa[hh]); };
+        // signed same-spin D2 element of pairs (x, y), (u, v): skipped when a
+        // pair repeats an orbital, sign flipped for x > y and for u > v
+        auto signed_d2 = [&](int f, int x, int y, int u, int v, double c0) {
+            if (x == y || u == v) return;
+            const int hh = sy(x) ^ sy(y);
+            double c = c0;
+            if (x > y) c = -c;
+            if (u > v) c = -c;
+            R.add(dss(f, hh, L.aa(hh, x, y), L.aa(hh, u, v)), c);
+        };
+        for (int s = 0; s < 2; s++)  // T2aab, T2bba
+            for (int h = 0; h < H; h++) {
+                const auto& T = L.aab_trips[h];
+                const int t = (int)T.size();
+                const int d2s = s ? D2BB : D2AA, d1o = s ? D1A : D1B;
+                for (int a = 0; a < t; a++) {
+                    const auto [i, j, k] = T[a];
+                    for (int b = 0; b < t; b++) {
+                        const auto [l, m, nn] = T[b];
+                        R.add(gem(s ? T2BBA : T2AAB, h, a, b, t), -1.0);
+                        if (k == nn) {
+                            const int hij = sy(i) ^ sy(j);
+                            R.add(dss(d2s, hij, L.aa(hij, i, j), L.aa(hij, l, m)), 1.0);
+                        }
+                        if (j == m && i == l) R.add(L.one_body(d1o, nn, k), 1.0);
+                        if (j == l && i == m) R.add(L.one_body(d1o, nn, k), -1.0);
+                        if (i == l) {
+                            const int hh = sy(nn) ^ sy(j);
+                            R.add(s ? dab(hh, L.ab(hh, nn, j), L.ab(hh, k, m)) : dab(hh, L.ab(hh, j, nn), L.ab(hh, m, k)), -1.0);
+                        }
+                        if (j == l) {
+                            const int hh = sy(nn) ^ sy(i);
+                            R.add(s ? dab(hh, L.ab(hh, nn, i), L.ab(hh, k, m)) : dab(hh, L.ab(hh, i, nn), L.ab(hh, m, k)), 1.0);
+                        }
+                        if (i == m) {
+                            const int hh = sy(nn) ^ sy(j);
+                            R.add(s ? dab(hh, L.ab(hh, nn, j), L.ab(hh, k, l)) : dab(hh, L.ab(hh, j, nn), L.ab(hh, l, k)), 1.0);
+                        }
+                        if (j == m) {
+                            const int hh = sy(nn) ^ sy(i);
+                            R.add(s ? dab(hh, L.ab(hh, nn, i), L.ab(hh, k, l)) : dab(hh, L.ab(hh, i, nn), L.ab(hh, l, k)), -1.0);
+                        }
+                        R.end(0.0);
+                    }
+                }
+            }
+        for (int s = 0; s < 2; s++)  // T2aaa (aaa + abb parts), T2bbb (bbb + baa parts)
+            for (int h = 0; h < H; h++) {
+                const auto& A3 = L.aab_trips[h];
+                const auto& B3 = L.aba_trips[h];
+                const int ta = (int)A3.size(), tb = (int)B3.size(), D = ta + tb;
+                const int f = s ? T2BBB : T2AAA;
+                const int d2s = s ? D2BB : D2AA, d2o = s ? D2AA : D2BB, d1s = s ? D1B : D1A, d1o = s ? D1A : D1B;
+                for (int a = 0; a < D; a++) {
+                    const auto [i, j, k] = a < ta ? A3[a] : B3[a - ta];
+                    for (int b = 0; b < D; b++) {
+                        const auto [l, m, nn] = b < ta ? A3[b] : B3[b - ta];
+                        R.add(gem(f, h, a, b, D), -1.0);
+                        if (a < ta && b < ta) {         // aab x aab
+                            if (k == nn) {
+                                const int hij = sy(i) ^ sy(j);
+                                R.add(dss(d2s, hij, L.aa(hij, i, j), L.aa(hij, l, m)), 1.0);
+                            }
+                            if (j == m && i == l) R.add(L.one_body(d1s, nn, k), 1.0);
+                            if (j == l && i == m) R.add(L.one_body(d1s, nn, k), -1.0);
+                            if (i == l) signed_d2(d2s, nn, j, k, m, -1.0);
+                            if (j == l) signed_d2(d2s, nn, i, k, m, 1.0);
+                            if (i == m) signed_d2(d2s, nn, j, k, l, 1.0);
+                            if (j == m) signed_d2(d2s, nn, i, k, l, -1.0);
+                        } else if (a < ta) {            // aab x aba
+                            if (i == l) {
+                                const int hh = sy(j) ^ sy(nn);
+                                R.add(s ? dab(hh, L.ab(hh, nn, j), L.ab(hh, m, k)) : dab(hh, L.ab(hh, j, nn), L.ab(hh, k, m)), 1.0);
+                            }
+                            if (j == l) {
+                                const int hh = sy(i) ^ sy(nn);
+                                R.add(s ? dab(hh, L.ab(hh, nn, i), L.ab(hh, m, k)) : dab(hh, L.ab(hh, i, nn), L.ab(hh, k, m)), -1.0);
+                            }
+                        } else if (b < ta) {            // aba x aab
+                            if (i == l) {
+                                const int hh = sy(j) ^ sy(nn);
+                                R.add(s ? dab(hh, L.ab(hh, j, nn), L.ab(hh, k, m)) : dab(hh, L.ab(hh, nn, j), L.ab(hh, m, k)), 1.0);
+                            }
+                            if (i == m) {
+                                const int hh = sy(j) ^ sy(nn);
+                                R.add(s ? dab(hh, L.ab(hh, j, nn), L.ab(hh, k, l)) : dab(hh, L.ab(hh, nn, j), L.ab(hh, l, k)), -1.0);
+                            }
+                        } else {                        // aba x aba
+                            if (k == nn) {
+                                const int hij = sy(i) ^ sy(j);
+                                R.add(s ? dab(hij, L.ab(hij, j, i), L.ab(hij, m, l)) : dab(hij, L.ab(hij, i, j), L.ab(hij, l, m)), 1.0);
+                            }
+                            if (j == m && i == l) R.add(L.one_body(d1o, nn, k), 1.0);
+                            if (i == l) signed_d2(d2o, nn, j, k, m, -1.0);
+                            if (j == m) {
+                                const int hh = sy(nn) ^ sy(i);
+                                R.add(s ? dab(hh, L.ab(hh, nn, i), L.ab(hh, k, l)) : dab(hh, L.ab(hh, i, nn), L.ab(hh, l, k)), -1.0);
+                            }
                         }
                         R.end(0.0);
                     }

@@ -11,6 +11,8 @@
 //   basis.cc:642-744    triplets (T1): loop order i, j, k within an irrep;
 //                       aab keeps i < j, aaa keeps i < j < k
 //   v2rdm_solver.cc:3848-3873  T1aaa, T1bbb, T1aab, T1bba after G2
+//   v2rdm_solver.cc:3879-3908  T2aaa, T2bbb (trip_aab + trip_aba), T2aab,
+//                       T2bba after T1
 #include "layout.hpp"
 
 #include <algorithm>
@@ -23,7 +25,7 @@ namespace v2h {
 std::string ProblemDesc::key() const {
     std::ostringstream k;
     k << nmo << ':' << nalpha << ':' << nbeta << ':' << (multiplicity ? multiplicity : nalpha - nbeta + 1)
-      << ':' << nirrep << ':' << q2 << g2 << spin << sz << t1 << ':';
+      << ':' << nirrep << ':' << q2 << g2 << spin << sz << t1 << t2 << ':';
     std::vector<int> s = irreps.empty() ? std::vector<int>(nmo, 0) : irreps;
     std::sort(s.begin(), s.end());  // the tables depend only on the irrep counts
     for (int v : s) k << v << ',';
@@ -39,7 +41,8 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
     if (n <= 0) throw std::invalid_argument("nmo must be positive");
     if (H != 1 && H != 2 && H != 4 && H != 8) throw std::invalid_argument("nirrep must be 1, 2, 4 or 8");
     if (na < 0 || nb < 0 || na > n || nb > n) throw std::invalid_argument("electron counts out of range");
-    if (d.t1 && !(d.q2 && d.g2)) throw std::invalid_argument("the T1 condition needs Q2 and G2 (POSITIVITY DQGT1)");
+    if ((d.t1 || d.t2) && !(d.q2 && d.g2))
+        throw std::invalid_argument("the T1 and T2 conditions need Q2 and G2 (POSITIVITY DQGT1, DQGT2, DQGT1T2)");
     std::vector<int> irr = d.irreps.empty() ? std::vector<int>(n, 0) : d.irreps;
     if ((int)irr.size() != n) throw std::invalid_argument("orbital_irreps has the wrong length");
     for (int v : irr)
@@ -76,7 +79,7 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
         gaa[h] = (int)aa_pairs[h].size();
     }
 
-    if (d.t1)
+    if (d.t1 || d.t2)
         for (int h = 0; h < H; h++) {
             aab_index[h].assign((size_t)n * n * n, -1);
             aaa_index[h].assign((size_t)n * n * n, -1);
@@ -84,7 +87,9 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
             for (int i = 0; i < n; i++)
                 for (int j = 0; j < n; j++)
                     for (int k = 0; k < n; k++) {
-                        if ((sym[i] ^ sym[j] ^ sym[k]) != h || i >= j) continue;
+                        if ((sym[i] ^ sym[j] ^ sym[k]) != h) continue;
+                        aba_trips[h].push_back({i, j, k});
+                        if (i >= j) continue;
                         const int c = (int)aab_trips[h].size();
                         at(aab_index[h], i, j, k) = at(aab_index[h], j, i, k) = c;
                         aab_trips[h].push_back({i, j, k});
@@ -125,6 +130,12 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
         for (int h = 0; h < H; h++) push(T1BBB, h, taaa(h));
         for (int h = 0; h < H; h++) push(T1AAB, h, taab(h));
         for (int h = 0; h < H; h++) push(T1BBA, h, taab(h));
+    }
+    if (d.t2) {
+        for (int h = 0; h < H; h++) push(T2AAA, h, taab(h) + taba(h));
+        for (int h = 0; h < H; h++) push(T2BBB, h, taab(h) + taba(h));
+        for (int h = 0; h < H; h++) push(T2AAB, h, taab(h));
+        for (int h = 0; h < H; h++) push(T2BBA, h, taab(h));
     }
     n_primal = o;
     if (n_primal >= (int64_t)1 << 24)

@@ -21,6 +21,7 @@ struct ProblemDesc {
     std::vector<int> irreps;  // per orbital (user order); empty = all 0
     bool q2 = true, g2 = true, spin = true, sz = true;
     bool t1 = false;          // POSITIVITY DQGT1 (needs q2 and g2)
+    bool t2 = false;          // POSITIVITY DQGT2 (needs q2 and g2); both: DQGT1T2
 
     // Everything that determines the tables (structures with equal keys share them).
     std::string key() const;
@@ -30,6 +31,7 @@ struct ProblemDesc {
 enum Fam : int {
     D2AB = 0, D2AA, D2BB, D200, D1A, D1B, Q1A, Q1B, Q2AB, Q2AA, Q2BB, G2AB, G2BA, G2AA,
     T1AAA, T1BBB, T1AAB, T1BBA,  // after G2 (v2rdm_solver.cc:3848-3873)
+    T2AAA, T2BBB, T2AAB, T2BBA,  // after T1 (v2rdm_solver.cc:3879-3908)
     NFAM
 };
 
@@ -47,9 +49,10 @@ struct Layout {
     // pairs[h][k] = (first, second); index tables [h][i*n + j] (-1 when absent)
     std::array<std::vector<std::array<int, 2>>, kMaxIrrep> ab_pairs, aa_pairs;
     std::array<std::vector<int>, kMaxIrrep> ab_index, aa_index;
-    // T1 only: triplets of irrep h (basis.cc:642-744), aab with i < j, aaa
-    // with i < j < k; index tables [h][(i*n + j)*n + k] (-1 when absent)
-    std::array<std::vector<std::array<int, 3>>, kMaxIrrep> aab_trips, aaa_trips;
+    // T1 / T2 only: triplets of irrep h (basis.cc:642-744), aba all of them,
+    // aab with i < j, aaa with i < j < k; index tables [h][(i*n + j)*n + k]
+    // (-1 when absent)
+    std::array<std::vector<std::array<int, 3>>, kMaxIrrep> aab_trips, aaa_trips, aba_trips;
     std::array<std::vector<int>, kMaxIrrep> aab_index, aaa_index;
     std::array<std::array<int64_t, kMaxIrrep>, NFAM> off{};
     std::vector<Block> blocks;  // in layout order, zero-size blocks included
@@ -61,6 +64,7 @@ struct Layout {
     int aa(int h, int i, int j) const { return aa_index[h][i * n + j]; }
     int taab(int h) const { return (int)aab_trips[h].size(); }
     int taaa(int h) const { return (int)aaa_trips[h].size(); }
+    int taba(int h) const { return (int)aba_trips[h].size(); }
     int pair_irrep(int i, int j) const { return sym[i] ^ sym[j]; }
     // element (i, j) of a one-body block of family f; i, j in the same irrep
     int64_t one_body(int f, int i, int j) const {

@@ -60,9 +60,11 @@ DEFAULTS = {
     "NIRREP": None,                 # addition: 1, 2, 4 or 8 (default: from IRREPS)
 }
 
-# POSITIVITY -> (q2, g2, t1) (v2rdm_solver.cc:4081-4112)
-_POSITIVITY = {"D": (False, False, False), "DQ": (True, False, False), "DG": (False, True, False),
-               "DQG": (True, True, False), "DQGT1": (True, True, True)}
+# POSITIVITY -> (q2, g2, t1, t2) (v2rdm_solver.cc:4081-4112)
+_POSITIVITY = {"D": (False, False, False, False), "DQ": (True, False, False, False),
+               "DG": (False, True, False, False), "DQG": (True, True, False, False),
+               "DQGT1": (True, True, True, False), "DQGT2": (True, True, False, True),
+               "DQGT1T2": (True, True, True, True)}
 
 
 def resolve_options(options=None) -> dict:
@@ -74,7 +76,8 @@ def resolve_options(options=None) -> dict:
             raise KeyError(f"unknown option {k!r}")
         opts[key] = v.upper() if isinstance(v, str) else v
     if opts["POSITIVITY"] not in _POSITIVITY:
-        raise NotImplementedError(f"POSITIVITY {opts['POSITIVITY']}: only D, DQ, DG, DQG and DQGT1 are implemented")
+        raise NotImplementedError(f"POSITIVITY {opts['POSITIVITY']}: only D, DQ, DG, DQG, DQGT1, DQGT2 and "
+                                  "DQGT1T2 are implemented")
     if opts["SDP_SOLVER"] != "BPSDP":
         raise NotImplementedError(f"SDP_SOLVER {opts['SDP_SOLVER']}: only BPSDP is implemented")
     if opts["TPDM_GUESS"] not in ("RANDOM", "HF"):
@@ -114,13 +117,13 @@ class v2RDMHelper:
         n = int(nmo)
         self.h = np.asarray(h, dtype=np.float64).reshape(n, n)
         self.g = np.asarray(g, dtype=np.float64).reshape(n, n, n, n)
-        q2, g2, t1 = _POSITIVITY[o["POSITIVITY"]]
+        q2, g2, t1, t2 = _POSITIVITY[o["POSITIVITY"]]
         irreps = o["IRREPS"]
         nirrep = o["NIRREP"]
         if irreps is not None and nirrep is None:
             nirrep = 1 << int(np.ceil(np.log2(max(int(np.max(irreps)) + 1, 1))))
         self.problem = Problem(n, int(nalpha), int(nbeta), None if irreps is None else list(irreps), nirrep,
-                               0, q2, g2, bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1)
+                               0, q2, g2, bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1, t2)
         self._batch = None
         self.energy = None
 

@@ -61,6 +61,8 @@ typedef struct v2h_problem_desc {
     int constrain_sz;                 /* CONSTRAIN_SZ (plugin.cc:309) */
     int constrain_t1;                 /* POSITIVITY DQGT1 (v2rdm_solver.cc:4101-4104; needs q2 and g2):
                                          T1 blocks after G2, rows t1.cc:419-793 */
+    int constrain_t2;                 /* POSITIVITY DQGT2 / DQGT1T2 (v2rdm_solver.cc:4105-4112):
+                                         T2 blocks after T1, rows t2.cc:1236-1843 */
 } v2h_problem_desc;
 
 typedef struct v2h_status {

@@ -65,10 +65,10 @@ def lib():
         L.orc_pack_c_adjoint.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_hubbard_integrals.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.orc_df_integrals.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _dp, _dp, _dp]
-        L.orc_problem_enable_t1.restype = C.c_int
-        L.orc_problem_enable_t1.argtypes = [C.c_void_p]
-        L.orc_get_t1_offsets.argtypes = [C.c_void_p, _lp]
-        L.orc_get_trip_counts.argtypes = [C.c_void_p, _ip, _ip]
+        L.orc_problem_enable_t.restype = C.c_int
+        L.orc_problem_enable_t.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_get_t_offsets.argtypes = [C.c_void_p, _lp]
+        L.orc_get_trip_counts.argtypes = [C.c_void_p, _ip, _ip, _ip]
         L.orc_fcidump_map.argtypes = [C.c_int, C.c_int, C.c_void_p, _ip, _ip]
         L.orc_fcidump_integrals.restype = C.c_long
         L.orc_fcidump_integrals.argtypes = [C.c_int, C.c_void_p, _ip, _dp, C.c_int, _dp, C.c_double, C.c_void_p,
@@ -181,7 +181,7 @@ class OracleProblem:
     """One DQG problem in the oracle (tables, b, c-packing, maps, solve)."""
 
     def __init__(self, nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=None,
-                 q2=True, g2=True, spin=True, sz=True, t1=False):
+                 q2=True, g2=True, spin=True, sz=True, t1=False, t2=False):
         L = lib()
         if irreps is None:
             irreps = [0] * nmo
@@ -199,9 +199,9 @@ class OracleProblem:
                                     int(q2), int(g2), int(spin), int(sz))
         if not self._p:
             raise ValueError("invalid oracle problem")
-        self.t1 = bool(t1)
-        if t1 and L.orc_problem_enable_t1(self._p) != 0:
-            raise ValueError("T1 needs Q2 and G2")
+        self.t1, self.t2 = bool(t1), bool(t2)
+        if (t1 or t2) and L.orc_problem_enable_t(self._p, int(t1), int(t2)) != 0:
+            raise ValueError("T1 and T2 need Q2 and G2")
         self.n_primal = L.orc_n_primal(self._p)
         self.n_dual = L.orc_n_dual(self._p)
         nb = L.orc_nblocks(self._p)
@@ -257,17 +257,20 @@ class OracleProblem:
         return out
 
     def t1_offsets(self):
-        """{"t1aaa", "t1bbb", "t1aab", "t1bba"}: per-irrep block offsets."""
+        """{"t1aaa", "t1bbb", "t1aab", "t1bba", "t2aaa", "t2bbb", "t2aab",
+        "t2bba"}: per-irrep block offsets."""
         H = self.nirrep
-        off = np.zeros(4 * H, dtype=np.int64)
-        lib().orc_get_t1_offsets(self._p, off)
-        return {k: off[a * H:(a + 1) * H] for a, k in enumerate(("t1aaa", "t1bbb", "t1aab", "t1bba"))}
+        off = np.zeros(8 * H, dtype=np.int64)
+        lib().orc_get_t_offsets(self._p, off)
+        names = ("t1aaa", "t1bbb", "t1aab", "t1bba", "t2aaa", "t2bbb", "t2aab", "t2bba")
+        return {k: off[a * H:(a + 1) * H] for a, k in enumerate(names)}
 
-    def trip_counts(self):
+    def trip_counts(self, aba=False):
         taab = np.zeros(self.nirrep, dtype=np.int32)
         taaa = np.zeros(self.nirrep, dtype=np.int32)
-        lib().orc_get_trip_counts(self._p, taab, taaa)
-        return taab, taaa
+        taba = np.zeros(self.nirrep, dtype=np.int32)
+        lib().orc_get_trip_counts(self._p, taab, taaa, taba)
+        return (taab, taaa, taba) if aba else (taab, taaa)
 
     def fcidump_rdms(self, x, act_full, lab_map, ninact):
         """(values, labels[:, 4]) of the RDM file, in writing order."""

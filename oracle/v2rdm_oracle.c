@@ -163,15 +163,15 @@ void orc_problem_free(orc_problem *p) {
         free(p->ibas_ab[h]); free(p->ibas_aa[h]);
         free(p->bas_ab[h]); free(p->bas_aa[h]);
         free(p->ibas_aab[h]); free(p->ibas_aaa[h]);
-        free(p->bas_aab[h]); free(p->bas_aaa[h]);
+        free(p->bas_aab[h]); free(p->bas_aaa[h]); free(p->bas_aba[h]);
     }
     free(p->symmetry);
     memset(p, 0, sizeof(*p));
 }
 
-/* basis.cc:642-744: triplets (i,j,k) of irrep h in loop order i, j, k; aab
- * keeps i < j, aaa keeps i < j < k; the index tables are filled for the
- * swapped first two (aab) and for all six orders (aaa). */
+/* basis.cc:642-744: triplets (i,j,k) of irrep h in loop order i, j, k; aba
+ * keeps all, aab keeps i < j, aaa keeps i < j < k; the index tables are
+ * filled for the swapped first two (aab) and for all six orders (aaa). */
 static void build_triplets(orc_problem *p) {
     const int n = NMO;
     const long n3 = (long)n * n * n;
@@ -180,13 +180,18 @@ static void build_triplets(orc_problem *p) {
         p->ibas_aaa[h] = (int *)malloc(sizeof(int) * n3);
         p->bas_aab[h] = (int *)malloc(sizeof(int) * 3 * n3);
         p->bas_aaa[h] = (int *)malloc(sizeof(int) * 3 * n3);
+        p->bas_aba[h] = (int *)malloc(sizeof(int) * 3 * n3);
         for (long e = 0; e < n3; e++) p->ibas_aab[h][e] = p->ibas_aaa[h][e] = -999;
-        int caab = 0, caaa = 0;
+        int caab = 0, caaa = 0, caba = 0;
 #define I3(a, b, c) (((long)(a) * n + (b)) * n + (c))
         for (int i = 0; i < n; i++)
             for (int j = 0; j < n; j++)
                 for (int k = 0; k < n; k++) {
                     if ((SYM(i) ^ SYM(j) ^ SYM(k)) != h) continue;
+                    p->bas_aba[h][3 * caba] = i;
+                    p->bas_aba[h][3 * caba + 1] = j;
+                    p->bas_aba[h][3 * caba + 2] = k;
+                    caba++;
                     if (i >= j) continue;
                     p->ibas_aab[h][I3(i, j, k)] = p->ibas_aab[h][I3(j, i, k)] = caab;
                     p->bas_aab[h][3 * caab] = i;
@@ -204,41 +209,51 @@ static void build_triplets(orc_problem *p) {
 #undef I3
         p->trip_aab[h] = caab;
         p->trip_aaa[h] = caaa;
+        p->trip_aba[h] = caba;
     }
 }
 
-int orc_problem_enable_t1(orc_problem *p) {
-    if (!p->constrain_q2 || !p->constrain_g2 || p->constrain_t1) return -1;
-    p->constrain_t1 = 1;
+int orc_problem_enable_t(orc_problem *p, int t1, int t2) {
+    if (!p->constrain_q2 || !p->constrain_g2 || p->constrain_t1 || p->constrain_t2) return -1;
+    if (!t1 && !t2) return 0;
+    p->constrain_t1 = t1 != 0;
+    p->constrain_t2 = t2 != 0;
     build_triplets(p);
     const int H = p->nirrep;
     long off = p->n_primal;
     int nb = p->nblocks;
-    /* v2rdm_solver.cc:3848-3873 */
-    for (int h = 0; h < H; h++) { p->t1aaaoff[h] = off; off += (long)p->trip_aaa[h] * p->trip_aaa[h]; p->dims[nb++] = p->trip_aaa[h]; }
-    for (int h = 0; h < H; h++) { p->t1bbboff[h] = off; off += (long)p->trip_aaa[h] * p->trip_aaa[h]; p->dims[nb++] = p->trip_aaa[h]; }
-    for (int h = 0; h < H; h++) { p->t1aaboff[h] = off; off += (long)p->trip_aab[h] * p->trip_aab[h]; p->dims[nb++] = p->trip_aab[h]; }
-    for (int h = 0; h < H; h++) { p->t1bbaoff[h] = off; off += (long)p->trip_aab[h] * p->trip_aab[h]; p->dims[nb++] = p->trip_aab[h]; }
+#define PUSHT(arr, dim) do { (arr)[h] = off; off += (long)(dim) * (dim); p->dims[nb++] = (dim); \
+                             p->n_dual += (long)(dim) * (dim); } while (0)
+    if (t1) { /* v2rdm_solver.cc:3848-3873; rows :3554-3567 */
+        for (int h = 0; h < H; h++) PUSHT(p->t1aaaoff, p->trip_aaa[h]);
+        for (int h = 0; h < H; h++) PUSHT(p->t1bbboff, p->trip_aaa[h]);
+        for (int h = 0; h < H; h++) PUSHT(p->t1aaboff, p->trip_aab[h]);
+        for (int h = 0; h < H; h++) PUSHT(p->t1bbaoff, p->trip_aab[h]);
+    }
+    if (t2) { /* v2rdm_solver.cc:3879-3908; rows :3568-3581 */
+        for (int h = 0; h < H; h++) PUSHT(p->t2aaaoff, p->trip_aab[h] + p->trip_aba[h]);
+        for (int h = 0; h < H; h++) PUSHT(p->t2bbboff, p->trip_aab[h] + p->trip_aba[h]);
+        for (int h = 0; h < H; h++) PUSHT(p->t2aaboff, p->trip_aab[h]);
+        for (int h = 0; h < H; h++) PUSHT(p->t2bbaoff, p->trip_aab[h]);
+    }
+#undef PUSHT
     p->n_primal = off;
     p->nblocks = nb;
-    /* v2rdm_solver.cc:3554-3567 */
-    for (int h = 0; h < H; h++) p->n_dual += 2L * p->trip_aaa[h] * p->trip_aaa[h] + 2L * p->trip_aab[h] * p->trip_aab[h];
     return 0;
 }
 
-void orc_get_t1_offsets(const orc_problem *p, long *off) {
-    for (int h = 0; h < p->nirrep; h++) {
-        off[h] = p->t1aaaoff[h];
-        off[p->nirrep + h] = p->t1bbboff[h];
-        off[2 * p->nirrep + h] = p->t1aaboff[h];
-        off[3 * p->nirrep + h] = p->t1bbaoff[h];
-    }
+void orc_get_t_offsets(const orc_problem *p, long *off) {
+    const long *arrs[8] = {p->t1aaaoff, p->t1bbboff, p->t1aaboff, p->t1bbaoff,
+                           p->t2aaaoff, p->t2bbboff, p->t2aaboff, p->t2bbaoff};
+    for (int a = 0; a < 8; a++)
+        for (int h = 0; h < p->nirrep; h++) off[a * p->nirrep + h] = arrs[a][h];
 }
 
-void orc_get_trip_counts(const orc_problem *p, int *taab, int *taaa) {
+void orc_get_trip_counts(const orc_problem *p, int *taab, int *taaa, int *taba) {
     for (int h = 0; h < p->nirrep; h++) {
         taab[h] = p->trip_aab[h];
         taaa[h] = p->trip_aaa[h];
+        taba[h] = p->trip_aba[h];
     }
 }
 
@@ -667,6 +682,156 @@ static long t1_rows(const orc_problem *p, long r, t1_term_fn term, void *ctx) {
     return r;
 }
 
+/* ---- T2 rows (t2.cc:1236-1843, T2_constraints_Au_slow) ------------------ */
+/* Row order T2aab, T2bba, T2aaa (one block of dimension trip_aab + trip_aba:
+ * aab x aab, aab x aba, aba x aab, aba x aba), T2bbb (the same four parts);
+ * every row is - T2(ijk,lmn) plus the D2 and D1 elements t2.cc names. */
+static long t2_rows(const orc_problem *p, long r, t1_term_fn term, void *ctx) {
+    const int H = p->nirrep;
+#define T3(arr, h, a, c) (p->arr[(h)][3 * (a) + (c)])
+#define D1(off, a, b) ((off)[SYM(a)] + (long)((a) - p->pitzer_offset[SYM(a)]) * p->amopi[SYM(a)] + ((b) - p->pitzer_offset[SYM(a)]))
+#define DAB(hh, a, b) (p->d2aboff[(hh)] + (long)(a) * GAB(hh) + (b))
+#define DSS(offs, hh, a, b) ((offs)[(hh)] + (long)(a) * GAA(hh) + (b))
+    /* signed D2aa / D2bb term of the T2aaa-type parts: s (x > y) (u > v) on
+     * the pair (x, y), (u, v), skipped when a pair repeats an orbital */
+#define SIGNED(offs, x, y, u, v, c0)                                                          \
+    do {                                                                                      \
+        if ((x) != (y) && (u) != (v)) {                                                       \
+            const int hh = SYM(x) ^ SYM(y);                                                   \
+            double sg = (c0);                                                                 \
+            if ((x) > (y)) sg = -sg;                                                          \
+            if ((u) > (v)) sg = -sg;                                                          \
+            term(row, DSS(offs, hh, IAA(hh, x, y), IAA(hh, u, v)), sg, ctx);                  \
+        }                                                                                     \
+    } while (0)
+    for (int s = 0; s < 2; s++) /* 0: T2aab, 1: T2bba */
+        for (int h = 0; h < H; h++) {
+            const int t = p->trip_aab[h];
+            const long toff = s == 0 ? p->t2aaboff[h] : p->t2bbaoff[h];
+            const long *d2s = s == 0 ? p->d2aaoff : p->d2bboff;
+            const long *d1o = s == 0 ? p->d1boff : p->d1aoff;
+            for (int ijk = 0; ijk < t; ijk++) {
+                const int i = T3(bas_aab, h, ijk, 0), j = T3(bas_aab, h, ijk, 1), k = T3(bas_aab, h, ijk, 2);
+                for (int lmn = 0; lmn < t; lmn++) {
+                    const int l = T3(bas_aab, h, lmn, 0), m = T3(bas_aab, h, lmn, 1), n = T3(bas_aab, h, lmn, 2);
+                    const long row = r + (long)ijk * t + lmn;
+                    term(row, toff + (long)ijk * t + lmn, -1.0, ctx);
+                    if (k == n) {
+                        const int hij = SYM(i) ^ SYM(j);
+                        term(row, DSS(d2s, hij, IAA(hij, i, j), IAA(hij, l, m)), 1.0, ctx);
+                    }
+                    if (j == m && i == l) term(row, D1(d1o, n, k), 1.0, ctx);
+                    if (j == l && i == m) term(row, D1(d1o, n, k), -1.0, ctx);
+                    if (i == l) {
+                        const int hh = SYM(n) ^ SYM(j);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, j, n), IAB(hh, m, k)) : DAB(hh, IAB(hh, n, j), IAB(hh, k, m)), -1.0, ctx);
+                    }
+                    if (j == l) {
+                        const int hh = SYM(n) ^ SYM(i);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, i, n), IAB(hh, m, k)) : DAB(hh, IAB(hh, n, i), IAB(hh, k, m)), 1.0, ctx);
+                    }
+                    if (i == m) {
+                        const int hh = SYM(n) ^ SYM(j);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, j, n), IAB(hh, l, k)) : DAB(hh, IAB(hh, n, j), IAB(hh, k, l)), 1.0, ctx);
+                    }
+                    if (j == m) {
+                        const int hh = SYM(n) ^ SYM(i);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, i, n), IAB(hh, l, k)) : DAB(hh, IAB(hh, n, i), IAB(hh, k, l)), -1.0, ctx);
+                    }
+                }
+            }
+            r += (long)t * t;
+        }
+    for (int s = 0; s < 2; s++) /* 0: T2aaa (+ abb), 1: T2bbb (+ baa) */
+        for (int h = 0; h < H; h++) {
+            const int ta = p->trip_aab[h], tb = p->trip_aba[h], D = ta + tb;
+            const long toff = s == 0 ? p->t2aaaoff[h] : p->t2bbboff[h];
+            const long *d2s = s == 0 ? p->d2aaoff : p->d2bboff;  /* same-spin D2 of the aab part */
+            const long *d2o = s == 0 ? p->d2bboff : p->d2aaoff;  /* same-spin D2 of the aba part */
+            const long *d1s = s == 0 ? p->d1aoff : p->d1boff;
+            const long *d1o = s == 0 ? p->d1boff : p->d1aoff;
+            /* aab x aab */
+            for (int ijk = 0; ijk < ta; ijk++) {
+                const int i = T3(bas_aab, h, ijk, 0), j = T3(bas_aab, h, ijk, 1), k = T3(bas_aab, h, ijk, 2);
+                for (int lmn = 0; lmn < ta; lmn++) {
+                    const int l = T3(bas_aab, h, lmn, 0), m = T3(bas_aab, h, lmn, 1), n = T3(bas_aab, h, lmn, 2);
+                    const long id = (long)ijk * D + lmn, row = r + id;
+                    term(row, toff + id, -1.0, ctx);
+                    if (k == n) {
+                        const int hij = SYM(i) ^ SYM(j);
+                        term(row, DSS(d2s, hij, IAA(hij, i, j), IAA(hij, l, m)), 1.0, ctx);
+                    }
+                    if (j == m && i == l) term(row, D1(d1s, n, k), 1.0, ctx);
+                    if (j == l && i == m) term(row, D1(d1s, n, k), -1.0, ctx);
+                    if (i == l) SIGNED(d2s, n, j, k, m, -1.0);
+                    if (j == l) SIGNED(d2s, n, i, k, m, 1.0);
+                    if (i == m) SIGNED(d2s, n, j, k, l, 1.0);
+                    if (j == m) SIGNED(d2s, n, i, k, l, -1.0);
+                }
+            }
+            /* aab x aba */
+            for (int ijk = 0; ijk < ta; ijk++) {
+                const int i = T3(bas_aab, h, ijk, 0), j = T3(bas_aab, h, ijk, 1), k = T3(bas_aab, h, ijk, 2);
+                for (int lmn = 0; lmn < tb; lmn++) {
+                    const int l = T3(bas_aba, h, lmn, 0), m = T3(bas_aba, h, lmn, 1), n = T3(bas_aba, h, lmn, 2);
+                    const long id = (long)ijk * D + (lmn + ta), row = r + id;
+                    term(row, toff + id, -1.0, ctx);
+                    if (i == l) {
+                        const int hh = SYM(j) ^ SYM(n);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, j, n), IAB(hh, k, m)) : DAB(hh, IAB(hh, n, j), IAB(hh, m, k)), 1.0, ctx);
+                    }
+                    if (j == l) {
+                        const int hh = SYM(i) ^ SYM(n);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, i, n), IAB(hh, k, m)) : DAB(hh, IAB(hh, n, i), IAB(hh, m, k)), -1.0, ctx);
+                    }
+                }
+            }
+            /* aba x aab */
+            for (int ijk = 0; ijk < tb; ijk++) {
+                const int i = T3(bas_aba, h, ijk, 0), j = T3(bas_aba, h, ijk, 1), k = T3(bas_aba, h, ijk, 2);
+                for (int lmn = 0; lmn < ta; lmn++) {
+                    const int l = T3(bas_aab, h, lmn, 0), m = T3(bas_aab, h, lmn, 1), n = T3(bas_aab, h, lmn, 2);
+                    const long id = (long)(ijk + ta) * D + lmn, row = r + id;
+                    term(row, toff + id, -1.0, ctx);
+                    if (i == l) {
+                        const int hh = SYM(j) ^ SYM(n);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, n, j), IAB(hh, m, k)) : DAB(hh, IAB(hh, j, n), IAB(hh, k, m)), 1.0, ctx);
+                    }
+                    if (i == m) {
+                        const int hh = SYM(j) ^ SYM(n);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, n, j), IAB(hh, l, k)) : DAB(hh, IAB(hh, j, n), IAB(hh, k, l)), -1.0, ctx);
+                    }
+                }
+            }
+            /* aba x aba */
+            for (int ijk = 0; ijk < tb; ijk++) {
+                const int i = T3(bas_aba, h, ijk, 0), j = T3(bas_aba, h, ijk, 1), k = T3(bas_aba, h, ijk, 2);
+                for (int lmn = 0; lmn < tb; lmn++) {
+                    const int l = T3(bas_aba, h, lmn, 0), m = T3(bas_aba, h, lmn, 1), n = T3(bas_aba, h, lmn, 2);
+                    const long id = (long)(ijk + ta) * D + (lmn + ta), row = r + id;
+                    term(row, toff + id, -1.0, ctx);
+                    if (k == n) {
+                        const int hij = SYM(i) ^ SYM(j);
+                        term(row, s == 0 ? DAB(hij, IAB(hij, i, j), IAB(hij, l, m)) : DAB(hij, IAB(hij, j, i), IAB(hij, m, l)), 1.0, ctx);
+                    }
+                    if (j == m && i == l) term(row, D1(d1o, n, k), 1.0, ctx);
+                    if (i == l) SIGNED(d2o, n, j, k, m, -1.0);
+                    if (j == m) {
+                        const int hh = SYM(n) ^ SYM(i);
+                        term(row, s == 0 ? DAB(hh, IAB(hh, i, n), IAB(hh, l, k)) : DAB(hh, IAB(hh, n, i), IAB(hh, k, l)), -1.0, ctx);
+                    }
+                }
+            }
+            r += (long)D * D;
+        }
+#undef SIGNED
+#undef DSS
+#undef DAB
+#undef D1
+#undef T3
+    return r;
+}
+
 struct t1_ctx { const double *in; double *out; };
 static void t1_gather(long row, long col, double coef, void *c) {
     struct t1_ctx *x = (struct t1_ctx *)c;
@@ -1026,6 +1191,10 @@ void orc_Au(const orc_problem *p, const double *u, double *A) {
         struct t1_ctx c = {u, A};
         r = t1_rows(p, r, t1_gather, &c);
     }
+    if (p->constrain_t2) {
+        struct t1_ctx c = {u, A};
+        r = t2_rows(p, r, t1_gather, &c);
+    }
 }
 
 /* ---- A^T u -------------------------------------------------------------- */
@@ -1381,6 +1550,10 @@ void orc_ATu(const orc_problem *p, const double *u, double *A) {
     if (p->constrain_t1) {
         struct t1_ctx c = {u, A};
         r = t1_rows(p, r, t1_scatter, &c);
+    }
+    if (p->constrain_t2) {
+        struct t1_ctx c = {u, A};
+        r = t2_rows(p, r, t1_scatter, &c);
     }
 }
 

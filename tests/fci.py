@@ -246,3 +246,37 @@ def pack_t1(P, F, psi, x):
                     L_, M_, N_ = l + sp[0] * n, m + sp[1] * n, q + sp[2] * n
                     x[off[name][h] + a * t + b] = A3[K, J, I] @ A3[N_, M_, L_] + C3[K, J, I] @ C3[N_, M_, L_]
     return x
+
+
+def pack_t2(P, F, psi, x):
+    """Fill the T2 blocks of x (oracle layout, P built with t2=True) with the
+    exact T2(ijk,lmn) = <B_ijk B+_lmn> + <B+_lmn B_ijk>, B_ijk = a+_i a+_j a_k,
+    of psi.  Spins: T2aab (a, a, b), T2bba (b, b, a); T2aaa is one block over
+    the aab triplets with spins (a, a, a) followed by all triplets with
+    spins (a, b, b); T2bbb likewise with (b, b, b) and (b, a, a)."""
+    import itertools as it
+    n, M = F.n, F.M
+    irr = np.asarray(P.irreps)
+    ann = [F._ann[k] for k in range(M)]
+    cre = [F._ann[k].T.tocsr() for k in range(M)]
+    Bv, Bd = {}, {}
+    for a, b, c in it.product(range(M), repeat=3):
+        Bv[a, b, c] = cre[a] @ (cre[b] @ (ann[c] @ psi))
+        Bd[a, b, c] = cre[c] @ (ann[b] @ (ann[a] @ psi))
+    off = P.t1_offsets()
+
+    def val(p, q):
+        return Bd[p] @ Bd[q] + Bv[q] @ Bv[p]
+
+    for h in range(P.nirrep):
+        trips = [(i, j, k) for i in range(n) for j in range(n) for k in range(n) if irr[i] ^ irr[j] ^ irr[k] == h]
+        aab = [t for t in trips if t[0] < t[1]]
+        for name, parts in (("t2aab", [(aab, (0, 0, 1))]), ("t2bba", [(aab, (1, 1, 0))]),
+                            ("t2aaa", [(aab, (0, 0, 0)), (trips, (0, 1, 1))]),
+                            ("t2bbb", [(aab, (1, 1, 1)), (trips, (1, 0, 0))])):
+            rows = [tuple(o + s * n for o, s in zip(t, sp)) for tl, sp in parts for t in tl]
+            d = len(rows)
+            for a, p in enumerate(rows):
+                for b, q in enumerate(rows):
+                    x[off[name][h] + a * d + b] = val(p, q)
+    return x

@@ -5,7 +5,7 @@ quantisation), with the D200 blocks left zero as in the reference."""
 import numpy as np
 import pytest
 
-from fci import Fock, RDMs, pack_primal, pack_t1
+from fci import Fock, RDMs, pack_primal, pack_t1, pack_t2
 from hilbert_amd.structure import Structure
 from oracle.pyoracle import OracleProblem
 
@@ -75,3 +75,25 @@ def test_hf_guess_with_t1_is_the_determinant(n, na, irr, H):
         x_ref[off["d200"][h]:off["d200"][h] + d * d] = 0.0
     x = S.hf_guess()
     assert np.abs(x - x_ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("n,na,irr,H", CASES[:3])
+def test_hf_guess_with_t1_t2_is_the_determinant(n, na, irr, H):
+    """POSITIVITY DQGT1T2: T1 and T2 blocks of the guess are those of the
+    determinant (every T2 row holds one T2 element besides D1 and D2)."""
+    irreps = irr if irr is not None else [0] * n
+    P = OracleProblem(n, na, na, irreps, H, t1=True, t2=True)
+    P0 = OracleProblem(n, na, na, irreps, H)
+    S = Structure(n, na, na, irr, H, t1=True, t2=True)
+    F = Fock(n)
+    psi = _determinant(F, n, na, na)
+    x_ref = np.zeros(P.n_primal)
+    x_ref[:P0.n_primal] = pack_primal(P0, RDMs(F, psi))
+    pack_t1(P, F, psi, x_ref)
+    pack_t2(P, F, psi, x_ref)
+    off = P.offsets()
+    dims = S.block_dims()
+    for h in range(H):
+        d = dims[3 * H + h]
+        x_ref[off["d200"][h]:off["d200"][h] + d * d] = 0.0
+    assert np.abs(S.hf_guess() - x_ref).max() < 1e-13

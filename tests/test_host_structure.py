@@ -62,14 +62,17 @@ def test_constraints_match_oracle(case, flags):
 T1_CASES = CASES[:5]
 
 
+@pytest.mark.parametrize("t12", [(True, False), (False, True), (True, True)], ids=["T1", "T2", "T1T2"])
 @pytest.mark.parametrize("spin_sz", [(True, True), (False, True), (True, False)])
 @pytest.mark.parametrize("case", T1_CASES)
-def test_t1_constraints_match_oracle(case, spin_sz):
-    """POSITIVITY DQGT1: T1 blocks after G2 and the T1 rows of t1.cc."""
+def test_t1_t2_constraints_match_oracle(case, spin_sz, t12):
+    """POSITIVITY DQGT1 / DQGT2 / DQGT1T2: T1 blocks after G2, T2 blocks after
+    them, and the rows of t1.cc and t2.cc's slow path."""
     n, na, nb, irr, H = case
     spin, sz = spin_sz
-    S = Structure(n, na, nb, irr, H, spin=spin, sz=sz, t1=True)
-    P = OracleProblem(n, na, nb, irr, H, spin=spin, sz=sz, t1=True)
+    t1, t2 = t12
+    S = Structure(n, na, nb, irr, H, spin=spin, sz=sz, t1=t1, t2=t2)
+    P = OracleProblem(n, na, nb, irr, H, spin=spin, sz=sz, t1=t1, t2=t2)
     assert (S.n_primal, S.n_dual) == (P.n_primal, P.n_dual)
     assert S.block_dims().tolist() == P.dims.tolist()
     A, b = S.matrix()
@@ -83,9 +86,11 @@ def test_t1_constraints_match_oracle(case, spin_sz):
     assert np.abs(A.T @ y - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
 
 
-def test_t1_needs_q2_and_g2():
+def test_t1_t2_need_q2_and_g2():
     with pytest.raises(Exception, match="Q2 and G2"):
         Structure(4, 2, 2, q2=False, t1=True)
+    with pytest.raises(Exception, match="Q2 and G2"):
+        Structure(4, 2, 2, g2=False, t2=True)
 
 
 @pytest.mark.parametrize("case", CASES)

@@ -18,7 +18,7 @@ def test_case_insensitive_names_and_values():
     assert o["POSITIVITY"] == "DQ" and o["MAXITER"] == 20000
 
 
-@pytest.mark.parametrize("opt", [{"POSITIVITY": "DQGT2"}, {"SDP_SOLVER": "RRSDP"},
+@pytest.mark.parametrize("opt", [{"POSITIVITY": "DQGT3"}, {"SDP_SOLVER": "RRSDP"},
                                  {"CONSTRAIN_D3": True}, {"GPC_CONSTRAINTS": "1RDM"}, {"TAU_PARAMETER": 1.5}])
 def test_unimplemented_settings_raise(opt):
     with pytest.raises(NotImplementedError):
