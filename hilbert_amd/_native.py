@@ -31,7 +31,7 @@ class ProblemDesc(C.Structure):
                 ("nirrep", C.c_int), ("orbital_irreps", C.POINTER(C.c_int)),
                 ("constrain_q2", C.c_int), ("constrain_g2", C.c_int), ("constrain_spin", C.c_int),
                 ("constrain_sz", C.c_int), ("constrain_t1", C.c_int),
-                ("constrain_t2", C.c_int)]
+                ("constrain_t2", C.c_int), ("constrain_d3", C.c_int)]
 
 
 class Status(C.Structure):
@@ -149,7 +149,7 @@ def check(rc: int):
 
 
 def make_desc(nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=0, q2=True, g2=True, spin=True,
-              sz=True, t1=False, t2=False):
+              sz=True, t1=False, t2=False, d3=False):
     """ProblemDesc plus the array that keeps its irrep pointer alive."""
     arr = None
     ptr = None
@@ -161,7 +161,7 @@ def make_desc(nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=0, q2=T
             while nirrep <= max(int(v) for v in irreps):
                 nirrep *= 2
     d = ProblemDesc(int(nmo), int(nalpha), int(nbeta), int(multiplicity), int(nirrep or 1), ptr,
-                    int(q2), int(g2), int(spin), int(sz), int(t1), int(t2))
+                    int(q2), int(g2), int(spin), int(sz), int(t1), int(t2), int(d3))
     return d, arr
 
 

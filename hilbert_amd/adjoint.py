@@ -32,7 +32,7 @@ def _problem(nmo, nalpha, nbeta, o):
     if irreps is not None and nirrep is None:
         nirrep = 1 << int(np.ceil(np.log2(max(int(np.max(irreps)) + 1, 1))))
     return Problem(nmo, nalpha, nbeta, None if irreps is None else list(irreps), nirrep, 0, q2, g2,
-                   bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1, t2)
+                   bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1, t2, bool(o["CONSTRAIN_D3"]))
 
 
 class _Energies(torch.autograd.Function):

@@ -26,6 +26,7 @@ class Problem:
     sz: bool = True
     t1: bool = False          # POSITIVITY DQGT1 (and DQGT1T2)
     t2: bool = False          # POSITIVITY DQGT2 (and DQGT1T2)
+    d3: bool = False          # CONSTRAIN_D3
 
 
 def options(e_convergence=1e-6, r_convergence=1e-5, cg_convergence=1e-9, cg_maxiter=10000, maxiter=50,
@@ -45,7 +46,8 @@ class Batch:
         keep = []
         for k, p in enumerate(problems):
             d, arr = N.make_desc(p.nmo, p.nalpha, p.nbeta, p.irreps, p.nirrep, p.multiplicity,
-                                 p.q2, p.g2, p.spin, p.sz, getattr(p, "t1", False), getattr(p, "t2", False))
+                                 p.q2, p.g2, p.spin, p.sz, getattr(p, "t1", False), getattr(p, "t2", False),
+                                 getattr(p, "d3", False))
             descs[k] = d
             keep.append(arr)
         self.problems = problems

@@ -153,6 +153,7 @@ int v2h_batch_create(const v2h_problem_desc* descs, int count, const v2h_sdp_opt
             d.sz = s.constrain_sz != 0;
             d.t1 = s.constrain_t1 != 0;
             d.t2 = s.constrain_t2 != 0;
+            d.d3 = s.constrain_d3 != 0;
         }
         auto h = std::make_unique<v2h_batch>();
         h->impl = std::make_unique<v2h::Batch>(ds, to_opts(opt), device);
@@ -614,6 +615,7 @@ int v2h_structure_create(const v2h_problem_desc* s, v2h_structure** out) {
         d.sz = s->constrain_sz != 0;
         d.t1 = s->constrain_t1 != 0;
         d.t2 = s->constrain_t2 != 0;
+        d.d3 = s->constrain_d3 != 0;
         *out = new v2h_structure(d);
     });
 }

@@ -3,8 +3,9 @@
 // Row families and their order follow hilbert's bpsdp_Au
 // (v2rdm_solver.cc:2995-3050): D2 (traces, hermiticity, D1/Q1, contractions;
 // d2.cc:314-594), spin (spin.cc:276-508), Q2 (q2.cc:170-317), G2
-// (g2.cc:245-441), then T1 and T2 when asked for (t1.cc:419-793 and the slow
-// path t2.cc:1236-1843, dispatched at v2rdm_solver.cc:3015-3021).  Each row is written here as the linear
+// (g2.cc:245-441), then T1, T2 and D3 when asked for (t1.cc:419-793, the
+// slow path t2.cc:1236-1843, d3.cc:45-406; dispatched at
+// v2rdm_solver.cc:3015-3044).  Each row is written here as the linear
 // functional it imposes on the primal vector, with its b value.
 #include "constraints.hpp"
 
@@ -540,6 +541,101 @@ Csr build_constraints(const Layout& L) {
             }
     }
 
+    // ---- D3 (d3.cc:45-406): contractions to D2 (with CONSTRAIN_SZ: aaa->aa,
+    // bbb->bb, aab->aa, bba->bb, aab->ab, bba->ab; without: (aaa+aab)->aa,
+    // (bbb+bba)->bb, (aab+bba)->ab), then for singlets with CONSTRAIN_SPIN
+    // D3aab = D3bba and D3aaa / D3bbb from D3aab / D3bba; b = 0
+    if (d.d3) {
+        auto sy = [&](int p) { return L.sym[p]; };
+        const int nsame = d.sz ? 2 : 1;
+        for (int pass = 0; pass < 2 * nsame; pass++) {
+            const int spin = pass % 2;
+            const int part = d.sz ? pass / 2 : -1;  // 0: from D3aaa / bbb, 1: from D3aab / bba, -1: both
+            const int d2 = spin ? D2BB : D2AA, f3s = spin ? D3BBB : D3AAA, f3m = spin ? D3BBA : D3AAB;
+            const double coef = part == 0 ? (spin ? nb : na) - 2.0 : part == 1 ? (spin ? na : nb) : na + nb - 2.0;
+            for (int h = 0; h < H; h++) {
+                const int g = L.gaa[h];
+                for (int a = 0; a < g; a++) {
+                    const auto [i, j] = L.aa_pairs[h][a];
+                    for (int b = 0; b < g; b++) {
+                        const auto [k, l] = L.aa_pairs[h][b];
+                        R.add(gem(d2, h, a, b, g), coef);
+                        for (int q = 0; q < n; q++) {
+                            const int h2 = h ^ sy(q);
+                            if (part != 0) R.add(gem(f3m, h2, L.aab(h2, i, j, q), L.aab(h2, k, l, q), L.taab(h2)), -1.0);
+                            if (part != 1 && i != q && j != q && k != q && l != q) {
+                                double c = -1.0;
+                                if (i < q && q < j) c = -c;
+                                if (k < q && q < l) c = -c;
+                                R.add(gem(f3s, h2, L.aaa(h2, i, j, q), L.aaa(h2, k, l, q), L.taaa(h2)), c);
+                            }
+                        }
+                        R.end(0.0);
+                    }
+                }
+            }
+        }
+        for (int pass = 0; pass < nsame; pass++) {
+            const double coef = d.sz ? (pass == 0 ? na - 1.0 : nb - 1.0) : na + nb - 2.0;
+            for (int h = 0; h < H; h++) {
+                const int g = L.gab[h];
+                for (int a = 0; a < g; a++) {
+                    const auto [i, j] = L.ab_pairs[h][a];
+                    for (int b = 0; b < g; b++) {
+                        const auto [k, l] = L.ab_pairs[h][b];
+                        R.add(gem(D2AB, h, a, b, g), coef);
+                        for (int q = 0; q < n; q++) {
+                            const int h2 = h ^ sy(q);
+                            if ((!d.sz || pass == 0) && i != q && k != q) {
+                                double c = -1.0;
+                                if (q < i) c = -c;
+                                if (q < k) c = -c;
+                                R.add(gem(D3AAB, h2, L.aab(h2, i, q, j), L.aab(h2, k, q, l), L.taab(h2)), c);
+                            }
+                            if ((!d.sz || pass == 1) && j != q && l != q) {
+                                double c = -1.0;
+                                if (q < j) c = -c;
+                                if (q < l) c = -c;
+                                R.add(gem(D3BBA, h2, L.aab(h2, j, q, i), L.aab(h2, l, q, k), L.taab(h2)), c);
+                            }
+                        }
+                        R.end(0.0);
+                    }
+                }
+            }
+        }
+        if (d.spin && L.singlet()) {
+            for (int h = 0; h < H; h++) {
+                const int t = L.taab(h);
+                for (int a = 0; a < t; a++)
+                    for (int b = 0; b < t; b++) {
+                        R.add(gem(D3AAB, h, a, b, t), 1.0);
+                        R.add(gem(D3BBA, h, a, b, t), -1.0);
+                        R.end(0.0);
+                    }
+            }
+            for (int s = 0; s < 2; s++)
+                for (int h = 0; h < H; h++) {
+                    const auto& T = L.aaa_trips[h];
+                    const int t = (int)T.size(), ta = L.taab(h);
+                    for (int a = 0; a < t; a++) {
+                        const auto [p, q, r] = T[a];
+                        const int rows3[3] = {L.aab(h, p, q, r), L.aab(h, p, r, q), L.aab(h, q, r, p)};
+                        for (int b = 0; b < t; b++) {
+                            const auto [s1, t1, v1] = T[b];
+                            const int cols3[3] = {L.aab(h, s1, t1, v1), L.aab(h, s1, v1, t1), L.aab(h, t1, v1, s1)};
+                            R.add(gem(s ? D3BBB : D3AAA, h, a, b, t), 1.0);
+                            for (int x = 0; x < 3; x++)
+                                for (int y = 0; y < 3; y++)
+                                    R.add(gem(s ? D3BBA : D3AAB, h, rows3[x], cols3[y], ta),
+                                          ((x + y) % 2 == 0 ? -1.0 : 1.0) / 3.0);
+                            R.end(0.0);
+                        }
+                    }
+                }
+        }
+    }
+
     A.nrows = (int64_t)A.rhs.size();
     return A;
 }
@@ -597,7 +693,8 @@ std::vector<double> hf_guess(const Layout& L, const Csr& A, int nocc_alpha, int 
     // elements that earlier rows have fixed; solve each row for its one
     // unknown, in row order
     std::vector<char> known(L.n_primal, 1);
-    for (int64_t c = 0; c < L.n_primal; c++) known[c] = fam[c] < Q2AB;
+    // the D3 blocks stay zero: Guess() has no D3 part (v2rdm_solver.cc:1808-1823)
+    for (int64_t c = 0; c < L.n_primal; c++) known[c] = fam[c] < Q2AB || fam[c] >= D3AAA;
     for (int64_t r = 0; r < A.nrows; r++) {
         int64_t e = -1;
         int nd = 0;

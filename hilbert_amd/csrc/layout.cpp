@@ -13,6 +13,7 @@
 //   v2rdm_solver.cc:3848-3873  T1aaa, T1bbb, T1aab, T1bba after G2
 //   v2rdm_solver.cc:3879-3908  T2aaa, T2bbb (trip_aab + trip_aba), T2aab,
 //                       T2bba after T1
+//   v2rdm_solver.cc:4000-4026  D3aaa, D3bbb, D3aab, D3bba after T2
 #include "layout.hpp"
 
 #include <algorithm>
@@ -25,7 +26,7 @@ namespace v2h {
 std::string ProblemDesc::key() const {
     std::ostringstream k;
     k << nmo << ':' << nalpha << ':' << nbeta << ':' << (multiplicity ? multiplicity : nalpha - nbeta + 1)
-      << ':' << nirrep << ':' << q2 << g2 << spin << sz << t1 << t2 << ':';
+      << ':' << nirrep << ':' << q2 << g2 << spin << sz << t1 << t2 << d3 << ':';
     std::vector<int> s = irreps.empty() ? std::vector<int>(nmo, 0) : irreps;
     std::sort(s.begin(), s.end());  // the tables depend only on the irrep counts
     for (int v : s) k << v << ',';
@@ -79,7 +80,7 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
         gaa[h] = (int)aa_pairs[h].size();
     }
 
-    if (d.t1 || d.t2)
+    if (d.t1 || d.t2 || d.d3)
         for (int h = 0; h < H; h++) {
             aab_index[h].assign((size_t)n * n * n, -1);
             aaa_index[h].assign((size_t)n * n * n, -1);
@@ -136,6 +137,12 @@ Layout::Layout(const ProblemDesc& d) : desc(d) {
         for (int h = 0; h < H; h++) push(T2BBB, h, taab(h) + taba(h));
         for (int h = 0; h < H; h++) push(T2AAB, h, taab(h));
         for (int h = 0; h < H; h++) push(T2BBA, h, taab(h));
+    }
+    if (d.d3) {
+        for (int h = 0; h < H; h++) push(D3AAA, h, taaa(h));
+        for (int h = 0; h < H; h++) push(D3BBB, h, taaa(h));
+        for (int h = 0; h < H; h++) push(D3AAB, h, taab(h));
+        for (int h = 0; h < H; h++) push(D3BBA, h, taab(h));
     }
     n_primal = o;
     if (n_primal >= (int64_t)1 << 24)

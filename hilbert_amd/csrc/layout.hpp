@@ -22,6 +22,7 @@ struct ProblemDesc {
     bool q2 = true, g2 = true, spin = true, sz = true;
     bool t1 = false;          // POSITIVITY DQGT1 (needs q2 and g2)
     bool t2 = false;          // POSITIVITY DQGT2 (needs q2 and g2); both: DQGT1T2
+    bool d3 = false;          // CONSTRAIN_D3
 
     // Everything that determines the tables (structures with equal keys share them).
     std::string key() const;
@@ -32,6 +33,7 @@ enum Fam : int {
     D2AB = 0, D2AA, D2BB, D200, D1A, D1B, Q1A, Q1B, Q2AB, Q2AA, Q2BB, G2AB, G2BA, G2AA,
     T1AAA, T1BBB, T1AAB, T1BBA,  // after G2 (v2rdm_solver.cc:3848-3873)
     T2AAA, T2BBB, T2AAB, T2BBA,  // after T1 (v2rdm_solver.cc:3879-3908)
+    D3AAA, D3BBB, D3AAB, D3BBA,  // after T2 (v2rdm_solver.cc:4000-4026)
     NFAM
 };
 
@@ -65,6 +67,8 @@ struct Layout {
     int taab(int h) const { return (int)aab_trips[h].size(); }
     int taaa(int h) const { return (int)aaa_trips[h].size(); }
     int taba(int h) const { return (int)aba_trips[h].size(); }
+    int aab(int h, int i, int j, int k) const { return aab_index[h][((size_t)i * n + j) * n + k]; }
+    int aaa(int h, int i, int j, int k) const { return aaa_index[h][((size_t)i * n + j) * n + k]; }
     int pair_irrep(int i, int j) const { return sym[i] ^ sym[j]; }
     // element (i, j) of a one-body block of family f; i, j in the same irrep
     int64_t one_body(int f, int i, int j) const {

@@ -12,8 +12,8 @@ from . import _native as N
 
 class Structure:
     def __init__(self, nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=0, q2=True, g2=True,
-                 spin=True, sz=True, t1=False, t2=False):
-        d, keep = N.make_desc(nmo, nalpha, nbeta, irreps, nirrep, multiplicity, q2, g2, spin, sz, t1, t2)
+                 spin=True, sz=True, t1=False, t2=False, d3=False):
+        d, keep = N.make_desc(nmo, nalpha, nbeta, irreps, nirrep, multiplicity, q2, g2, spin, sz, t1, t2, d3)
         self.nirrep = d.nirrep
         self.nmo = nmo
         h = C.c_void_p()

@@ -84,7 +84,7 @@ def resolve_options(options=None) -> dict:
         raise ValueError(f"TPDM_GUESS {opts['TPDM_GUESS']}: RANDOM or HF")
     if opts["GPC_CONSTRAINTS"] != "NONE":
         raise NotImplementedError("GPC_CONSTRAINTS are not implemented")
-    for k in ("CONSTRAIN_D3", "CONSTRAIN_E3", "CONSTRAIN_F3", "CONSTRAIN_Q3", "CONSTRAIN_D4"):
+    for k in ("CONSTRAIN_E3", "CONSTRAIN_F3", "CONSTRAIN_Q3", "CONSTRAIN_D4"):
         if opts[k]:
             raise NotImplementedError(f"{k} is not implemented")
     if float(opts["TAU_PARAMETER"]) != 1.0:
@@ -123,7 +123,8 @@ class v2RDMHelper:
         if irreps is not None and nirrep is None:
             nirrep = 1 << int(np.ceil(np.log2(max(int(np.max(irreps)) + 1, 1))))
         self.problem = Problem(n, int(nalpha), int(nbeta), None if irreps is None else list(irreps), nirrep,
-                               0, q2, g2, bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1, t2)
+                               0, q2, g2, bool(o["CONSTRAIN_SPIN"]), bool(o["CONSTRAIN_SZ"]), t1, t2,
+                               bool(o["CONSTRAIN_D3"]))
         self._batch = None
         self.energy = None
 

@@ -63,6 +63,8 @@ typedef struct v2h_problem_desc {
                                          T1 blocks after G2, rows t1.cc:419-793 */
     int constrain_t2;                 /* POSITIVITY DQGT2 / DQGT1T2 (v2rdm_solver.cc:4105-4112):
                                          T2 blocks after T1, rows t2.cc:1236-1843 */
+    int constrain_d3;                 /* CONSTRAIN_D3 (v2rdm_solver.cc:4122-4124): D3 blocks after
+                                         T2, rows d3.cc:45-406 */
 } v2h_problem_desc;
 
 typedef struct v2h_status {

@@ -66,7 +66,7 @@ def lib():
         L.orc_hubbard_integrals.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.orc_df_integrals.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _dp, _dp, _dp]
         L.orc_problem_enable_t.restype = C.c_int
-        L.orc_problem_enable_t.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_problem_enable_t.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_get_t_offsets.argtypes = [C.c_void_p, _lp]
         L.orc_get_trip_counts.argtypes = [C.c_void_p, _ip, _ip, _ip]
         L.orc_fcidump_map.argtypes = [C.c_int, C.c_int, C.c_void_p, _ip, _ip]
@@ -181,7 +181,7 @@ class OracleProblem:
     """One DQG problem in the oracle (tables, b, c-packing, maps, solve)."""
 
     def __init__(self, nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=None,
-                 q2=True, g2=True, spin=True, sz=True, t1=False, t2=False):
+                 q2=True, g2=True, spin=True, sz=True, t1=False, t2=False, d3=False):
         L = lib()
         if irreps is None:
             irreps = [0] * nmo
@@ -199,8 +199,8 @@ class OracleProblem:
                                     int(q2), int(g2), int(spin), int(sz))
         if not self._p:
             raise ValueError("invalid oracle problem")
-        self.t1, self.t2 = bool(t1), bool(t2)
-        if (t1 or t2) and L.orc_problem_enable_t(self._p, int(t1), int(t2)) != 0:
+        self.t1, self.t2, self.d3 = bool(t1), bool(t2), bool(d3)
+        if (t1 or t2 or d3) and L.orc_problem_enable_t(self._p, int(t1), int(t2), int(d3)) != 0:
             raise ValueError("T1 and T2 need Q2 and G2")
         self.n_primal = L.orc_n_primal(self._p)
         self.n_dual = L.orc_n_dual(self._p)
@@ -260,9 +260,10 @@ class OracleProblem:
         """{"t1aaa", "t1bbb", "t1aab", "t1bba", "t2aaa", "t2bbb", "t2aab",
         "t2bba"}: per-irrep block offsets."""
         H = self.nirrep
-        off = np.zeros(8 * H, dtype=np.int64)
+        off = np.zeros(12 * H, dtype=np.int64)
         lib().orc_get_t_offsets(self._p, off)
-        names = ("t1aaa", "t1bbb", "t1aab", "t1bba", "t2aaa", "t2bbb", "t2aab", "t2bba")
+        names = ("t1aaa", "t1bbb", "t1aab", "t1bba", "t2aaa", "t2bbb", "t2aab", "t2bba",
+                 "d3aaa", "d3bbb", "d3aab", "d3bba")
         return {k: off[a * H:(a + 1) * H] for a, k in enumerate(names)}
 
     def trip_counts(self, aba=False):

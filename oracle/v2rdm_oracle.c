@@ -213,11 +213,13 @@ static void build_triplets(orc_problem *p) {
     }
 }
 
-int orc_problem_enable_t(orc_problem *p, int t1, int t2) {
-    if (!p->constrain_q2 || !p->constrain_g2 || p->constrain_t1 || p->constrain_t2) return -1;
-    if (!t1 && !t2) return 0;
+int orc_problem_enable_t(orc_problem *p, int t1, int t2, int d3) {
+    if ((t1 || t2) && (!p->constrain_q2 || !p->constrain_g2)) return -1;
+    if (p->constrain_t1 || p->constrain_t2 || p->constrain_d3) return -1;
+    if (!t1 && !t2 && !d3) return 0;
     p->constrain_t1 = t1 != 0;
     p->constrain_t2 = t2 != 0;
+    p->constrain_d3 = d3 != 0;
     build_triplets(p);
     const int H = p->nirrep;
     long off = p->n_primal;
@@ -237,15 +239,28 @@ int orc_problem_enable_t(orc_problem *p, int t1, int t2) {
         for (int h = 0; h < H; h++) PUSHT(p->t2bbaoff, p->trip_aab[h]);
     }
 #undef PUSHT
+    if (d3) { /* v2rdm_solver.cc:4000-4026; rows :3624-3675 */
+        for (int h = 0; h < H; h++) { p->d3aaaoff[h] = off; off += (long)p->trip_aaa[h] * p->trip_aaa[h]; p->dims[nb++] = p->trip_aaa[h]; }
+        for (int h = 0; h < H; h++) { p->d3bbboff[h] = off; off += (long)p->trip_aaa[h] * p->trip_aaa[h]; p->dims[nb++] = p->trip_aaa[h]; }
+        for (int h = 0; h < H; h++) { p->d3aaboff[h] = off; off += (long)p->trip_aab[h] * p->trip_aab[h]; p->dims[nb++] = p->trip_aab[h]; }
+        for (int h = 0; h < H; h++) { p->d3bbaoff[h] = off; off += (long)p->trip_aab[h] * p->trip_aab[h]; p->dims[nb++] = p->trip_aab[h]; }
+        for (int h = 0; h < H; h++) {
+            const long saa = (long)GAA(h) * GAA(h), sab = (long)GAB(h) * GAB(h);
+            p->n_dual += p->constrain_sz ? 4 * saa + 2 * sab : 2 * saa + sab;
+            if (p->constrain_spin && p->nalpha == p->nbeta)
+                p->n_dual += (long)p->trip_aab[h] * p->trip_aab[h] + 2L * p->trip_aaa[h] * p->trip_aaa[h];
+        }
+    }
     p->n_primal = off;
     p->nblocks = nb;
     return 0;
 }
 
 void orc_get_t_offsets(const orc_problem *p, long *off) {
-    const long *arrs[8] = {p->t1aaaoff, p->t1bbboff, p->t1aaboff, p->t1bbaoff,
-                           p->t2aaaoff, p->t2bbboff, p->t2aaboff, p->t2bbaoff};
-    for (int a = 0; a < 8; a++)
+    const long *arrs[12] = {p->t1aaaoff, p->t1bbboff, p->t1aaboff, p->t1bbaoff,
+                            p->t2aaaoff, p->t2bbboff, p->t2aaboff, p->t2bbaoff,
+                            p->d3aaaoff, p->d3bbboff, p->d3aaboff, p->d3bbaoff};
+    for (int a = 0; a < 12; a++)
         for (int h = 0; h < p->nirrep; h++) off[a * p->nirrep + h] = arrs[a][h];
 }
 
@@ -832,6 +847,128 @@ static long t2_rows(const orc_problem *p, long r, t1_term_fn term, void *ctx) {
     return r;
 }
 
+/* ---- D3 rows (d3.cc:45-406) ------------------------------------------------- */
+/* With CONSTRAIN_SZ: D3aaa -> D2aa, D3bbb -> D2bb, D3aab -> D2aa, D3bba ->
+ * D2bb, D3aab -> D2ab, D3bba -> D2ab; without: (D3aaa + D3aab) -> D2aa,
+ * (D3bbb + D3bba) -> D2bb, (D3aab + D3bba) -> D2ab; then for singlets with
+ * CONSTRAIN_SPIN: D3aab = D3bba and D3aaa, D3bbb from D3aab, D3bba. */
+static long d3_rows(const orc_problem *p, long r, t1_term_fn term, void *ctx) {
+    const int H = p->nirrep, n = NMO;
+    const double na = p->nalpha, nb = p->nbeta;
+#define IAAA(hh, a, b, c) (p->ibas_aaa[(hh)][((long)(a) * n + (b)) * n + (c)])
+#define IAAB(hh, a, b, c) (p->ibas_aab[(hh)][((long)(a) * n + (b)) * n + (c)])
+    /* sum over p of the D3 same-spin (aaa) element (ijp, klp), sign of
+     * d3.cc's (i < p < j), (k < p < l) rule */
+#define AAA_TERM(off3, c)                                                                     \
+    do {                                                                                      \
+        if (i != q && j != q && k != q && l != q) {                                           \
+            const int h2 = h ^ SYM(q);                                                        \
+            double sg = (c);                                                                  \
+            if (i < q && q < j) sg = -sg;                                                     \
+            if (k < q && q < l) sg = -sg;                                                     \
+            term(row, (off3)[h2] + (long)IAAA(h2, i, j, q) * p->trip_aaa[h2] + IAAA(h2, k, l, q), sg, ctx); \
+        }                                                                                     \
+    } while (0)
+    const int nsame = p->constrain_sz ? 2 : 1;
+    /* same-spin D2 rows: sz: aaa->aa, bbb->bb, aab->aa, bba->bb; else (aaa+aab)->aa, (bbb+bba)->bb */
+    for (int pass = 0; pass < 2 * nsame; pass++) {
+        const int spin = pass % 2;        /* 0: alpha rows (D2aa), 1: beta rows (D2bb) */
+        const int part = p->constrain_sz ? pass / 2 : -1;  /* 0: from D3aaa/bbb, 1: from D3aab/bba, -1: both */
+        const long *d2o = spin ? p->d2bboff : p->d2aaoff;
+        const long *o3s = spin ? p->d3bbboff : p->d3aaaoff;
+        const long *o3m = spin ? p->d3bbaoff : p->d3aaboff;
+        double coef;
+        if (part == 0) coef = (spin ? nb : na) - 2.0;
+        else if (part == 1) coef = spin ? na : nb;
+        else coef = na + nb - 2.0;
+        for (int h = 0; h < H; h++) {
+            const int g = GAA(h);
+            for (int ij = 0; ij < g; ij++) {
+                const int i = BAA0(h, ij), j = BAA1(h, ij);
+                for (int kl = 0; kl < g; kl++) {
+                    const int k = BAA0(h, kl), l = BAA1(h, kl);
+                    const long row = r + (long)ij * g + kl;
+                    term(row, d2o[h] + (long)ij * g + kl, coef, ctx);
+                    for (int q = 0; q < n; q++) {
+                        const int h2 = h ^ SYM(q);
+                        if (part != 0)
+                            term(row, o3m[h2] + (long)IAAB(h2, i, j, q) * p->trip_aab[h2] + IAAB(h2, k, l, q), -1.0, ctx);
+                        if (part != 1) AAA_TERM(o3s, -1.0);
+                    }
+                }
+            }
+            r += (long)g * g;
+        }
+    }
+    /* D2ab rows: sz: from D3aab, then from D3bba; else one row from both */
+    for (int pass = 0; pass < nsame; pass++) {
+        double coef = p->constrain_sz ? (pass == 0 ? na - 1.0 : nb - 1.0) : na + nb - 2.0;
+        for (int h = 0; h < H; h++) {
+            const int g = GAB(h);
+            for (int ij = 0; ij < g; ij++) {
+                const int i = BAB0(h, ij), j = BAB1(h, ij);
+                for (int kl = 0; kl < g; kl++) {
+                    const int k = BAB0(h, kl), l = BAB1(h, kl);
+                    const long row = r + (long)ij * g + kl;
+                    term(row, p->d2aboff[h] + (long)ij * g + kl, coef, ctx);
+                    for (int q = 0; q < n; q++) {
+                        const int h2 = h ^ SYM(q);
+                        if ((!p->constrain_sz || pass == 0) && i != q && k != q) {
+                            double sg = -1.0;
+                            if (q < i) sg = -sg;
+                            if (q < k) sg = -sg;
+                            term(row, p->d3aaboff[h2] + (long)IAAB(h2, i, q, j) * p->trip_aab[h2] + IAAB(h2, k, q, l), sg, ctx);
+                        }
+                        if ((!p->constrain_sz || pass == 1) && j != q && l != q) {
+                            double sg = -1.0;
+                            if (q < j) sg = -sg;
+                            if (q < l) sg = -sg;
+                            term(row, p->d3bbaoff[h2] + (long)IAAB(h2, j, q, i) * p->trip_aab[h2] + IAAB(h2, l, q, k), sg, ctx);
+                        }
+                    }
+                }
+            }
+            r += (long)g * g;
+        }
+    }
+    if (p->constrain_spin && p->nalpha == p->nbeta) {
+        for (int h = 0; h < H; h++) { /* D3aab - D3bba = 0 */
+            const long t2 = (long)p->trip_aab[h] * p->trip_aab[h];
+            for (long e = 0; e < t2; e++) {
+                term(r + e, p->d3aaboff[h] + e, 1.0, ctx);
+                term(r + e, p->d3bbaoff[h] + e, -1.0, ctx);
+            }
+            r += t2;
+        }
+        for (int s = 0; s < 2; s++) /* D3aaa from D3aab, D3bbb from D3bba */
+            for (int h = 0; h < H; h++) {
+                const int t = p->trip_aaa[h], ta = p->trip_aab[h];
+                const long o3 = s ? p->d3bbboff[h] : p->d3aaaoff[h], om = s ? p->d3bbaoff[h] : p->d3aaboff[h];
+                for (int pqr = 0; pqr < t; pqr++) {
+                    const int a = p->bas_aaa[h][3 * pqr], b = p->bas_aaa[h][3 * pqr + 1], c = p->bas_aaa[h][3 * pqr + 2];
+                    const long rows3[3] = {IAAB(h, a, b, c), IAAB(h, a, c, b), IAAB(h, b, c, a)};
+                    for (int stv = 0; stv < t; stv++) {
+                        const int s1 = p->bas_aaa[h][3 * stv], t1 = p->bas_aaa[h][3 * stv + 1], v1 = p->bas_aaa[h][3 * stv + 2];
+                        const long cols3[3] = {IAAB(h, s1, t1, v1), IAAB(h, s1, v1, t1), IAAB(h, t1, v1, s1)};
+                        const long row = r + (long)pqr * t + stv;
+                        term(row, o3 + (long)pqr * t + stv, 1.0, ctx);
+                        for (int x = 0; x < 3; x++)
+                            for (int y = 0; y < 3; y++) {
+                                /* signs - + - / + - + / - + - of d3.cc:360-371 */
+                                const double sg = ((x + y) % 2 == 0 ? -1.0 : 1.0) / 3.0;
+                                term(row, om + rows3[x] * ta + cols3[y], sg, ctx);
+                            }
+                    }
+                }
+                r += (long)t * t;
+            }
+    }
+#undef AAA_TERM
+#undef IAAB
+#undef IAAA
+    return r;
+}
+
 struct t1_ctx { const double *in; double *out; };
 static void t1_gather(long row, long col, double coef, void *c) {
     struct t1_ctx *x = (struct t1_ctx *)c;
@@ -1195,6 +1332,10 @@ void orc_Au(const orc_problem *p, const double *u, double *A) {
         struct t1_ctx c = {u, A};
         r = t2_rows(p, r, t1_gather, &c);
     }
+    if (p->constrain_d3) {
+        struct t1_ctx c = {u, A};
+        r = d3_rows(p, r, t1_gather, &c);
+    }
 }
 
 /* ---- A^T u -------------------------------------------------------------- */
@@ -1554,6 +1695,10 @@ void orc_ATu(const orc_problem *p, const double *u, double *A) {
     if (p->constrain_t2) {
         struct t1_ctx c = {u, A};
         r = t2_rows(p, r, t1_scatter, &c);
+    }
+    if (p->constrain_d3) {
+        struct t1_ctx c = {u, A};
+        r = d3_rows(p, r, t1_scatter, &c);
     }
 }
 

@@ -51,17 +51,18 @@ typedef struct orc_problem {
     long g2aboff[ORC_MAX_IRREP], g2baoff[ORC_MAX_IRREP], g2aaoff[ORC_MAX_IRREP];
     long n_primal, n_dual;
     int nblocks;
-    int dims[22 * ORC_MAX_IRREP];
+    int dims[26 * ORC_MAX_IRREP];
 
     /* T1 / T2 (POSITIVITY DQGT1, DQGT2, DQGT1T2), set by orc_problem_enable_t:
      * triplet tables (basis.cc:642-744), the T1 blocks after G2
      * (v2rdm_solver.cc:3848-3873) and the T2 blocks after them (:3879-3908) */
-    int constrain_t1, constrain_t2;
+    int constrain_t1, constrain_t2, constrain_d3;
     int trip_aab[ORC_MAX_IRREP], trip_aaa[ORC_MAX_IRREP], trip_aba[ORC_MAX_IRREP];
     int *bas_aab[ORC_MAX_IRREP], *bas_aaa[ORC_MAX_IRREP], *bas_aba[ORC_MAX_IRREP];  /* [trip*3] */
     int *ibas_aab[ORC_MAX_IRREP], *ibas_aaa[ORC_MAX_IRREP];  /* [nmo^3], -999 when absent */
     long t1aaaoff[ORC_MAX_IRREP], t1bbboff[ORC_MAX_IRREP], t1aaboff[ORC_MAX_IRREP], t1bbaoff[ORC_MAX_IRREP];
     long t2aaaoff[ORC_MAX_IRREP], t2bbboff[ORC_MAX_IRREP], t2aaboff[ORC_MAX_IRREP], t2bbaoff[ORC_MAX_IRREP];
+    long d3aaaoff[ORC_MAX_IRREP], d3bbboff[ORC_MAX_IRREP], d3aaboff[ORC_MAX_IRREP], d3bbaoff[ORC_MAX_IRREP];
 } orc_problem;
 
 /* Build tables/offsets.  irreps: [nmo] orbital irreps in Pitzer (non-decreasing)
@@ -70,15 +71,16 @@ int  orc_problem_init(orc_problem *p, int nmo, int nalpha, int nbeta, int multip
                       const int *irreps, int nirrep,
                       int constrain_q2, int constrain_g2, int constrain_spin, int constrain_sz);
 void orc_problem_free(orc_problem *p);
-/* Add the T1 (t1.cc) and / or T2 (t2.cc, the slow path used at
- * v2rdm_solver.cc:3021) conditions to an initialised problem with Q2 and G2:
+/* Add the T1 (t1.cc), T2 (t2.cc, the slow path used at v2rdm_solver.cc:3021)
+ * and D3 (d3.cc, CONSTRAIN_D3) conditions to an initialised problem:
  * triplet tables, the T1aaa, T1bbb, T1aab, T1bba blocks, then the T2aaa,
- * T2bbb (dimension trip_aab + trip_aba), T2aab, T2bba blocks, and their rows
- * (b = 0, v2rdm_solver.cc:2076-2150).  Call once.  Returns 0, or -1 without
- * Q2 and G2. */
-int  orc_problem_enable_t(orc_problem *p, int t1, int t2);
-/* [8][nirrep] offsets of the T1aaa, T1bbb, T1aab, T1bba, T2aaa, T2bbb, T2aab,
- * T2bba blocks */
+ * T2bbb (dimension trip_aab + trip_aba), T2aab, T2bba blocks, then the D3aaa,
+ * D3bbb, D3aab, D3bba blocks, and their rows (b = 0, v2rdm_solver.cc:
+ * 2076-2150, 2269-2380).  Call once.  Returns 0, or -1 when T1 or T2 is
+ * asked for without Q2 and G2. */
+int  orc_problem_enable_t(orc_problem *p, int t1, int t2, int d3);
+/* [12][nirrep] offsets of the T1aaa, T1bbb, T1aab, T1bba, T2aaa, T2bbb,
+ * T2aab, T2bba, D3aaa, D3bbb, D3aab, D3bba blocks */
 void orc_get_t_offsets(const orc_problem *p, long *off);
 void orc_get_trip_counts(const orc_problem *p, int *trip_aab, int *trip_aaa, int *trip_aba);
 

@@ -280,3 +280,33 @@ def pack_t2(P, F, psi, x):
                 for b, q in enumerate(rows):
                     x[off[name][h] + a * d + b] = val(p, q)
     return x
+
+
+def pack_d3(P, F, psi, x, order="nml"):
+    """Fill the D3 blocks of x (oracle layout, P built with d3=True) with
+    the exact 3-RDM D3(ijk,lmn) = <a+_i a+_j a+_k a_n a_m a_l> of psi
+    (order="lmn": <a+_i a+_j a+_k a_l a_m a_n>); spins aaa, bbb, aab
+    (i, j alpha, k beta), bba; triplets as in pack_t1."""
+    import itertools as it
+    n, M = F.n, F.M
+    irr = np.asarray(P.irreps)
+    ann = [F._ann[k] for k in range(M)]
+    A3 = {}
+    for a, b, c in it.product(range(M), repeat=3):
+        A3[a, b, c] = ann[a] @ (ann[b] @ (ann[c] @ psi))
+    off = P.t1_offsets()
+    for h in range(P.nirrep):
+        trips = [(i, j, k) for i in range(n) for j in range(n) for k in range(n) if irr[i] ^ irr[j] ^ irr[k] == h]
+        aab = [t for t in trips if t[0] < t[1]]
+        aaa = [t for t in aab if t[1] < t[2]]
+        for name, tl, sp in (("d3aaa", aaa, (0, 0, 0)), ("d3bbb", aaa, (1, 1, 1)),
+                             ("d3aab", aab, (0, 0, 1)), ("d3bba", aab, (1, 1, 0))):
+            t = len(tl)
+            for a, (i, j, k) in enumerate(tl):
+                I, J, K = i + sp[0] * n, j + sp[1] * n, k + sp[2] * n
+                for b, (l, m, q) in enumerate(tl):
+                    L_, M_, N_ = l + sp[0] * n, m + sp[1] * n, q + sp[2] * n
+                    right = A3[N_, M_, L_] if order == "nml" else A3[L_, M_, N_]
+                    # <a+_i a+_j a+_k X> = (a_k a_j a_i psi) . (X psi)
+                    x[off[name][h] + a * t + b] = A3[K, J, I] @ right
+    return x

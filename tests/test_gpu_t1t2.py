@@ -1,5 +1,5 @@
-"""GPU parity of the T1 and T2 conditions (POSITIVITY DQGT1, DQGT2, DQGT1T2)
-through the C ABI.
+"""GPU parity of the T1, T2 and D3 conditions (POSITIVITY DQGT1, DQGT2,
+DQGT1T2; CONSTRAIN_D3) through the C ABI.
 
 * maps: the device's A u and A^T y with the T1 blocks and rows equal the
   oracle's (t1.cc restated, pinned in test_oracle_t1.py) to 1e-13, for C1,
@@ -11,7 +11,10 @@ through the C ABI.
 * T2 (and T1 with T2): maps to 1e-13 against the oracle's t2.cc
   restatement (pinned in test_oracle_t2.py); a DQGT2 solve equals the
   oracle's within 1e-6 Eh at 1e-7 / 1e-6, and on a C1 instance DQGT2 lies
-  above DQG by more than 1 mEh and at most 1e-5 Eh above FCI.
+  above DQG by more than 1 mEh and at most 1e-5 Eh above FCI;
+* D3: maps to 1e-13 (with and without CONSTRAIN_SZ, singlet spin rows); a
+  DQG+D3 solve equals the oracle's within 1e-6 Eh at 1e-8 / 1e-7, and on
+  (6e,6o) D3 raises the DQG bound and stays below FCI.
 """
 import numpy as np
 import pytest
@@ -128,4 +131,60 @@ def test_t2_solve_matches_oracle_and_fci():
     print("DQGT2", e[0], "oracle", ref["energy_primal"], "C1 DQGT2", e[1], "DQG", e[2], "FCI", e_fci, flush=True)
     assert abs(e[0] - ref["energy_primal"]) < 1e-6
     assert e[2] < e[1] - 1e-3 and e[1] <= e_fci + 1e-5
+    B.close()
+
+
+D3_CASES = [
+    (4, 2, 2, None, 1, True),
+    (5, 3, 2, None, 1, False),
+    (5, 3, 3, [0, 0, 1, 1, 1], 2, True),
+    (6, 3, 3, [0, 0, 1, 2, 3, 3], 4, False),
+]
+
+
+def test_d3_maps_match_oracle():
+    """CONSTRAIN_D3 (with and without CONSTRAIN_SZ, singlet spin rows) maps
+    against the oracle's d3.cc restatement (pinned in test_oracle_d3.py)."""
+    B = Batch([Problem(n, na, nb, irr, H, sz=sz, d3=True) for n, na, nb, irr, H, sz in D3_CASES], options())
+    rng = np.random.default_rng(10)
+    for i, (n, na, nb, irr, H, sz) in enumerate(D3_CASES):
+        P = OracleProblem(n, na, nb, irr if irr is not None else [0] * n, H, sz=sz, d3=True)
+        assert (B.n_primal[i], B.n_dual[i]) == (P.n_primal, P.n_dual)
+        u = rng.standard_normal(P.n_primal)
+        ref = P.Au(u)
+        assert np.abs(B.Au(i, u) - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+        y = rng.standard_normal(P.n_dual)
+        ref = P.ATu(y)
+        assert np.abs(B.ATu(i, y) - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+    B.close()
+
+
+def test_d3_solves():
+    """(6e,4o): the DQG+D3 solve equals the oracle's within 1e-6 Eh at
+    1e-8 / 1e-7; (6e,6o) with 4 irreps: D3 raises the DQG bound by more than
+    0.5 mEh and stays below FCI (default thresholds, band 5e-5 Eh)."""
+    n, na, nb = 4, 3, 3
+    h, g = synthetic_integrals(n, 10, 5)
+    tight = dict(e_convergence=1e-8, r_convergence=1e-7)
+    P = OracleProblem(n, na, nb, d3=True)
+    ref = P.solve(P.pack_c(h, g), maxiter=50000, nthreads=4, **tight)
+    assert ref["converged"]
+    A = Batch([Problem(n, na, nb, d3=True)], options(maxiter=50000, **tight))
+    A.set_integrals(0, h, g)
+    assert A.solve_to_convergence() == 1
+    e_tight = A.status(0)["energy_primal"]
+    A.close()
+    irr6 = [0, 0, 1, 2, 3, 3]
+    h6, g6 = synthetic_integrals(6, 10, 5, irreps=irr6)
+    B = Batch([Problem(6, 3, 3, irr6, 4, d3=True), Problem(6, 3, 3, irr6, 4)], options(maxiter=50000))
+    B.set_integrals(0, h6, g6)
+    B.set_integrals(1, h6, g6)
+    assert B.solve_to_convergence() == 2
+    e = [B.status(i)["energy_primal"] for i in range(2)]
+    F = Fock(6)
+    e_fci, _ = ground_state(F, F.hamiltonian(h6, g6), 3, 3)
+    print("(6e,4o) D3", e_tight, "oracle", ref["energy_primal"], "(6e,6o) D3", e[0], "DQG", e[1], "FCI", e_fci,
+          flush=True)
+    assert abs(e_tight - ref["energy_primal"]) < 1e-6
+    assert e[1] < e[0] - 5e-4 and e[0] <= e_fci + 5e-5
     B.close()

@@ -62,17 +62,19 @@ def test_constraints_match_oracle(case, flags):
 T1_CASES = CASES[:5]
 
 
-@pytest.mark.parametrize("t12", [(True, False), (False, True), (True, True)], ids=["T1", "T2", "T1T2"])
+@pytest.mark.parametrize("t12", [(True, False, False), (False, True, False), (True, True, False), (False, False, True),
+                                 (True, True, True)], ids=["T1", "T2", "T1T2", "D3", "T1T2D3"])
 @pytest.mark.parametrize("spin_sz", [(True, True), (False, True), (True, False)])
 @pytest.mark.parametrize("case", T1_CASES)
-def test_t1_t2_constraints_match_oracle(case, spin_sz, t12):
-    """POSITIVITY DQGT1 / DQGT2 / DQGT1T2: T1 blocks after G2, T2 blocks after
-    them, and the rows of t1.cc and t2.cc's slow path."""
+def test_t1_t2_d3_constraints_match_oracle(case, spin_sz, t12):
+    """POSITIVITY DQGT1 / DQGT2 / DQGT1T2 and CONSTRAIN_D3: T1 blocks after
+    G2, T2 blocks after them, D3 blocks last, and the rows of t1.cc, t2.cc's
+    slow path and d3.cc."""
     n, na, nb, irr, H = case
     spin, sz = spin_sz
-    t1, t2 = t12
-    S = Structure(n, na, nb, irr, H, spin=spin, sz=sz, t1=t1, t2=t2)
-    P = OracleProblem(n, na, nb, irr, H, spin=spin, sz=sz, t1=t1, t2=t2)
+    t1, t2, d3 = t12
+    S = Structure(n, na, nb, irr, H, spin=spin, sz=sz, t1=t1, t2=t2, d3=d3)
+    P = OracleProblem(n, na, nb, irr, H, spin=spin, sz=sz, t1=t1, t2=t2, d3=d3)
     assert (S.n_primal, S.n_dual) == (P.n_primal, P.n_dual)
     assert S.block_dims().tolist() == P.dims.tolist()
     A, b = S.matrix()

@@ -19,7 +19,7 @@ def test_case_insensitive_names_and_values():
 
 
 @pytest.mark.parametrize("opt", [{"POSITIVITY": "DQGT3"}, {"SDP_SOLVER": "RRSDP"},
-                                 {"CONSTRAIN_D3": True}, {"GPC_CONSTRAINTS": "1RDM"}, {"TAU_PARAMETER": 1.5}])
+                                 {"CONSTRAIN_E3": True}, {"GPC_CONSTRAINTS": "1RDM"}, {"TAU_PARAMETER": 1.5}])
 def test_unimplemented_settings_raise(opt):
     with pytest.raises(NotImplementedError):
         resolve_options(opt)
