@@ -173,6 +173,29 @@ def cpu_baseline(item, mean_oiter, seconds, gpu_energy):
     return out
 
 
+def energy_check(item, maxiter, gpu_energy, cpu):
+    """Instance 0 solved again on the GPU at E_CONVERGENCE 1e-9, R_CONVERGENCE
+    1e-8: the SDP optimum to about 1e-8 Eh.  At the bench thresholds (1e-6 /
+    1e-5) a solver stops anywhere in a band around it (CG on the singular
+    A A^T makes the stopping point depend on the last bits), so the GPU's and
+    the oracle's default-threshold energies are both compared with it."""
+    from hilbert_amd.batch import Batch, options
+    prob, h, g = item
+    B = Batch([prob], options(e_convergence=1e-9, r_convergence=1e-8, maxiter=maxiter))
+    B.set_integrals(0, h, g)
+    t0 = time.perf_counter()
+    solve_with_progress(B, "tight reference (instance 0)")
+    dt = time.perf_counter() - t0
+    st = B.status(0)
+    B.close()
+    e_ref = st["energy_primal"]
+    out = {"instance": 0, "thresholds": [1e-9, 1e-8], "converged": bool(st["converged"]), "seconds": dt,
+           "energy_reference": e_ref, "gpu_abs_diff": abs(gpu_energy - e_ref)}
+    if cpu and "energy_oracle" in cpu:
+        out["oracle_abs_diff"] = abs(cpu["energy_oracle"] - e_ref)
+    return out
+
+
 def solve_with_progress(B, label, chunk=1000):
     """B.solve_to_convergence() with a progress line on stderr per chunk of
     outer iterations."""
@@ -342,6 +365,8 @@ def main():
                     help="also solve the first K instances one at a time, each alone on the GPU (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=60.0,
                     help="CPU oracle budget: a full solve of instance 0 if it fits, else a sample (0 = skip)")
+    ap.add_argument("--energy-check", type=int, default=1,
+                    help="solve instance 0 again at 1e-9 / 1e-8 and compare the energies with it (0 = skip)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="after the solve, three plain copies of this many MiB (k_copy), which calibrate "
                          "the traffic counters of a profile (0 = none)")
@@ -431,6 +456,9 @@ def main():
     if args.cpu_seconds > 0:
         print(f"[bench] CPU oracle on instance 0 (budget {args.cpu_seconds:.0f} s)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(items[0], mean_oiter, args.cpu_seconds, energy0)
+    echeck = None
+    if args.energy_check:
+        echeck = energy_check(items[0], 200000, energy0, cpu)
 
     out = {
         "metric": "converged v2RDM SDPs/s (ΔE≤1e-6 Eh), batched vs single GPU vs CPU, (16e,16o)",
@@ -449,6 +477,7 @@ def main():
         "batched_over_single": (value / single["value"]) if single else None,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        "energy_check": echeck,
         "roofline": roof,
         "roofline_cg_a": roof_cg,
         "kernel_ms": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
