@@ -170,6 +170,17 @@ def cpu_baseline(item, mean_oiter, seconds, gpu_energy):
                    sample=f"oracle BPSDP on instance 0 for its first {r['oiter']} outer iterations "
                           f"({r['iiter']} CG steps, {dt:.1f} s, {cores} OpenMP threads); SDPs/s = 1 / "
                           f"(s per outer iteration x {mean_oiter:.0f} outer iterations per SDP)")
+    # one thread, as the reference runs (its logs are single-threaded): the
+    # first outer iterations of the same solve, scaled by the full count
+    its = out.get("oiter", mean_oiter)
+    k = max(5, min(40, int(10.0 / max(per_it * cores, 1e-3))))
+    t0 = time.time()
+    r1 = P.solve(c, maxiter=k, nthreads=1, e_convergence=E_CONV, r_convergence=R_CONV)
+    per1 = (time.time() - t0) / max(r1["oiter"], 1)
+    out["one_thread"] = {"value": 1.0 / (per1 * its), "unit": "SDPs/s", "cores": 1,
+                         "seconds_per_outer_iteration": per1,
+                         "sample": f"first {r1['oiter']} outer iterations of instance 0 on one thread, "
+                                   f"scaled by {its:.0f} outer iterations per SDP"}
     return out
 
 

@@ -83,6 +83,9 @@ void Batch::write_fcidump(int i, const FcidumpArgs& a, int64_t* n_int, int64_t* 
 
     // two-electron integrals: G = Qmo^T Qmo on the device, one copy back
     const int64_t ntri = (int64_t)nfull * (nfull + 1) / 2;
+    if ((double)ntri * (double)ntri * 8.0 > 64.0 * (1 << 30))
+        throw std::invalid_argument("write_fcidump: the pair Gram matrix of " + std::to_string(nfull) +
+                                    " orbitals exceeds 64 GiB (ntri^2 doubles, on the device and the host)");
     std::vector<double> G((size_t)(ntri * ntri));
     {
         DBuf<double> dq, dg;

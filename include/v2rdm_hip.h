@@ -144,7 +144,9 @@ int v2h_batch_set_df_integrals(v2h_batch *b, int inst, int nfull, const double *
  * full_irreps may be NULL (C1); core may be NULL when ncore = 0; active[u] is
  * the full-space index of the instance's orbital u.  The reference writes
  * after transforming to natural orbitals; pass integrals and x in the basis
- * wanted.  n_int / n_rdm (may be NULL) receive the record counts. */
+ * wanted.  n_int / n_rdm (may be NULL) receive the record counts.  The pair
+ * Gram matrix, (nfull (nfull + 1) / 2)^2 doubles, is formed on the device and
+ * copied to the host once; beyond 64 GiB the call is refused. */
 int v2h_batch_write_fcidump(v2h_batch *b, int inst, int nfull, const int *full_irreps, const double *h_full, int nq,
                             const double *qmo, int ncore, const int *core, const int *active, double enuc,
                             const char *int_path, const char *rdm_path, int64_t *n_int, int64_t *n_rdm);
