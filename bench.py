@@ -146,7 +146,8 @@ def cpu_baseline(item, mean_oiter, seconds, gpu_energy):
     energy of the same instance."""
     from oracle.pyoracle import OracleProblem
     prob, h, g = item
-    P = OracleProblem(prob.nmo, prob.nalpha, prob.nbeta, prob.irreps, prob.nirrep)
+    P = OracleProblem(prob.nmo, prob.nalpha, prob.nbeta, prob.irreps, prob.nirrep, t1=prob.t1, t2=prob.t2,
+                      d3=prob.d3)
     c = P.pack_c(h, g)
     cores = min(os.cpu_count() or 1, 16)
     t0 = time.time()
@@ -207,9 +208,13 @@ def energy_check(item, maxiter, gpu_energy, cpu):
     return out
 
 
-def solve_with_progress(B, label, chunk=1000):
+PROGRESS_CHUNK = [1000]
+
+
+def solve_with_progress(B, label, chunk=None):
     """B.solve_to_convergence() with a progress line on stderr per chunk of
-    outer iterations."""
+    outer iterations (chunked solves give the same bits as one call)."""
+    chunk = chunk or PROGRESS_CHUNK[0]
     t0 = time.perf_counter()
     while True:
         B.solve(chunk)
@@ -381,10 +386,17 @@ def main():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="after the solve, three plain copies of this many MiB (k_copy), which calibrate "
                          "the traffic counters of a profile (0 = none)")
+    ap.add_argument("--nmo", type=int, default=16,
+                    help="sdp workload: orbitals (= electrons) of each instance, C2v labels at random")
+    ap.add_argument("--positivity", choices=("DQG", "DQGT1", "DQGT2", "DQGT1T2"), default="DQG",
+                    help="sdp workload: the N-representability conditions")
+    ap.add_argument("--d3", type=int, default=0, help="sdp workload: add the D3 condition (CONSTRAIN_D3)")
+    ap.add_argument("--chunk", type=int, default=1000, help="outer iterations between progress lines")
     ap.add_argument("--workload", choices=("sdp", "df", "fcidump"), default="sdp",
                     help="sdp: the batched (16e,16o) solve; df: the density-fitted integral intake; "
                          "fcidump: the FCIDUMP writer")
     args = ap.parse_args()
+    PROGRESS_CHUNK[0] = max(1, args.chunk)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
               "communication, and multi-GPU runs are not part of this build.", file=sys.stderr)
@@ -408,7 +420,11 @@ def main():
     if args.steps is None:
         args.steps = 1
 
-    items = configs.c2v_16(args.batch)
+    items = configs.c2v(args.nmo, args.batch)
+    t1 = args.positivity in ("DQGT1", "DQGT1T2")
+    t2 = args.positivity in ("DQGT2", "DQGT1T2")
+    for prob, _, _ in items:
+        prob.t1, prob.t2, prob.d3 = t1, t2, bool(args.d3)
     B = make_batch(items, args.maxiter)
     B.set_profiling(True)
     for w in range(args.warmup):
@@ -433,7 +449,9 @@ def main():
     value = sum(conv) / total
     mean_oiter = sum(sum(o) for o in oiters) / max(1, sum(len(o) for o in oiters))
     st = [B.status(i) for i in range(B.count)]
-    workload = f"c2v16x{args.batch}"
+    n = args.nmo
+    cond = args.positivity + ("+D3" if args.d3 else "")
+    workload = f"c2v{n}x{args.batch}" + ("" if cond == "DQG" else "_" + cond.lower().replace("+", ""))
     dom, roof = roofline(B, kt, counts, workload)
     # the HBM-bound gather kernel of the CG step, whichever kernel dominates
     roof_cg = roofline(B, kt, counts, workload, "cg_a")[1] if dom != "cg_a" else None
@@ -472,15 +490,15 @@ def main():
         echeck = energy_check(items[0], 200000, energy0, cpu)
 
     out = {
-        "metric": "converged v2RDM SDPs/s (ΔE≤1e-6 Eh), batched vs single GPU vs CPU, (16e,16o)",
+        "metric": f"converged v2RDM SDPs/s (ΔE≤1e-6 Eh), batched vs single GPU vs CPU, ({n}e,{n}o)",
         "value": value, "unit": "SDPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.batch} x (16e,16o) DQG SDPs, C2v labels at random, naux 48, "
+        "config": {"workload": f"{args.batch} x ({n}e,{n}o) {cond} SDPs, C2v labels at random, naux {3 * n}, "
                                "one batched solve to convergence per step",
-                   "batch": args.batch, "nmo": 16, "electrons": 16, "nirrep": 4,
-                   "e_convergence": E_CONV, "r_convergence": R_CONV, "positivity": "DQG",
-                   "constrain_spin": True, "constrain_sz": True},
+                   "batch": args.batch, "nmo": n, "electrons": n, "nirrep": 4,
+                   "e_convergence": E_CONV, "r_convergence": R_CONV, "positivity": args.positivity,
+                   "constrain_d3": bool(args.d3), "constrain_spin": True, "constrain_sz": True},
         "converged": sum(conv), "attempted": args.batch * args.steps,
         "mean_outer_iterations": mean_oiter,
         "mean_cg_per_outer": sum(s["iiter"] for s in st) / max(1, sum(s["oiter"] for s in st)),

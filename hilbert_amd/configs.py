@@ -41,8 +41,13 @@ def c1_10(seed=SEED + 1):
     return [synthetic(10, 30, 1, seed)]
 
 
+def c2v(n, count=1, seed=SEED + 3):
+    """(ne,no) with n electrons in n orbitals, C2v labels at random, naux 3 n."""
+    return [synthetic(n, 3 * n, 4, seed + k) for k in range(count)]
+
+
 def c2v_16(count=1, seed=SEED + 3):
-    return [synthetic(16, 48, 4, seed + k) for k in range(count)]
+    return c2v(16, count, seed)
 
 
 def batch64_d2h(count=64, seed=SEED + 2):
