@@ -34,6 +34,16 @@ class ProblemDesc(C.Structure):
                 ("constrain_t2", C.c_int), ("constrain_d3", C.c_int)]
 
 
+class OrbOptOptions(C.Structure):
+    _fields_ = [("g_convergence", C.c_double), ("e_convergence", C.c_double), ("maxiter", C.c_int),
+                ("active_active", C.c_int)]
+
+
+class OrbOptResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("nrot", C.c_int), ("gnorm", C.c_double),
+                ("dE", C.c_double), ("energy", C.c_double), ("efzc", C.c_double)]
+
+
 class Status(C.Structure):
     _fields_ = [("mu", C.c_double), ("energy_primal", C.c_double), ("energy_dual", C.c_double),
                 ("primal_error", C.c_double), ("dual_error", C.c_double),
@@ -68,6 +78,10 @@ SIGNATURES = {
     "v2h_batch_set_df_integrals": (C.c_int, [_vp, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, _ip, _dp]),
     "v2h_batch_write_fcidump": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, _dp, C.c_int, _ip, _ip,
                                           C.c_double, C.c_char_p, C.c_char_p, _vp, _vp]),
+    "v2h_batch_optimize_orbitals": (C.c_int, [_vp, C.c_int, _vp, C.POINTER(OrbOptOptions), C.POINTER(OrbOptResult)]),
+    "v2h_batch_get_df_orbitals": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp]),
+    "v2h_batch_orbital_gradient": (C.c_int, [_vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp, C.c_int,
+                                             C.POINTER(C.c_int)]),
     "v2h_batch_set_cost": (C.c_int, [_vp, C.c_int, _dp]),
     "v2h_batch_set_guess": (C.c_int, [_vp, C.c_int, _vp]),
     "v2h_batch_set_guess_hf": (C.c_int, [_vp, C.c_int]),

@@ -55,6 +55,7 @@ class Batch:
         h = C.c_void_p()
         N.check(N.lib().v2h_batch_create(descs, len(problems), C.byref(self.opts), int(device), C.byref(h)))
         self._h = h
+        self._df = {}
         self.count = len(problems)
         self.n_primal, self.n_dual, self.n_blocks = [], [], []
         for i in range(self.count):
@@ -85,6 +86,7 @@ class Batch:
 
     def set_df_integrals(self, i, h_full, qmo, core, active) -> float:
         """Density-fitted Hamiltonian contracted on the device; returns E_fzc.
+        The integrals stay resident for optimize_orbitals.
 
         h_full: (nfull, nfull); qmo: (nq, nfull (nfull + 1) / 2), the packed
         lower-triangle (Q|mn) layout of the reference's Qmo_; core: full-space
@@ -102,7 +104,105 @@ class Batch:
         e = np.zeros(1)
         N.check(N.lib().v2h_batch_set_df_integrals(self._h, i, nfull, h_full.reshape(-1), qmo.shape[0],
                                                    qmo.reshape(-1), len(core), core, active, e))
+        self._df[i] = (nfull, qmo.shape[0])
         return float(e[0])
+
+    def optimize_orbitals(self, i, full_irreps=None, g_convergence=1e-4, e_convergence=1e-8, maxiter=10,
+                          active_active=False) -> dict:
+        """One orbital-optimisation call (RotateOrbitals with the DF optimiser)
+        on instance i's resident DF integrals and current primal vector.
+
+        Rotates the integrals on the device, repacks c and E_fzc and marks the
+        instance unconverged.  Returns dict(iterations, converged, nrot,
+        gnorm, dE, energy = c.x + E_fzc, efzc)."""
+        if i not in self._df:
+            raise ValueError("instance %d has no density-fitted integrals (set_df_integrals)" % i)
+        irr = None
+        if full_irreps is not None:
+            irr = np.ascontiguousarray(full_irreps, dtype=np.int32).reshape(-1)
+            if len(irr) != self._df[i][0]:
+                raise ValueError("full_irreps must have one label per orbital")
+        opt = N.OrbOptOptions(float(g_convergence), float(e_convergence), int(maxiter), int(bool(active_active)))
+        res = N.OrbOptResult()
+        N.check(N.lib().v2h_batch_optimize_orbitals(self._h, i, None if irr is None else irr.ctypes.data_as(C.c_void_p),
+                                                    C.byref(opt), C.byref(res)))
+        return {k: getattr(res, k) for k, _ in N.OrbOptResult._fields_}
+
+    def orbital_gradient(self, i, full_irreps=None, active_active=False):
+        """(pairs (nrot, 2), gradient, exact diagonal Hessian) at the current
+        orbitals and primal vector; nothing is rotated."""
+        if i not in self._df:
+            raise ValueError("instance %d has no density-fitted integrals (set_df_integrals)" % i)
+        irr = None if full_irreps is None else np.ascontiguousarray(full_irreps, dtype=np.int32).reshape(-1)
+        ip = None if irr is None else irr.ctypes.data_as(C.c_void_p)
+        nrot = C.c_int(0)
+        N.check(N.lib().v2h_batch_orbital_gradient(self._h, i, ip, int(bool(active_active)), None, None, None, 0,
+                                                   C.byref(nrot)))
+        m = nrot.value
+        pairs = np.zeros((max(m, 1), 2), dtype=np.int32)
+        g = np.zeros(max(m, 1))
+        H = np.zeros(max(m, 1))
+        N.check(N.lib().v2h_batch_orbital_gradient(self._h, i, ip, int(bool(active_active)),
+                                                   pairs.ctypes.data_as(C.c_void_p), g.ctypes.data_as(C.c_void_p),
+                                                   H.ctypes.data_as(C.c_void_p), m, C.byref(nrot)))
+        return pairs[:m], g[:m], H[:m]
+
+    def df_orbitals(self, i):
+        """(U, h_full, qmo) of instance i after its orbital steps: new orbital r
+        = sum_s old_s U[s, r] relative to the orbitals of set_df_integrals."""
+        if i not in self._df:
+            raise ValueError("instance %d has no density-fitted integrals (set_df_integrals)" % i)
+        nfull, nq = self._df[i]
+        U = np.zeros((nfull, nfull))
+        h = np.zeros((nfull, nfull))
+        q = np.zeros((nq, nfull * (nfull + 1) // 2))
+        N.check(N.lib().v2h_batch_get_df_orbitals(self._h, i, U.ctypes.data_as(C.c_void_p),
+                                                  h.ctypes.data_as(C.c_void_p), q.ctypes.data_as(C.c_void_p)))
+        return U, h, q
+
+    def solve_casscf(self, orbopt_frequency=500, full_irreps=None, monitor=None, **orbopt) -> list:
+        """v2RDM-CASSCF: alternate orbopt_frequency BPSDP iterations with one
+        optimize_orbitals call per instance until the SDP and the orbitals
+        are converged (the do { solve; RotateOrbitals } while loop of
+        v2rdm_solver.cc:1296-1361; ORBOPT_FREQUENCY default 500).
+
+        full_irreps: None, one array for every instance, or a list of
+        per-instance arrays.  Returns per instance dict(energy = c.x + E_fzc
+        after the last orbital step, sdp_converged, orbopt_converged,
+        macro = orbital calls, last = the last call's result)."""
+        n = self.count
+        if full_irreps is None or (len(full_irreps) and np.ndim(full_irreps[0]) == 0):
+            irr = [full_irreps] * n
+        else:
+            irr = list(full_irreps)
+        oo_conv = [False] * n
+        sdp_conv = [False] * n
+        done = [False] * n
+        last = [None] * n
+        macro = [0] * n
+        while not all(done):
+            self.solve(orbopt_frequency, monitor)
+            for i in range(n):
+                if done[i]:
+                    continue
+                st = self.status(i)
+                if st["status_bits"] or st["oiter"] >= self.opts.maxiter:
+                    done[i] = True
+                    continue
+                last[i] = self.optimize_orbitals(i, irr[i], **orbopt)
+                macro[i] += 1
+                if last[i]["converged"]:
+                    oo_conv[i] = True
+                if last[i]["converged"] and st["converged"]:
+                    # a converged orbital call leaves the SDP marked converged
+                    done[i] = sdp_conv[i] = True
+        out = []
+        for i in range(n):
+            st = self.status(i)
+            out.append(dict(energy=None if last[i] is None else last[i]["energy"],
+                            sdp_converged=sdp_conv[i], orbopt_converged=oo_conv[i],
+                            macro=macro[i], oiter=st["oiter"], last=last[i]))
+        return out
 
     def write_fcidump(self, i, int_path, rdm_path, h_full, qmo, core, active, enuc=0.0, full_irreps=None):
         """hilbert's FCIDUMP files (integrals, RDMs of the current x) for
@@ -180,7 +280,7 @@ class Batch:
 
     KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project_jacobi16", "project_tri64", "project_tri128",
                     "project_tri192", "project_tri224", "project_tri256", "project_jacobi_big", "df_intake",
-                    "pair_gram")
+                    "pair_gram", "orbopt")
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))

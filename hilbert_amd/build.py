@@ -15,10 +15,10 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libv2rdm_hip.so")
 SOURCES = ["layout.cpp", "constraints.cpp", "sell.cpp", "rdm.cpp", "projector.cpp", "solver.cpp", "sdp_host.cpp",
-           "fcidump.cpp", "abi.cpp",
+           "fcidump.cpp", "orbopt_driver.cpp", "abi.cpp",
            "kernels.hip",
            "eig_tri.hip",
-           "df.hip"]
+           "df.hip", "orbopt.hip"]
 ARCH = os.environ.get("V2H_OFFLOAD_ARCH", "gfx950")
 
 

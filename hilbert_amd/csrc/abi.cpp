@@ -255,6 +255,51 @@ int v2h_batch_write_fcidump(v2h_batch* b, int inst, int nfull, const int* full_i
     });
 }
 
+int v2h_batch_optimize_orbitals(v2h_batch* b, int inst, const int* full_irreps, const v2h_orbopt_options* opt,
+                                v2h_orbopt_result* res) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        v2h::OrbOptArgs a;
+        a.full_irreps = full_irreps;
+        if (opt) {
+            a.g_convergence = opt->g_convergence;
+            a.e_convergence = opt->e_convergence;
+            a.maxiter = opt->maxiter;
+            a.active_active = opt->active_active;
+        }
+        v2h::OrbOptResult r;
+        B(b).optimize_orbitals(inst, a, &r);
+        if (res) {
+            res->iterations = r.iterations;
+            res->converged = r.converged;
+            res->nrot = r.nrot;
+            res->gnorm = r.gnorm;
+            res->dE = r.dE;
+            res->energy = r.energy;
+            res->efzc = r.efzc;
+        }
+    });
+}
+
+int v2h_batch_orbital_gradient(v2h_batch* b, int inst, const int* full_irreps, int active_active, int32_t* pairs,
+                               double* g, double* H, int cap, int* nrot) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        v2h::OrbOptArgs a;
+        a.full_irreps = full_irreps;
+        a.active_active = active_active;
+        const int n = B(b).orbital_gradient(inst, a, pairs, g, H, cap);
+        if (nrot) *nrot = n;
+    });
+}
+
+int v2h_batch_get_df_orbitals(v2h_batch* b, int inst, double* U, double* h_full, double* qmo) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        B(b).get_df_orbitals(inst, U, h_full, qmo);
+    });
+}
+
 int v2h_batch_set_cost(v2h_batch* b, int inst, const double* c) {
     return guarded([&] {
         check_inst(B(b), inst);

@@ -1,0 +1,449 @@
+// orbopt.hip -- device kernels of the v2RDM-CASSCF orbital step.
+//
+// The step (orbopt_driver.cpp) keeps the density-fitted integrals of the full
+// orbital space resident as dense slices B^Q (nq x nfull x nfull).  Every
+// contraction is a product of row-major matrices (k_dgemm):
+//   gO   (rs|tu), r..u in O          = BO^T BO             (O: core + active)
+//   Z    Z[x r][Q] = sum_st d[x r s t] BO[Q][s t]
+//   F    F[x][y] = sum_(r,Q) Z[x r][Q] BFO[y][r][Q] + sum_r D[x][r] h[y][occ r]
+//   Jd   (pp|tu)                     = Bdiag BO
+//   Kd   (ps|pu)                     = BFO[p] BFO[p]^T, batched over p
+//   U    exp(K): Taylor terms and squarings
+//   B^Q  U^T B^Q U, batched over Q; h the same
+// and the per-rotation gradient and exact diagonal Hessian read those
+// intermediates (k_oo_grad_hess).  The formulas restate the focas optimiser
+// (focas/orbopt_gradient.cc, orbopt_hessian.cc) for every rotation class at
+// once through the generalized Fock matrix and the second derivative of the
+// energy along one rotation.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "orbopt.hpp"
+
+namespace v2h {
+
+namespace {
+
+constexpr int kT = 64;    // output tile
+constexpr int kKc = 16;   // K chunk staged in LDS
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, double alpha, const double* __restrict__ A,
+                                               int64_t lda, int64_t sA, const double* __restrict__ B, int64_t ldb,
+                                               int64_t sB, double beta, double* __restrict__ C, int64_t ldc,
+                                               int64_t sC) {
+    __shared__ double As[kKc][kT + 1];
+    __shared__ double Bs[kKc][kT + 1];
+    const int64_t bz = blockIdx.z;
+    A += bz * sA;
+    B += bz * sB;
+    C += bz * sC;
+    const int m0 = blockIdx.y * kT, n0 = blockIdx.x * kT;
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int k0 = 0; k0 < K; k0 += kKc) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int idx = t + 256 * e;
+            int m, k;
+            if (!TA) { m = idx >> 4; k = idx & 15; } else { k = idx >> 6; m = idx & 63; }
+            double v = 0.0;
+            if (m0 + m < M && k0 + k < K)
+                v = TA ? A[(int64_t)(k0 + k) * lda + (m0 + m)] : A[(int64_t)(m0 + m) * lda + (k0 + k)];
+            As[k][m] = v;
+            int n, k2;
+            if (!TB) { k2 = idx >> 6; n = idx & 63; } else { n = idx >> 4; k2 = idx & 15; }
+            double w = 0.0;
+            if (n0 + n < N && k0 + k2 < K)
+                w = TB ? B[(int64_t)(n0 + n) * ldb + (k0 + k2)] : B[(int64_t)(k0 + k2) * ldb + (n0 + n)];
+            Bs[k2][n] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = __fma_rn(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int m = m0 + ty + 16 * i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int n = n0 + tx + 16 * j;
+            if (n >= N) continue;
+            double* c = C + (int64_t)m * ldc + n;
+            *c = beta != 0.0 ? __fma_rn(alpha, acc[i][j], beta * *c) : alpha * acc[i][j];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_df_unpack(const double* __restrict__ qmo, int nq, int nfull,
+                                                   double* __restrict__ bd) {
+    const int64_t nn = (int64_t)nfull * nfull, total = nn * nq;
+    const int64_t ntri = (int64_t)nfull * (nfull + 1) / 2;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t Q = e / nn, pq = e - Q * nn;
+        const int p = (int)(pq / nfull), q = (int)(pq - (int64_t)p * nfull);
+        const int64_t a = p >= q ? (int64_t)p * (p + 1) / 2 + q : (int64_t)q * (q + 1) / 2 + p;
+        bd[e] = qmo[Q * ntri + a];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_df_pack(const double* __restrict__ bd, int nq, int nfull,
+                                                 double* __restrict__ qmo) {
+    const int64_t nn = (int64_t)nfull * nfull, total = nn * nq;
+    const int64_t ntri = (int64_t)nfull * (nfull + 1) / 2;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t Q = e / nn, pq = e - Q * nn;
+        const int p = (int)(pq / nfull), q = (int)(pq - (int64_t)p * nfull);
+        if (p < q) continue;
+        qmo[Q * ntri + (int64_t)p * (p + 1) / 2 + q] = bd[e];
+    }
+}
+
+// spin-free D of two active (Pitzer) orbitals
+__device__ double oo_d1(const OoTables& t, const double* x, int i, int j) {
+    const int64_t o = t.d1[i * t.n + j];
+    return o < 0 ? 0.0 : x[o] + x[t.d1[t.n * t.n + i * t.n + j]];
+}
+
+__device__ double oo_ab(const OoTables& t, const double* x, int i, int j, int k, int l) {
+    const int4 a = t.pair[i * t.n + j], b = t.pair[k * t.n + l];
+    if (a.x != b.x) return 0.0;
+    return x[t.offab[a.x] + (int64_t)a.y * t.gab[a.x] + b.y];
+}
+
+__device__ double oo_aa(const OoTables& t, const double* x, int i, int j, int k, int l) {
+    const int4 a = t.pair[i * t.n + j], b = t.pair[k * t.n + l];
+    if (a.z < 0 || b.z < 0 || a.x != b.x) return 0.0;
+    const int64_t e = (int64_t)a.z * t.gaa[a.x] + b.z;
+    return (double)(a.w * b.w) * (x[t.offaa[a.x] + e] + x[t.offbb[a.x] + e]);
+}
+
+// d_pqrs = sum <a+_p a+_r a_s a_q> over O positions (core first, then active)
+__device__ double oo_raw_d(const OoTables& t, const double* x, int p, int q, int r, int s) {
+    const int nc = t.ncore;
+    const bool cp = p < nc, cq = q < nc, cr = r < nc, cs = s < nc;
+    const int ncnt = (int)cp + (int)cq + (int)cr + (int)cs;
+    if (ncnt == 0) {
+        p -= nc; q -= nc; r -= nc; s -= nc;
+        return oo_ab(t, x, p, r, q, s) + oo_ab(t, x, r, p, s, q) + oo_aa(t, x, p, r, q, s);
+    }
+    if (ncnt == 4) return 4.0 * (p == q && r == s) - 2.0 * (p == s && q == r);
+    if (ncnt != 2) return 0.0;
+    if (cp && cq) return p == q ? 2.0 * oo_d1(t, x, r - nc, s - nc) : 0.0;
+    if (cr && cs) return r == s ? 2.0 * oo_d1(t, x, p - nc, q - nc) : 0.0;
+    if (cp && cs) return p == s ? -oo_d1(t, x, q - nc, r - nc) : 0.0;
+    if (cq && cr) return q == r ? -oo_d1(t, x, p - nc, s - nc) : 0.0;
+    return 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_oo_density(OoTables t, const double* __restrict__ x, double* __restrict__ D,
+                                                    double* __restrict__ d) {
+    const int no = t.no;
+    const int64_t n2 = (int64_t)no * no, n4 = n2 * n2;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n4 + n2; e += (int64_t)gridDim.x * blockDim.x) {
+        if (e >= n4) {
+            const int64_t f = e - n4;
+            const int a = (int)(f / no), b = (int)(f - (int64_t)a * no);
+            double v;
+            if (a < t.ncore || b < t.ncore) v = (a == b) ? 2.0 : 0.0;
+            else v = oo_d1(t, x, a - t.ncore, b - t.ncore);
+            if ((a < t.ncore) != (b < t.ncore)) v = 0.0;
+            D[f] = v;
+            continue;
+        }
+        const int p = (int)(e / (n2 * no));
+        const int q = (int)((e / n2) % no);
+        const int r = (int)((e / no) % no);
+        const int s = (int)(e % no);
+        d[e] = 0.25 * (oo_raw_d(t, x, p, q, r, s) + oo_raw_d(t, x, q, p, r, s) + oo_raw_d(t, x, p, q, s, r) +
+                       oo_raw_d(t, x, q, p, s, r));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_oo_gather(const double* __restrict__ bd, int nq, int nfull,
+                                                   const int* __restrict__ occ, int no, double* __restrict__ BO,
+                                                   double* __restrict__ BFO, double* __restrict__ Bdiag) {
+    const int64_t nn = (int64_t)nfull * nfull;
+    const int64_t n1 = (int64_t)nq * no * no, n2 = (int64_t)nfull * no * nq, n3 = (int64_t)nfull * nq;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n1 + n2 + n3; e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < n1) {
+            const int64_t Q = e / ((int64_t)no * no);
+            const int rs = (int)(e - Q * no * no), r = rs / no, s = rs - r * no;
+            BO[e] = bd[Q * nn + (int64_t)occ[r] * nfull + occ[s]];
+        } else if (e < n1 + n2) {
+            const int64_t f = e - n1;
+            const int64_t p = f / ((int64_t)no * nq);
+            const int64_t rQ = f - p * no * nq;
+            const int r = (int)(rQ / nq), Q = (int)(rQ - (int64_t)r * nq);
+            BFO[f] = bd[(int64_t)Q * nn + p * nfull + occ[r]];
+        } else {
+            const int64_t f = e - n1 - n2;
+            const int64_t p = f / nq, Q = f - p * nq;
+            Bdiag[f] = bd[Q * nn + p * nfull + p];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_oo_fock1(const double* __restrict__ D, const double* __restrict__ h,
+                                                  const int* __restrict__ occ, int no, int nfull, double* __restrict__ F) {
+    const int64_t total = (int64_t)no * nfull;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int xo = (int)(e / nfull), y = (int)(e - (int64_t)xo * nfull);
+        double acc = 0.0;
+        for (int r = 0; r < no; r++) acc = __fma_rn(D[xo * no + r], h[(int64_t)y * nfull + occ[r]], acc);
+        F[e] += acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_oo_grad_hess(OoGradArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nrot) return;
+    const int p = a.pairs[k].x, q = a.pairs[k].y;
+    const int no = a.no, nf = a.nfull;
+    const int64_t n2 = (int64_t)no * no;
+    const int ip = a.pos[p], iq = a.pos[q];
+    const double Fpq = ip >= 0 ? a.F[(int64_t)ip * nf + q] : 0.0;
+    const double Fqp = iq >= 0 ? a.F[(int64_t)iq * nf + p] : 0.0;
+    a.g[k] = 2.0 * (Fqp - Fpq);
+    const double Fpp = ip >= 0 ? a.F[(int64_t)ip * nf + p] : 0.0;
+    const double Fqq = iq >= 0 ? a.F[(int64_t)iq * nf + q] : 0.0;
+    const double Dpp = ip >= 0 ? a.D[ip * no + ip] : 0.0;
+    const double Dqq = iq >= 0 ? a.D[iq * no + iq] : 0.0;
+    const double Dpq = (ip >= 0 && iq >= 0) ? a.D[ip * no + iq] : 0.0;
+    const double hpp = a.h[(int64_t)p * nf + p], hqq = a.h[(int64_t)q * nf + q], hpq = a.h[(int64_t)p * nf + q];
+    double val = -2.0 * (Fpp + Fqq) + 2.0 * (Dpp * hqq + Dqq * hpp - 2.0 * Dpq * hpq);
+    double g12 = 0.0, g13 = 0.0;
+    const double* Jp = a.Jd + (int64_t)p * n2;
+    const double* Jq = a.Jd + (int64_t)q * n2;
+    const double* Kp = a.Kd + (int64_t)p * n2;
+    const double* Kq = a.Kd + (int64_t)q * n2;
+    if (ip >= 0) {
+        const double* dpp = a.d + ((int64_t)ip * no + ip) * n2;
+        for (int64_t tu = 0; tu < n2; tu++) g12 = __fma_rn(dpp[tu], Jq[tu], g12);
+        for (int s = 0; s < no; s++)
+            for (int u = 0; u < no; u++)
+                g13 = __fma_rn(a.d[(((int64_t)ip * no + s) * no + ip) * no + u], Kq[s * no + u], g13);
+    }
+    if (iq >= 0) {
+        const double* dqq = a.d + ((int64_t)iq * no + iq) * n2;
+        for (int64_t tu = 0; tu < n2; tu++) g12 = __fma_rn(dqq[tu], Jp[tu], g12);
+        for (int s = 0; s < no; s++)
+            for (int u = 0; u < no; u++)
+                g13 = __fma_rn(a.d[(((int64_t)iq * no + s) * no + iq) * no + u], Kp[s * no + u], g13);
+    }
+    if (ip >= 0 && iq >= 0) {
+        const int64_t row = ((int64_t)ip * no + iq) * n2;
+        double c12 = 0.0, c13 = 0.0;
+        for (int64_t tu = 0; tu < n2; tu++) c12 = __fma_rn(a.d[row + tu], a.gO[row + tu], c12);
+        for (int s = 0; s < no; s++)
+            for (int u = 0; u < no; u++)
+                c13 = __fma_rn(a.d[(((int64_t)ip * no + s) * no + iq) * no + u],
+                               a.gO[(((int64_t)iq * no + s) * no + ip) * no + u], c13);
+        g12 -= 2.0 * c12;
+        g13 -= 2.0 * c13;
+    }
+    a.H[k] = val + 2.0 * g12 + 4.0 * g13;
+}
+
+constexpr int kRed = 1024;
+
+__device__ void block_sum2(double& a, double& b, double* sa, double* sb) {
+    const int t = threadIdx.x;
+    sa[t] = a;
+    sb[t] = b;
+    __syncthreads();
+    for (int w = kRed / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sa[t] += sa[t + w];
+            sb[t] += sb[t + w];
+        }
+        __syncthreads();
+    }
+    a = sa[0];
+    b = sb[0];
+}
+
+__global__ __launch_bounds__(kRed) void k_oo_precondition(const double* __restrict__ g, const double* __restrict__ H,
+                                                          int nrot, double r, double* __restrict__ kref,
+                                                          double* __restrict__ kcur, double* __restrict__ ksave,
+                                                          double* __restrict__ out) {
+    __shared__ double sa[kRed], sb[kRed];
+    double gg = 0.0, quad = 0.0;
+    for (int k = threadIdx.x; k < nrot; k += kRed) {
+        const double gk = g[k], hk = H[k], ck = kcur[k];
+        gg = __fma_rn(gk, gk, gg);
+        quad += gk * ck + 0.5 * ck * hk * ck;
+        double hv = hk < 0.0 ? -hk : hk;
+        if (hv < 1.0e-2) hv = 1.0e-2;
+        const double kr = -gk / hv;
+        kref[k] = kr;
+        kcur[k] = r * kr;
+        ksave[k] = r * kr;
+    }
+    block_sum2(gg, quad, sa, sb);
+    if (threadIdx.x == 0) {
+        out[0] = gg;
+        out[1] = quad;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_oo_update_kappa(const double* __restrict__ kref, int nrot, double r,
+                                                         double* __restrict__ kcur, double* __restrict__ ksave) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrot) return;
+    const double kn = r * kref[k] - kcur[k];
+    kcur[k] = kn;
+    ksave[k] += kn;
+}
+
+__global__ __launch_bounds__(256) void k_oo_fill(double* __restrict__ p, int64_t n, double v) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) p[e] = v;
+}
+
+__global__ __launch_bounds__(256) void k_oo_scatter_kappa(const double* __restrict__ kappa, const int2* __restrict__ pairs,
+                                                          int nrot, double scale, int nfull, double* __restrict__ K) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrot) return;
+    const int p = pairs[k].x, q = pairs[k].y;
+    const double v = scale * kappa[k];
+    K[(int64_t)p * nfull + q] = v;
+    K[(int64_t)q * nfull + p] = -v;
+}
+
+__global__ __launch_bounds__(kRed) void k_oo_norm1(const double* __restrict__ K, int nfull, double* __restrict__ out) {
+    __shared__ double sa[kRed], sb[kRed];
+    double mx = 0.0, unused = 0.0;
+    for (int j = threadIdx.x; j < nfull; j += kRed) {
+        double s = 0.0;
+        for (int i = 0; i < nfull; i++) s += fabs(K[(int64_t)i * nfull + j]);
+        mx = s > mx ? s : mx;
+    }
+    sa[threadIdx.x] = mx;
+    sb[threadIdx.x] = unused;
+    __syncthreads();
+    for (int w = kRed / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sa[threadIdx.x] = fmax(sa[threadIdx.x], sa[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sa[0];
+}
+
+__global__ __launch_bounds__(256) void k_oo_identity(double* __restrict__ U, int nfull) {
+    const int64_t n = (int64_t)nfull * nfull;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        U[e] = (e / nfull == e % nfull) ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_oo_axpy(double a, const double* x, double* y, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        y[e] = __fma_rn(a, x[e], y[e]);
+}
+
+__global__ __launch_bounds__(256) void k_oo_sub(const double* __restrict__ x, const double* __restrict__ z,
+                                                double* __restrict__ y, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        y[e] = x[e] - z[e];
+}
+
+unsigned grid_for(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+}  // namespace
+
+void launch_dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda, int64_t sA,
+                  const double* B, int64_t ldb, int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
+                  int batch, hipStream_t s) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    const int64_t gy = (M + kT - 1) / kT, gx = (N + kT - 1) / kT;
+    if (gy > 65535 || batch > 65535 || gx > (int64_t)1 << 30)
+        throw std::invalid_argument("launch_dgemm: shape " + std::to_string(M) + " x " + std::to_string(N) +
+                                    " x batch " + std::to_string(batch) + " exceeds the grid");
+    const dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)batch);
+    if (!ta && !tb) hipLaunchKernelGGL((k_dgemm<false, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    else if (ta && !tb) hipLaunchKernelGGL((k_dgemm<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    else if (!ta && tb) hipLaunchKernelGGL((k_dgemm<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    else hipLaunchKernelGGL((k_dgemm<true, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+}
+
+void launch_df_unpack(const double* qmo, int nq, int nfull, double* bd, hipStream_t s) {
+    hipLaunchKernelGGL(k_df_unpack, dim3(grid_for((int64_t)nq * nfull * nfull)), dim3(256), 0, s, qmo, nq, nfull, bd);
+}
+
+void launch_df_pack(const double* bd, int nq, int nfull, double* qmo, hipStream_t s) {
+    hipLaunchKernelGGL(k_df_pack, dim3(grid_for((int64_t)nq * nfull * nfull)), dim3(256), 0, s, bd, nq, nfull, qmo);
+}
+
+void launch_oo_density(const OoTables& t, const double* x, double* D, double* d, hipStream_t s) {
+    const int64_t n2 = (int64_t)t.no * t.no;
+    hipLaunchKernelGGL(k_oo_density, dim3(grid_for(n2 * n2 + n2)), dim3(256), 0, s, t, x, D, d);
+}
+
+void launch_oo_gather(const double* bd, int nq, int nfull, const int* occ, int no, double* BO, double* BFO,
+                      double* Bdiag, hipStream_t s) {
+    const int64_t total = (int64_t)nq * no * no + (int64_t)nfull * no * nq + (int64_t)nfull * nq;
+    hipLaunchKernelGGL(k_oo_gather, dim3(grid_for(total)), dim3(256), 0, s, bd, nq, nfull, occ, no, BO, BFO, Bdiag);
+}
+
+void launch_oo_fock1(const double* D, const double* h, const int* occ, int no, int nfull, double* F, hipStream_t s) {
+    hipLaunchKernelGGL(k_oo_fock1, dim3(grid_for((int64_t)no * nfull)), dim3(256), 0, s, D, h, occ, no, nfull, F);
+}
+
+void launch_oo_grad_hess(const OoGradArgs& a, hipStream_t s) {
+    if (a.nrot <= 0) return;
+    hipLaunchKernelGGL(k_oo_grad_hess, dim3((a.nrot + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_oo_precondition(const double* g, const double* H, int nrot, double r, double* kref, double* kcur,
+                            double* ksave, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_oo_precondition, dim3(1), dim3(kRed), 0, s, g, H, nrot, r, kref, kcur, ksave, out);
+}
+
+void launch_oo_update_kappa(const double* kref, int nrot, double r, double* kcur, double* ksave, hipStream_t s) {
+    if (nrot <= 0) return;
+    hipLaunchKernelGGL(k_oo_update_kappa, dim3((nrot + 255) / 256), dim3(256), 0, s, kref, nrot, r, kcur, ksave);
+}
+
+void launch_oo_kappa_matrix(const double* kappa, const int2* pairs, int nrot, double scale, int nfull, double* K,
+                            double* out, hipStream_t s) {
+    const int64_t nn = (int64_t)nfull * nfull;
+    hipLaunchKernelGGL(k_oo_fill, dim3(grid_for(nn)), dim3(256), 0, s, K, nn, 0.0);
+    if (nrot > 0)
+        hipLaunchKernelGGL(k_oo_scatter_kappa, dim3((nrot + 255) / 256), dim3(256), 0, s, kappa, pairs, nrot, scale,
+                           nfull, K);
+    hipLaunchKernelGGL(k_oo_norm1, dim3(1), dim3(kRed), 0, s, K, nfull, out);
+}
+
+void launch_oo_identity(double* U, int nfull, hipStream_t s) {
+    hipLaunchKernelGGL(k_oo_identity, dim3(grid_for((int64_t)nfull * nfull)), dim3(256), 0, s, U, nfull);
+}
+
+void launch_oo_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_oo_axpy, dim3(grid_for(n)), dim3(256), 0, s, a, x, y, n);
+}
+
+void launch_oo_sub(const double* x, const double* z, double* y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_oo_sub, dim3(grid_for(n)), dim3(256), 0, s, x, z, y, n);
+}
+
+}  // namespace v2h

@@ -432,9 +432,7 @@ void Batch::set_integrals(int i, const double* h, const double* g) {
 // formed on the device in the source array, then packed like host integrals.
 double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, const double* qmo, int ncore,
                                const int* core, const int* active) {
-    const InstDev& I = inst_[i];
-    const Structure& S = *structs_[inst_sid_[i]];
-    const int n = S.L.n;
+    const int n = layout(i).n;
     if (nfull < 1 || nq < 1 || ncore < 0 || ncore + n > nfull)
         throw std::invalid_argument("set_df_integrals: need nfull >= ncore + nmo, nq >= 1");
     std::vector<char> seen(nfull, 0);
@@ -457,51 +455,82 @@ double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, c
     for (int c = 0; c < ncore; c++)
         for (int c2 = 0; c2 < ncore; c2++) idx.push_back(tri(core[c], core[c2]));
     const int64_t ntri = (int64_t)nfull * (nfull + 1) / 2;
-    DBuf<double> dq, dh, rows, dvec, gu, sq, e;
-    DBuf<int64_t> didx;
-    DBuf<int> dact, dcore;
-    dq.alloc((size_t)nq * ntri);
-    V2H_HIP(hipMemcpyAsync(dq.p, qmo, sizeof(double) * nq * ntri, hipMemcpyHostToDevice, stream_));
-    dh.alloc((size_t)nfull * nfull);
-    V2H_HIP(hipMemcpyAsync(dh.p, h_full, sizeof(double) * nfull * nfull, hipMemcpyHostToDevice, stream_));
-    didx.upload(idx);
-    dact.upload(std::vector<int>(active, active + n));
-    dcore.upload(ncore > 0 ? std::vector<int>(core, core + ncore) : std::vector<int>{0});
-    rows.alloc(idx.size() * (size_t)nq);
-    dvec.alloc(nq);
-    gu.alloc((size_t)kDfMaxSplits * nu * nu);
-    sq.alloc(std::max<size_t>(1, (size_t)ncore * ncore));
-    e.alloc(1);
+    if (df_.size() != (size_t)count()) df_.resize(count());
+    df_[i].reset(new DfState());
+    DfState& D = *df_[i];
+    D.nfull = nfull;
+    D.nq = nq;
+    D.n = n;
+    D.ncore = ncore;
+    D.no = ncore + n;
+    D.core.assign(core, core + ncore);
+    D.active.assign(active, active + n);
+    D.qmo.alloc((size_t)nq * ntri);
+    V2H_HIP(hipMemcpyAsync(D.qmo.p, qmo, sizeof(double) * nq * ntri, hipMemcpyHostToDevice, stream_));
+    D.h.alloc((size_t)nfull * nfull);
+    V2H_HIP(hipMemcpyAsync(D.h.p, h_full, sizeof(double) * nfull * nfull, hipMemcpyHostToDevice, stream_));
+    D.idx.upload(idx);
+    D.act.upload(std::vector<int>(active, active + n));
+    D.cor.upload(ncore > 0 ? std::vector<int>(core, core + ncore) : std::vector<int>{0});
+    D.m_rows = (int64_t)idx.size();
+    D.rows.alloc(idx.size() * (size_t)nq);
+    D.dvec.alloc(nq);
+    D.gu.alloc((size_t)kDfMaxSplits * nu * nu);
+    D.sq.alloc(std::max<size_t>(1, (size_t)ncore * ncore));
+    D.e.alloc(1);
+    D.uacc.alloc((size_t)nfull * nfull);
+    launch_oo_identity(D.uacc.p, nfull, stream_);
+    return run_df_intake(i);
+}
+
+DfState& Batch::df_state(int i) {
+    if ((size_t)i >= df_.size() || !df_[i])
+        throw std::invalid_argument("instance " + std::to_string(i) + " has no density-fitted integrals (set_df_integrals)");
+    return *df_[i];
+}
+
+double Batch::run_df_intake(int i) {
+    const InstDev& I = inst_[i];
+    const Structure& S = *structs_[inst_sid_[i]];
+    DfState& D = df_state(i);
     DfArgs a{};
-    a.qmo = dq.p;
-    a.h_full = dh.p;
-    a.idx = didx.p;
-    a.active = dact.p;
-    a.core = dcore.p;
-    a.ntri = ntri;
-    a.m_rows = (int64_t)idx.size();
-    a.nfull = nfull;
-    a.nq = nq;
-    a.n = n;
-    a.ncore = ncore;
-    a.rows = rows.p;
-    a.dvec = dvec.p;
-    a.gu = gu.p;
-    a.sq = sq.p;
+    a.qmo = D.qmo.p;
+    a.h_full = D.h.p;
+    a.idx = D.idx.p;
+    a.active = D.act.p;
+    a.core = D.cor.p;
+    a.ntri = (int64_t)D.nfull * (D.nfull + 1) / 2;
+    a.m_rows = D.m_rows;
+    a.nfull = D.nfull;
+    a.nq = D.nq;
+    a.n = D.n;
+    a.ncore = D.ncore;
+    a.rows = D.rows.p;
+    a.dvec = D.dvec.p;
+    a.gu = D.gu.p;
+    a.sq = D.sq.p;
     a.h_out = src_.p + I.soff;
-    a.g_out = src_.p + I.soff + (int64_t)n * n;
-    a.efzc = e.p;
+    a.g_out = src_.p + I.soff + (int64_t)D.n * D.n;
+    a.efzc = D.e.p;
     tstart(kKindDf);
     launch_df_intake(a, stream_);
     launch_pack_cost(S.cost1.p, S.cost2.p, src_.p + I.soff, c_.p + I.poff, S.n_primal, stream_);
-    launch_copy(v1_.p + I.poff, c_.p + I.poff, S.n_primal, stream_);  // v1 = c - z
+    launch_oo_sub(c_.p + I.poff, z_.p + I.poff, v1_.p + I.poff, S.n_primal, stream_);  // v1 = c - z
     tstop(kKindDf);
     V2H_HIP(hipGetLastError());
-    double efzc = 0.0;
-    V2H_HIP(hipMemcpyAsync(&efzc, e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
-    V2H_HIP(hipStreamSynchronize(stream_));  // before the scratch buffers go
+    V2H_HIP(hipMemcpyAsync(&D.efzc, D.e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
     refresh_norms(i);
-    return efzc;
+    return D.efzc;
+}
+
+double Batch::instance_energy(int i) {
+    const InstDev& I = inst_[i];
+    launch_dot1(c_.p + I.poff, x_.p + I.poff, n_primal(i), scratch_.p, stream_);
+    double cx = 0.0;
+    V2H_HIP(hipMemcpyAsync(&cx, scratch_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
+    return cx + df_state(i).efzc;
 }
 
 void Batch::set_cost(int i, const double* c) {

@@ -15,6 +15,7 @@
 #include "dbuf.hpp"
 #include "constraints.hpp"
 #include "layout.hpp"
+#include "orbopt.hpp"
 #include "projector.hpp"
 #include "sell.hpp"
 
@@ -59,6 +60,42 @@ struct FcidumpArgs {
     const char* rdm_path = nullptr;
 };
 
+// Resident density-fitted integrals of one instance (set_df_integrals): the
+// full-space slices, h, the accumulated orbital rotation and the scratch of
+// the intake and of the orbital step.
+struct DfState {
+    int nfull = 0, nq = 0, n = 0, ncore = 0, no = 0;
+    std::vector<int> core, active;
+    double efzc = 0.0;
+    int64_t m_rows = 0;
+    DBuf<double> qmo, h, bd, tmp, uacc, uscr, hscr;
+    DBuf<int64_t> idx;
+    DBuf<int> act, cor;
+    DBuf<double> rows, dvec, gu, sq, e;
+    // orbital step, set up by the first optimize_orbitals
+    std::vector<int> pair_key;
+    int nrot = -1;
+    OoTables tabs{};
+    DBuf<int2> pairs;
+    DBuf<int> occ, pos;
+    DBuf<int4> ptab;
+    DBuf<int64_t> d1tab;
+    DBuf<double> D, d, BO, BFO, Bdiag, gO, Z, F, Jd, Kd, g, H, kref, kcur, ksave, K, U, T1, T2, red;
+};
+
+// Options and results of one orbital-optimisation call (the ORBOPT_* options
+// of hilbert and OrbitalOptimizer::optimize_orbitals' outputs).
+struct OrbOptArgs {
+    const int* full_irreps = nullptr;  // [nfull], nullptr = C1
+    double g_convergence = 1.0e-4, e_convergence = 1.0e-8;
+    int maxiter = 10;
+    int active_active = 0;
+};
+struct OrbOptResult {
+    int iterations = 0, converged = 0, nrot = 0;
+    double gnorm = 0.0, dE = 0.0, energy = 0.0, efzc = 0.0;
+};
+
 class Batch {
   public:
     Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, int device);
@@ -82,6 +119,18 @@ class Batch {
     // The integral and RDM files of hilbert's FCIDUMP option for instance i
     // (its current primal vector); record counts out (may be null).
     void write_fcidump(int i, const FcidumpArgs& a, int64_t* n_int, int64_t* n_rdm);
+    // One orbital-optimisation call on the instance's resident DF integrals
+    // with its current primal vector (orbopt_driver.cpp); rotates the integrals,
+    // refreshes the cost vector and E_fzc.
+    void optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* r);
+    // Rotation pairs, gradient and exact diagonal Hessian at the current
+    // orbitals and primal vector, nothing rotated; returns the pair count
+    // (the arrays are written when cap >= that count).
+    int orbital_gradient(int i, const OrbOptArgs& a, int* pairs, double* g, double* H, int cap);
+    // Accumulated rotation U (new orbitals = orbitals given to
+    // set_df_integrals times U) and the current h_full and packed qmo; any
+    // pointer may be null.
+    void get_df_orbitals(int i, double* U, double* h_full, double* qmo);
     void set_guess(int i, const double* x);       // nullptr: hilbert's random guess
     void set_guess_hf(int i);                     // hilbert's TPDM_GUESS HF
     void reset_state(int i);                      // y = z = 0, mu = mu0, counters cleared
@@ -114,11 +163,13 @@ class Batch {
     // tridiagonal d <= 64 / 128 / 192 / 224 / 256, Jacobi d > 256 (matrix in
     // global memory); 11: the device part of one set_df_integrals (its
     // contraction kernels and the cost packing); 12: the pair Gram matrix of
-    // write_fcidump.
+    // write_fcidump; 13: the device work of optimize_orbitals outside the
+    // intake (densities, Fock and Hessian products, exponential, transform).
     static constexpr int kProjClasses = Projector::kClasses;
     static constexpr int kKindDf = 4 + kProjClasses;
     static constexpr int kKindGram = kKindDf + 1;
-    static constexpr int kKinds = kKindGram + 1;
+    static constexpr int kKindOrb = kKindGram + 1;
+    static constexpr int kKinds = kKindOrb + 1;
     // The projection classes run concurrently on stream_ (lane 0) and three
     // side streams: with the runtime's default of four hardware queues per
     // process, every stream of a batch keeps a queue of its own, so classes
@@ -149,6 +200,13 @@ class Batch {
     void sync_state_to_host();
     void sync_state_to_device();
     void refresh_norms(int i);
+    DfState& df_state(int i);
+    double run_df_intake(int i);          // resident DF integrals -> c, v1, E_fzc
+    double instance_energy(int i);        // c.x + E_fzc
+    void oo_setup(int i, const OrbOptArgs& a);
+    void oo_gradient(int i);
+    double oo_apply(int i, const double* kappa, double sign);  // rotate, re-run the intake, energy
+    std::vector<std::unique_ptr<DfState>> df_;
     BatchDev dev() const;
 
     SolverOptions opt_;

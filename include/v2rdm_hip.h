@@ -150,6 +150,51 @@ int v2h_batch_set_df_integrals(v2h_batch *b, int inst, int nfull, const double *
 int v2h_batch_write_fcidump(v2h_batch *b, int inst, int nfull, const int *full_irreps, const double *h_full, int nq,
                             const double *qmo, int ncore, const int *core, const int *active, double enuc,
                             const char *int_path, const char *rdm_path, int64_t *n_int, int64_t *n_rdm);
+/* One orbital-optimisation call of v2RDM-CASSCF on the instance's resident
+ * density-fitted integrals (those of the last v2h_batch_set_df_integrals,
+ * rotated by every call since): v2RDMSolver::RotateOrbitals
+ * (v2rdm_casscf/v2rdm_solver.cc:3221-3292) with the DF optimiser
+ * OrbitalOptimizer::optimize_orbitals (focas/orbopt_optimize.cc:25-110,
+ * exact diagonal Hessian).  The current primal x gives the densities
+ * (PackSpatialDensity, v2rdm_solver.cc:3110-3219); rotations between core,
+ * active and virtual orbitals of one irrep (plus active-active ones when
+ * opt->active_active) are optimised for up to opt->maxiter steps, the
+ * integrals are rotated in place and c and E_fzc are repacked
+ * (RepackIntegrals).  The SDP state (y, z, mu) is kept; unless the call
+ * converged, the instance is marked unconverged.  full_irreps [nfull] (NULL: C1) label the full space; the
+ * active orbitals must carry the instance's irreps.  res (may be NULL):
+ * iterations, converged (|dE| < e_convergence and |g| < g_convergence, the
+ * test v2rdm_solver.cc:3271-3274 applies to the call), rotation count,
+ * gradient norm at the last evaluation, dE over the call, the energy c.x +
+ * E_fzc afterwards and E_fzc. */
+typedef struct v2h_orbopt_options {
+    double g_convergence; /* ORBOPT_GRADIENT_CONVERGENCE, hilbert default 1e-4 */
+    double e_convergence; /* ORBOPT_ENERGY_CONVERGENCE, default 1e-8 */
+    int maxiter;          /* ORBOPT_MAXITER, default 10 */
+    int active_active;    /* ORBOPT_ACTIVE_ACTIVE_ROTATIONS, default 0 */
+} v2h_orbopt_options;
+typedef struct v2h_orbopt_result {
+    int iterations, converged, nrot;
+    double gnorm, dE, energy, efzc;
+} v2h_orbopt_result;
+int v2h_batch_optimize_orbitals(v2h_batch *b, int inst, const int *full_irreps, const v2h_orbopt_options *opt,
+                                v2h_orbopt_result *res);
+/* The orbital gradient dE/dkappa and the exact diagonal Hessian at the
+ * current orbitals and primal x (ComputeG and ComputeH_diag of the DF
+ * optimiser, focas/orbopt_gradient.cc:26-44, orbopt_hessian.cc:25-37), with
+ * nothing rotated.  Rotation k turns orbitals p = pairs[2k] and q =
+ * pairs[2k+1] into each other: K[p][q] = kappa, K[q][p] = -kappa, U = exp(K);
+ * p is virtual and q active or core, or p active and q core, or (active-active)
+ * p > q.  *nrot receives the pair count; the arrays are written only when
+ * cap >= that count (call with cap = 0 to size them). */
+int v2h_batch_orbital_gradient(v2h_batch *b, int inst, const int *full_irreps, int active_active, int32_t *pairs,
+                               double *g, double *H, int cap, int *nrot);
+/* The orbitals after optimisation: U [nfull*nfull] with new orbital r =
+ * sum_s old_s U[s*nfull + r] relative to the orbitals of
+ * v2h_batch_set_df_integrals (the reference multiplies its Ca by the same
+ * rotation, orbopt_integraltransform_cpu.cc:35-58), and the rotated h_full
+ * [nfull*nfull] and qmo [nq*ntri].  Any pointer may be NULL. */
+int v2h_batch_get_df_orbitals(v2h_batch *b, int inst, double *U, double *h_full, double *qmo);
 /* Primal guess; NULL = hilbert's TPDM_GUESS RANDOM (v2rdm_solver.cc:1799-1806). */
 int v2h_batch_set_guess(v2h_batch *b, int inst, const double *x);
 /* TPDM_GUESS HF (Guess, v2rdm_solver.cc:1711-1798; Q2_constraints_guess,
@@ -199,13 +244,15 @@ int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
 
 /* HIP-event timing of the kernels launched by v2h_batch_solve, recorded on
- * the library's stream.  ms[13] and counts[13] per kernel kind: 0 iteration
+ * the library's stream.  ms[14] and counts[14] per kernel kind: 0 iteration
  * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..10 PSD
  * projection classes: Jacobi for blocks up to 16, tridiagonal for blocks up to
  * 64 / 128 / 192 / 224 / 256, Jacobi beyond 256 (matrix in global memory);
  * 11 the device part of v2h_batch_set_df_integrals (from the first
  * contraction kernel to the packed c), 12 the pair Gram matrix of
- * v2h_batch_write_fcidump.  Cumulative since the
+ * v2h_batch_write_fcidump, 13 the device work of v2h_batch_optimize_orbitals
+ * besides the intake (densities, Fock and Hessian products, exponential,
+ * integral transform).  Cumulative since the
  * last reset.  full_ms / full_counts (may be NULL) cover only the launches
  * made while every instance was running (for CG kernels: still iterating)
  * at the previous host poll, i.e. launches that processed the whole batch. */

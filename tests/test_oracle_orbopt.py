@@ -1,0 +1,207 @@
+"""Pin the oracle's orbital step (oracle/orbopt.py) without the reference.
+
+* Energy identity: the dense O-space energy of any primal vector equals the
+  packed cost vector times x plus the frozen-core energy of the DF intake
+  oracle (core / active / virtual scattered over the full space, C1 and
+  symmetry), so the closed-shell core terms folded into D and d are right.
+* Every gradient element equals a central difference of E(kappa) along its
+  rotation, and every diagonal Hessian element a second difference, for all
+  rotation classes (virtual-core, virtual-active, active-core and
+  active-active), with RDMs of exact states.
+* The exponential equals scipy.linalg.expm.
+* For two electrons DQG is exact, so the converged v2RDM-CASSCF energy must
+  equal a brute-force CASSCF minimisation (BFGS over the non-redundant
+  rotations of the full-CI energy); the optimiser's steps never raise the
+  energy across a call.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.optimize
+
+from fci import Fock, RDMs, ground_state, pack_primal, spin_adapted_random_state
+from hilbert_amd.integrals import synthetic_df
+from oracle import orbopt as oo
+from oracle.pyoracle import OracleProblem, df_integrals
+
+
+def _setup(nfull, core, active, naux, seed, irreps=None, act_irr=None, na=2, nb=1, exact=True):
+    rng = np.random.default_rng(seed)
+    h, qmo, _ = synthetic_df(nfull, naux, rng, irreps=irreps)
+    n = len(active)
+    P = OracleProblem(n, na, nb, act_irr if act_irr is not None else [0] * n,
+                      None if act_irr is None else 4)
+    if exact:
+        assert act_irr is None
+        F = Fock(n)
+        psi = spin_adapted_random_state(F, na, nb, rng)
+        x = pack_primal(P, RDMs(F, psi))
+    else:
+        x = rng.standard_normal(P.n_primal)
+    return h, qmo, P, x
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_energy_identity(sym):
+    nfull = 8
+    core, active = [6, 1], [0, 3, 4, 7]
+    irr = [0, 1, 0, 1, 2, 3, 0, 2] if sym else None
+    act_irr = [irr[a] for a in active] if sym else None
+    h, qmo, P, x = _setup(nfull, core, active, 9, 5 + sym, irreps=irr, act_irr=act_irr, exact=False)
+    ha, ga, efzc = df_integrals(h, qmo, core, active)
+    e_packed = P.pack_c(ha, ga) @ x + efzc
+    D1, d = oo.spatial_rdms(P, x)
+    pz = oo.pitzer_order(act_irr if sym else [0] * 4)
+    occ = list(core) + [active[u] for u in pz]
+    D, dd = oo.occupied_rdms(D1, d, len(core))
+    B = oo.unpack_df(qmo, nfull)
+    assert abs(oo.energy(h, B, D, dd, occ) - e_packed) < 1e-11 * max(1.0, abs(e_packed))
+
+
+def _E(h, B, D, dd, occ, pairs, kappa):
+    U = oo.expm(oo.kappa_matrix(kappa, pairs, h.shape[0]))
+    h2, B2 = oo.rotate(h, B, U)
+    return oo.energy(h2, B2, D, dd, occ)
+
+
+@pytest.mark.parametrize("case", ["c1", "c1_actact", "sym"])
+def test_gradient_and_diagonal_hessian_by_differences(case):
+    nfull, core, active = 7, [2, 5], [0, 3, 6]
+    irr = None
+    if case == "sym":
+        irr = [0, 1, 0, 0, 1, 0, 1]
+    h, qmo, P, x = _setup(nfull, core, active, 8, 17)
+    if irr is not None:  # symmetry-adapt the integrals for the rotation set
+        h, qmo, _ = synthetic_df(nfull, 8, np.random.default_rng(3), irreps=irr)
+    D1, d = oo.spatial_rdms(P, x)
+    D, dd = oo.occupied_rdms(D1, d, len(core))
+    occ = list(core) + list(active)
+    B = oo.unpack_df(qmo, nfull)
+    pairs = oo.rotation_pairs(nfull, core, active, irr, active_active=(case == "c1_actact"))
+    g, H = oo.gradient_hessian(h, B, D, dd, occ, pairs)
+    nrot = len(pairs)
+    assert nrot > 0
+    E0 = oo.energy(h, B, D, dd, occ)
+    for k in range(nrot):
+        e = np.zeros(nrot)
+        e[k] = 1.0
+        d1 = 1e-5
+        fd = (_E(h, B, D, dd, occ, pairs, d1 * e) - _E(h, B, D, dd, occ, pairs, -d1 * e)) / (2 * d1)
+        assert abs(fd - g[k]) < 1e-7 * max(1.0, abs(g[k])), (case, k, tuple(pairs[k]), fd, g[k])
+        d2 = 2e-3
+        sd = (_E(h, B, D, dd, occ, pairs, d2 * e) - 2 * E0 + _E(h, B, D, dd, occ, pairs, -d2 * e)) / d2 ** 2
+        assert abs(sd - H[k]) < 2e-5 * max(1.0, abs(H[k])), (case, k, tuple(pairs[k]), sd, H[k])
+
+
+def test_planted_mistake_is_caught():
+    """The Hessian formula needs d symmetrised over (pq) <-> (qp); fed the raw
+    d (energy and gradient unchanged), the second differences reject it."""
+    nfull, core, active = 6, [1], [0, 3, 4]
+    h, qmo, P, x = _setup(nfull, core, active, 7, 23)
+    D1, d = oo.spatial_rdms(P, x)
+    occ = list(core) + list(active)
+    B = oo.unpack_df(qmo, nfull)
+    pairs = oo.rotation_pairs(nfull, core, active)
+    D, dd = oo.occupied_rdms(D1, d, len(core))
+    _, draw = oo.occupied_rdms(D1, d, len(core), symmetrise=False)
+    g, H = oo.gradient_hessian(h, B, D, dd, occ, pairs)
+    g2, H2 = oo.gradient_hessian(h, B, D, draw, occ, pairs)
+    assert abs(oo.energy(h, B, D, draw, occ) - oo.energy(h, B, D, dd, occ)) < 1e-12
+    assert np.abs(g - g2).max() < 1e-12
+    assert np.abs(H - H2).max() > 1e-3
+
+
+def test_expm_matches_scipy():
+    rng = np.random.default_rng(4)
+    for scale in (1e-3, 0.3, 2.0, 7.0):
+        A = rng.standard_normal((9, 9)) * scale
+        K = A - A.T
+        U = oo.expm(K)
+        assert np.abs(U - scipy.linalg.expm(K)).max() < 1e-13 * max(1.0, scale)
+        assert np.abs(U.T @ U - np.eye(9)).max() < 1e-13
+
+
+def test_df_pack_round_trip():
+    rng = np.random.default_rng(8)
+    h, qmo, L = synthetic_df(6, 4, rng)
+    B = oo.unpack_df(qmo, 6)
+    assert np.array_equal(B, L)
+    assert np.array_equal(oo.pack_df(B), qmo)
+
+
+def _cas_model(nfull, ncore, seed):
+    """Integrals with a clear core < active < virtual orbital-energy order."""
+    rng = np.random.default_rng(seed)
+    naux = 6
+    L = rng.normal(0.0, 0.12, (naux, nfull, nfull))
+    L = np.triu(L) + np.transpose(np.triu(L, 1), (0, 2, 1))
+    h = rng.normal(0.0, 0.08, (nfull, nfull))
+    h = np.triu(h, 1)
+    h = h + h.T
+    diag = np.concatenate([[-4.0 - k for k in range(ncore)], [-1.6, -0.9], 0.6 + 0.5 * np.arange(nfull - ncore - 2)])
+    perm = rng.permutation(nfull)          # scatter the classes over the full order
+    h[np.diag_indices(nfull)] = diag[perm]       # orbital j has rank perm[j]
+    order = np.argsort(perm)
+    core = [int(order[k]) for k in range(ncore)]
+    active = [int(order[ncore]), int(order[ncore + 1])]
+    from hilbert_amd.integrals import pack_qmo
+    return h, pack_qmo(L), core, active
+
+
+def _fci_energy(h, qmo, core, active):
+    ha, ga, efzc = df_integrals(h, qmo, core, active)
+    F = Fock(len(active))
+    e, _ = ground_state(F, F.hamiltonian(ha, ga), 1, 1)
+    return e + efzc
+
+
+def _brute_force_casscf(h, qmo, core, active):
+    nfull = h.shape[0]
+    B = oo.unpack_df(qmo, nfull)
+    pairs = oo.rotation_pairs(nfull, core, active)
+
+    def f(kappa):
+        U = oo.expm(oo.kappa_matrix(kappa, pairs, nfull))
+        h2, B2 = oo.rotate(h, B, U)
+        return _fci_energy(h2, oo.pack_df(B2), core, active)
+
+    res = scipy.optimize.minimize(f, np.zeros(len(pairs)), method="BFGS", options=dict(gtol=1e-9))
+    return res.fun
+
+
+def oracle_casscf(h, qmo, core, active, outer=60):
+    """Alternate tight SDP solves and optimize_orbitals calls to convergence."""
+    nfull = h.shape[0]
+    P = OracleProblem(len(active), 1, 1)
+    B = oo.unpack_df(qmo, nfull)
+    pairs = oo.rotation_pairs(nfull, core, active)
+    occ = list(core) + list(active)
+    x = None
+    for _ in range(outer):
+        ha, ga, efzc = df_integrals(h, oo.pack_df(B), core, active)
+        c = P.pack_c(ha, ga)
+        res = P.solve(c, x=x, e_convergence=1e-10, r_convergence=1e-9, cg_convergence=1e-12)
+        x = res["x"]
+        D1, d = oo.spatial_rdms(P, x)
+        D, dd = oo.occupied_rdms(D1, d, len(core))
+        e_before = c @ x + efzc
+        assert abs(oo.energy(h, B, D, dd, occ) - e_before) < 1e-10
+        h, B, _, info = oo.optimize_orbitals(h, B, D, dd, occ, pairs)
+        assert info["dE"] <= 1e-12
+        if info["converged"] and abs(info["dE"]) < 1e-9:
+            break
+    ha, ga, efzc = df_integrals(h, oo.pack_df(B), core, active)
+    c = P.pack_c(ha, ga)
+    res = P.solve(c, x=x, e_convergence=1e-10, r_convergence=1e-9, cg_convergence=1e-12)
+    return c @ res["x"] + efzc, h, B
+
+
+@pytest.mark.parametrize("nfull,ncore", [(4, 0), (5, 1)])
+def test_two_electron_casscf_equals_brute_force(nfull, ncore):
+    h, qmo, core, active = _cas_model(nfull, ncore, 31 + nfull)
+    e_bf = _brute_force_casscf(h, qmo, core, active)
+    e_v2, _, _ = oracle_casscf(h, qmo, core, active)
+    e0 = _fci_energy(h, qmo, core, active)
+    print("E(fixed orbitals)", e0, "CASSCF brute force", e_bf, "oracle v2RDM-CASSCF", e_v2)
+    assert e_bf < e0 - 1e-4          # the orbitals do relax
+    assert abs(e_v2 - e_bf) < 1e-7
