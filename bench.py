@@ -28,7 +28,15 @@ tables/s of its device part, the pair Gram matrix (Q|mn)^T (Q|mn) over all
 of whole writes (device pass, copy back, 3.2 M integral records and the RDM
 records written to files) is reported beside it.
 
-Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1] [--workload sdp|df|fcidump]
+With --workload orbopt it measures one orbital-optimisation call of
+v2RDM-CASSCF (v2h_batch_optimize_orbitals, ORBOPT_MAXITER 10) on the same
+100-orbital space with the (16e,16o) HF-guess densities: value = calls/s of
+the whole call (densities, generalized Fock, exact diagonal Hessian, steps,
+exponentials, transforms of the resident (Q|mn) and the DF intakes; integrals
+already in HBM).  The roofline is the FP64 rate of the call's device work
+outside the intakes (HIP events) against the FP64 peak.
+
+Usage: python bench.py [--steps K] [--warmup W] [--batch B] [--gpus 1] [--workload sdp|df|fcidump|orbopt]
 """
 from __future__ import annotations
 
@@ -296,6 +304,146 @@ def bench_df(args):
     print(json.dumps(out), flush=True)
 
 
+def orbopt_flops(nfull, nq, no, evals, applies):
+    """FP64 flops of the GEMMs of one orbital call: per gradient evaluation
+    (O)^4 integrals, Z, the Fock product, (pp|tu) and (ps|pu); per applied
+    rotation the 14 Taylor terms of exp(K) and the transforms of every (Q|mn)
+    slice, h and the accumulated rotation (the squarings of exp(K), at most a
+    few nfull^3 products, are not counted)."""
+    n2 = no * no
+    per_eval = 2.0 * (n2 * n2 * nq * 2 + no * nfull * no * nq + nfull * nq * n2 + nfull * n2 * nq)
+    per_apply = 2.0 * nfull ** 3 * (14 + 2 * nq + 3)
+    return evals * per_eval + applies * per_apply
+
+
+def ordered_df(rng, irr):
+    """Synthetic DF integrals of a molecule-like shape: (Q|mn) scaled to
+    (pp|pp) ~ 0.4 Eh and h ordered core < active < virtual."""
+    import numpy as np
+    from hilbert_amd.integrals import synthetic_df
+    h, qmo, _ = synthetic_df(DF_NFULL, DF_NQ, rng, irreps=irr)
+    qmo *= 0.2
+    diag = np.concatenate([-4.0 + 0.1 * np.arange(DF_NCORE), -1.2 + 0.08 * np.arange(DF_NACT),
+                           0.6 + 0.05 * np.arange(DF_NFULL - DF_NCORE - DF_NACT)])
+    h[np.diag_indices(DF_NFULL)] = diag
+    return h, qmo
+
+
+def bench_orbopt(args):
+    import numpy as np
+    from hilbert_amd.batch import Batch, Problem, options
+    rng = np.random.default_rng(2024)
+    # C2v labels in turn: the (16e,16o) active space has 4 orbitals per irrep,
+    # the shape of the batched-solve workload
+    irr = (np.arange(DF_NFULL) % 4).astype(np.int32)
+    irr[DF_NCORE:DF_NCORE + DF_NACT] = np.repeat(np.arange(4), DF_NACT // 4)
+    h, qmo = ordered_df(rng, irr)
+    core = np.arange(DF_NCORE, dtype=np.int32)
+    active = np.arange(DF_NCORE, DF_NCORE + DF_NACT, dtype=np.int32)
+    prob = Problem(DF_NACT, DF_NACT // 2, DF_NACT // 2, irr[active].tolist(), 4)
+    B = Batch([prob], options(maxiter=20000))
+    # the densities of a converged SDP at the starting orbitals (untimed)
+    B.set_df_integrals(0, h, qmo, core, active)
+    B.set_guess(0, None)
+    print("[orbopt] integrals set, solving the SDP", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    while True:
+        B.solve(200)
+        sdp = B.status(0)
+        print(f"[orbopt] SDP at the starting orbitals: {sdp['oiter']} outer iterations, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if sdp["converged"] or sdp["oiter"] >= 20000 or sdp["status_bits"]:
+            break
+    for _ in range(args.warmup):
+        B.set_df_integrals(0, h, qmo, core, active)
+        B.optimize_orbitals(0, irr)
+    B.set_profiling(True)
+    B.reset_kernel_times()
+    call_s = 0.0
+    res = None
+    for k in range(args.steps):
+        B.set_df_integrals(0, h, qmo, core, active)
+        t0 = time.perf_counter()
+        res = B.optimize_orbitals(0, irr)
+        call_s += time.perf_counter() - t0
+        print(f"[orbopt] call {k}: {res['iterations']} iterations, dE {res['dE']:.3e}, |g| {res['gnorm']:.3e}",
+              file=sys.stderr, flush=True)
+    call_s /= args.steps
+    kt = B.kernel_times()
+    orb_ms = kt["orbopt"][0] / args.steps
+    U_gpu, h_gpu, _ = B.df_orbitals(0)
+    x = B.x(0)
+    # the whole v2RDM-CASSCF loop from the starting orbitals (ORBOPT_FREQUENCY
+    # 500), SDP state warm from the converged fixed-orbital solve
+    casscf = None
+    if args.casscf:
+        B.set_profiling(False)
+        e0 = B.set_df_integrals(0, h, qmo, core, active)
+        e_fixed = float(B.c(0) @ B.x(0)) + e0
+
+        def prog(m, last):
+            print(f"[orbopt] CASSCF macro iteration {m}: E {last[0]['energy']:.10f}, |g| {last[0]['gnorm']:.2e}, "
+                  f"dE {last[0]['dE']:.2e}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        out = B.solve_casscf(orbopt_frequency=500, full_irreps=irr, progress=prog)[0]
+        wall = time.perf_counter() - t0
+        casscf = {"seconds": wall, "energy": out["energy"], "fixed_orbital_energy": e_fixed,
+                  "macro_iterations": out["macro"], "sdp_outer_iterations": out["oiter"],
+                  "converged": bool(out["sdp_converged"] and out["orbopt_converged"]),
+                  "note": "one instance, warm start from the converged fixed-orbital SDP; "
+                          "energies c.x + E_fzc without nuclear repulsion"}
+    B.close()
+    flops = orbopt_flops(DF_NFULL, DF_NQ, DF_NCORE + DF_NACT, res["evaluations"], res["rotations"])
+    achieved = flops / (orb_ms * 1e-3) / 1e12
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle import orbopt as oo
+        from oracle.pyoracle import OracleProblem
+        P = OracleProblem(DF_NACT, DF_NACT // 2, DF_NACT // 2, irr[active].tolist(), 4)
+        D1, d = oo.spatial_rdms(P, x)
+        D, dd = oo.occupied_rdms(D1, d, DF_NCORE)
+        pz = oo.pitzer_order(irr[active])
+        occ = list(core) + [int(active[u]) for u in pz]
+        pairs = oo.rotation_pairs(DF_NFULL, core, active, irr)
+        Bd = oo.unpack_df(qmo, DF_NFULL)
+        print("[orbopt] oracle call", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        h2, _, Uo, info = oo.optimize_orbitals(h, Bd, D, dd, occ, pairs)
+        dt = time.perf_counter() - t0
+        try:
+            from threadpoolctl import threadpool_info
+            threads = max([t.get("num_threads", 1) for t in threadpool_info()] or [1])
+        except Exception:
+            threads = None
+        cpu = {"value": 1.0 / dt, "unit": "orbital calls/s", "cores": threads, "kind": "port",
+               "sample": f"one call of the oracle's optimize_orbitals (numpy, BLAS threads as reported) on the "
+                         f"same densities: {info['iterations']} iterations; max |U_gpu - U_oracle| = "
+                         f"{float(np.abs(U_gpu - Uo).max()):.1e}, |E_gpu - E_oracle| = "
+                         f"{abs(res['energy'] - info['energy']):.1e}"}
+    out = {
+        "metric": "v2RDM-CASSCF orbital-optimisation calls/s (ORBOPT_MAXITER 10), 100 orbitals, 400 aux, "
+                  "10 core, (16e,16o) active",
+        "value": 1.0 / call_s, "unit": "calls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": call_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "orbital step: densities, generalized Fock, exact diagonal Hessian, steps, "
+                               "exp(K), (Q|mn) transforms, DF intakes", "nfull": DF_NFULL, "naux": DF_NQ,
+                   "ncore": DF_NCORE, "nmo": DF_NACT, "nirrep": 4, "rotations": res["nrot"],
+                   "densities": "converged DQG SDP at the starting orbitals (%d outer iterations)" % sdp["oiter"],
+                   "iterations": res["iterations"], "gradient_evaluations": res["evaluations"],
+                   "rotations_applied": res["rotations"], "dE": res["dE"], "gnorm": res["gnorm"]},
+        "device_ms": {"orbopt_kernels": orb_ms, "df_intakes": kt["df_intake"][0] / args.steps},
+        "casscf": casscf,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "GEMM flops of the call (transforms of 400 slices dominate) over the HIP-event "
+                             "time of its device work outside the intakes; k_dgemm is a vector-FMA kernel"},
+        "cpu_baseline": cpu,
+        "host": host_info(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_fcidump(args):
     import tempfile
     import numpy as np
@@ -392,9 +540,11 @@ def main():
                     help="sdp workload: the N-representability conditions")
     ap.add_argument("--d3", type=int, default=0, help="sdp workload: add the D3 condition (CONSTRAIN_D3)")
     ap.add_argument("--chunk", type=int, default=1000, help="outer iterations between progress lines")
-    ap.add_argument("--workload", choices=("sdp", "df", "fcidump"), default="sdp",
+    ap.add_argument("--casscf", type=int, default=1,
+                    help="orbopt workload: also run the whole v2RDM-CASSCF loop once (0 = skip)")
+    ap.add_argument("--workload", choices=("sdp", "df", "fcidump", "orbopt"), default="sdp",
                     help="sdp: the batched (16e,16o) solve; df: the density-fitted integral intake; "
-                         "fcidump: the FCIDUMP writer")
+                         "fcidump: the FCIDUMP writer; orbopt: one v2RDM-CASSCF orbital step")
     args = ap.parse_args()
     PROGRESS_CHUNK[0] = max(1, args.chunk)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -416,6 +566,11 @@ def main():
         if args.steps is None:
             args.steps = 5
         bench_fcidump(args)
+        return
+    if args.workload == "orbopt":
+        if args.steps is None:
+            args.steps = 5
+        bench_orbopt(args)
         return
     if args.steps is None:
         args.steps = 1

@@ -40,7 +40,8 @@ class OrbOptOptions(C.Structure):
 
 
 class OrbOptResult(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("nrot", C.c_int), ("gnorm", C.c_double),
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("nrot", C.c_int), ("evaluations", C.c_int),
+                ("rotations", C.c_int), ("gnorm", C.c_double),
                 ("dE", C.c_double), ("energy", C.c_double), ("efzc", C.c_double)]
 
 

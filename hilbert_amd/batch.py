@@ -160,14 +160,15 @@ class Batch:
                                                   h.ctypes.data_as(C.c_void_p), q.ctypes.data_as(C.c_void_p)))
         return U, h, q
 
-    def solve_casscf(self, orbopt_frequency=500, full_irreps=None, monitor=None, **orbopt) -> list:
+    def solve_casscf(self, orbopt_frequency=500, full_irreps=None, monitor=None, progress=None, **orbopt) -> list:
         """v2RDM-CASSCF: alternate orbopt_frequency BPSDP iterations with one
         optimize_orbitals call per instance until the SDP and the orbitals
         are converged (the do { solve; RotateOrbitals } while loop of
         v2rdm_solver.cc:1296-1361; ORBOPT_FREQUENCY default 500).
 
         full_irreps: None, one array for every instance, or a list of
-        per-instance arrays.  Returns per instance dict(energy = c.x + E_fzc
+        per-instance arrays.  progress(macro, results) is called after every
+        round of orbital calls.  Returns per instance dict(energy = c.x + E_fzc
         after the last orbital step, sdp_converged, orbopt_converged,
         macro = orbital calls, last = the last call's result)."""
         n = self.count
@@ -196,6 +197,8 @@ class Batch:
                 if last[i]["converged"] and st["converged"]:
                     # a converged orbital call leaves the SDP marked converged
                     done[i] = sdp_conv[i] = True
+            if progress:
+                progress(max(macro), last)
         out = []
         for i in range(n):
             st = self.status(i)

@@ -273,6 +273,8 @@ int v2h_batch_optimize_orbitals(v2h_batch* b, int inst, const int* full_irreps, 
             res->iterations = r.iterations;
             res->converged = r.converged;
             res->nrot = r.nrot;
+            res->evaluations = r.evaluations;
+            res->rotations = r.rotations;
             res->gnorm = r.gnorm;
             res->dE = r.dE;
             res->energy = r.energy;

@@ -17,6 +17,7 @@
 // energy along one rotation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -30,17 +31,22 @@ namespace {
 constexpr int kT = 64;    // output tile
 constexpr int kKc = 16;   // K chunk staged in LDS
 
+// kc > 0: split K; batch entry bz covers k in [bz kc, min(K, (bz + 1) kc)) and
+// writes its partial product to C + bz sC (summed by k_dgemm_slices).
+// dg is added on the diagonal of the result (m == n).
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, double alpha, const double* __restrict__ A,
                                                int64_t lda, int64_t sA, const double* __restrict__ B, int64_t ldb,
                                                int64_t sB, double beta, double* __restrict__ C, int64_t ldc,
-                                               int64_t sC) {
+                                               int64_t sC, int kc, double dg) {
     __shared__ double As[kKc][kT + 1];
     __shared__ double Bs[kKc][kT + 1];
     const int64_t bz = blockIdx.z;
     A += bz * sA;
     B += bz * sB;
     C += bz * sC;
+    const int kbeg = kc > 0 ? (int)bz * kc : 0;
+    const int kend = kc > 0 ? min(K, kbeg + kc) : K;
     const int m0 = blockIdx.y * kT, n0 = blockIdx.x * kT;
     const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
     double acc[4][4];
@@ -48,20 +54,20 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, double alpha
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-    for (int k0 = 0; k0 < K; k0 += kKc) {
+    for (int k0 = kbeg; k0 < kend; k0 += kKc) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int idx = t + 256 * e;
             int m, k;
             if (!TA) { m = idx >> 4; k = idx & 15; } else { k = idx >> 6; m = idx & 63; }
             double v = 0.0;
-            if (m0 + m < M && k0 + k < K)
+            if (m0 + m < M && k0 + k < kend)
                 v = TA ? A[(int64_t)(k0 + k) * lda + (m0 + m)] : A[(int64_t)(m0 + m) * lda + (k0 + k)];
             As[k][m] = v;
             int n, k2;
             if (!TB) { k2 = idx >> 6; n = idx & 63; } else { n = idx >> 4; k2 = idx & 15; }
             double w = 0.0;
-            if (n0 + n < N && k0 + k2 < K)
+            if (n0 + n < N && k0 + k2 < kend)
                 w = TB ? B[(int64_t)(n0 + n) * ldb + (k0 + k2)] : B[(int64_t)(k0 + k2) * ldb + (n0 + n)];
             Bs[k2][n] = w;
         }
@@ -89,8 +95,23 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, double alpha
             const int n = n0 + tx + 16 * j;
             if (n >= N) continue;
             double* c = C + (int64_t)m * ldc + n;
-            *c = beta != 0.0 ? __fma_rn(alpha, acc[i][j], beta * *c) : alpha * acc[i][j];
+            double v = beta != 0.0 ? __fma_rn(alpha, acc[i][j], beta * *c) : alpha * acc[i][j];
+            if (m == n) v += dg;
+            *c = v;
         }
+    }
+}
+
+// C[m][n] = alpha sum_s W[s][m][n] + beta C[m][n], slices summed in order
+__global__ __launch_bounds__(256) void k_dgemm_slices(const double* __restrict__ W, int slices, int M, int N,
+                                                      double alpha, double beta, double* __restrict__ C, int64_t ldc) {
+    const int64_t mn = (int64_t)M * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mn; e += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < slices; k++) s += W[k * mn + e];
+        const int m = (int)(e / N), n = (int)(e - (int64_t)m * N);
+        double* c = C + (int64_t)m * ldc + n;
+        *c = beta != 0.0 ? __fma_rn(alpha, s, beta * *c) : alpha * s;
     }
 }
 
@@ -214,13 +235,50 @@ __global__ __launch_bounds__(256) void k_oo_fock1(const double* __restrict__ D, 
     }
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One wavefront per rotation pair; the lanes split the O^2 sums and combine
+// them with butterfly shuffles (the same order on every run).
 __global__ __launch_bounds__(256) void k_oo_grad_hess(OoGradArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nrot) return;
+    const int k = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= a.nrot) return;  // uniform per wavefront
     const int p = a.pairs[k].x, q = a.pairs[k].y;
     const int no = a.no, nf = a.nfull;
-    const int64_t n2 = (int64_t)no * no;
+    const int n2 = no * no;
     const int ip = a.pos[p], iq = a.pos[q];
+    const double* Jp = a.Jd + (int64_t)p * n2;
+    const double* Jq = a.Jd + (int64_t)q * n2;
+    const double* Kp = a.Kd + (int64_t)p * n2;
+    const double* Kq = a.Kd + (int64_t)q * n2;
+    const int64_t no3 = (int64_t)n2 * no;
+    double g12 = 0.0, g13 = 0.0, c12 = 0.0, c13 = 0.0;
+    for (int e = lane; e < n2; e += 64) {
+        const int s = e / no, u = e - s * no;
+        if (ip >= 0) {
+            g12 = __fma_rn(a.d[((int64_t)ip * no + ip) * n2 + e], Jq[e], g12);
+            g13 = __fma_rn(a.d[ip * no3 + (int64_t)s * n2 + (int64_t)ip * no + u], Kq[e], g13);
+        }
+        if (iq >= 0) {
+            g12 = __fma_rn(a.d[((int64_t)iq * no + iq) * n2 + e], Jp[e], g12);
+            g13 = __fma_rn(a.d[iq * no3 + (int64_t)s * n2 + (int64_t)iq * no + u], Kp[e], g13);
+        }
+        if (ip >= 0 && iq >= 0) {
+            const int64_t row = ((int64_t)ip * no + iq) * n2;
+            c12 = __fma_rn(a.d[row + e], a.gO[row + e], c12);
+            c13 = __fma_rn(a.d[ip * no3 + (int64_t)s * n2 + (int64_t)iq * no + u],
+                           a.gO[iq * no3 + (int64_t)s * n2 + (int64_t)ip * no + u], c13);
+        }
+    }
+    g12 = wave_sum(g12);
+    g13 = wave_sum(g13);
+    c12 = wave_sum(c12);
+    c13 = wave_sum(c13);
+    if (lane != 0) return;
     const double Fpq = ip >= 0 ? a.F[(int64_t)ip * nf + q] : 0.0;
     const double Fqp = iq >= 0 ? a.F[(int64_t)iq * nf + p] : 0.0;
     a.g[k] = 2.0 * (Fqp - Fpq);
@@ -230,38 +288,8 @@ __global__ __launch_bounds__(256) void k_oo_grad_hess(OoGradArgs a) {
     const double Dqq = iq >= 0 ? a.D[iq * no + iq] : 0.0;
     const double Dpq = (ip >= 0 && iq >= 0) ? a.D[ip * no + iq] : 0.0;
     const double hpp = a.h[(int64_t)p * nf + p], hqq = a.h[(int64_t)q * nf + q], hpq = a.h[(int64_t)p * nf + q];
-    double val = -2.0 * (Fpp + Fqq) + 2.0 * (Dpp * hqq + Dqq * hpp - 2.0 * Dpq * hpq);
-    double g12 = 0.0, g13 = 0.0;
-    const double* Jp = a.Jd + (int64_t)p * n2;
-    const double* Jq = a.Jd + (int64_t)q * n2;
-    const double* Kp = a.Kd + (int64_t)p * n2;
-    const double* Kq = a.Kd + (int64_t)q * n2;
-    if (ip >= 0) {
-        const double* dpp = a.d + ((int64_t)ip * no + ip) * n2;
-        for (int64_t tu = 0; tu < n2; tu++) g12 = __fma_rn(dpp[tu], Jq[tu], g12);
-        for (int s = 0; s < no; s++)
-            for (int u = 0; u < no; u++)
-                g13 = __fma_rn(a.d[(((int64_t)ip * no + s) * no + ip) * no + u], Kq[s * no + u], g13);
-    }
-    if (iq >= 0) {
-        const double* dqq = a.d + ((int64_t)iq * no + iq) * n2;
-        for (int64_t tu = 0; tu < n2; tu++) g12 = __fma_rn(dqq[tu], Jp[tu], g12);
-        for (int s = 0; s < no; s++)
-            for (int u = 0; u < no; u++)
-                g13 = __fma_rn(a.d[(((int64_t)iq * no + s) * no + iq) * no + u], Kp[s * no + u], g13);
-    }
-    if (ip >= 0 && iq >= 0) {
-        const int64_t row = ((int64_t)ip * no + iq) * n2;
-        double c12 = 0.0, c13 = 0.0;
-        for (int64_t tu = 0; tu < n2; tu++) c12 = __fma_rn(a.d[row + tu], a.gO[row + tu], c12);
-        for (int s = 0; s < no; s++)
-            for (int u = 0; u < no; u++)
-                c13 = __fma_rn(a.d[(((int64_t)ip * no + s) * no + iq) * no + u],
-                               a.gO[(((int64_t)iq * no + s) * no + ip) * no + u], c13);
-        g12 -= 2.0 * c12;
-        g13 -= 2.0 * c13;
-    }
-    a.H[k] = val + 2.0 * g12 + 4.0 * g13;
+    const double val = -2.0 * (Fpp + Fqq) + 2.0 * (Dpp * hqq + Dqq * hpp - 2.0 * Dpq * hpq);
+    a.H[k] = val + 2.0 * (g12 - 2.0 * c12) + 4.0 * (g13 - 2.0 * c13);
 }
 
 constexpr int kRed = 1024;
@@ -371,19 +399,51 @@ unsigned grid_for(int64_t n) {
 
 }  // namespace
 
-void launch_dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda, int64_t sA,
-                  const double* B, int64_t ldb, int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
-                  int batch, hipStream_t s) {
+namespace {
+
+void dgemm_raw(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda, int64_t sA,
+               const double* B, int64_t ldb, int64_t sB, double beta, double* C, int64_t ldc, int64_t sC, int batch,
+               int kc, double dg, hipStream_t s) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
     const int64_t gy = (M + kT - 1) / kT, gx = (N + kT - 1) / kT;
     if (gy > 65535 || batch > 65535 || gx > (int64_t)1 << 30)
         throw std::invalid_argument("launch_dgemm: shape " + std::to_string(M) + " x " + std::to_string(N) +
                                     " x batch " + std::to_string(batch) + " exceeds the grid");
     const dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)batch);
-    if (!ta && !tb) hipLaunchKernelGGL((k_dgemm<false, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
-    else if (ta && !tb) hipLaunchKernelGGL((k_dgemm<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
-    else if (!ta && tb) hipLaunchKernelGGL((k_dgemm<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
-    else hipLaunchKernelGGL((k_dgemm<true, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
+    if (!ta && !tb) hipLaunchKernelGGL((k_dgemm<false, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+    else if (ta && !tb) hipLaunchKernelGGL((k_dgemm<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+    else if (!ta && tb) hipLaunchKernelGGL((k_dgemm<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+    else hipLaunchKernelGGL((k_dgemm<true, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+}
+
+}  // namespace
+
+void launch_dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda, int64_t sA,
+                  const double* B, int64_t ldb, int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
+                  int batch, hipStream_t s, double dg) {
+    dgemm_raw(ta, tb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, 0, dg, s);
+}
+
+int dgemm_split_slices(int M, int N, int K) {
+    const int64_t tiles = (int64_t)((M + kT - 1) / kT) * ((N + kT - 1) / kT);
+    int sl = (int)std::min<int64_t>(kMaxSlices, std::max<int64_t>(1, 512 / std::max<int64_t>(1, tiles)));
+    sl = std::min(sl, std::max(1, K / 256));
+    return sl;
+}
+
+void launch_dgemm_split(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda,
+                        const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* work,
+                        hipStream_t s) {
+    const int sl = dgemm_split_slices(M, N, K);
+    if (sl <= 1) {
+        dgemm_raw(ta, tb, M, N, K, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, 0, 0.0, s);
+        return;
+    }
+    const int kc = ((K + sl - 1) / sl + kKc - 1) / kKc * kKc;
+    const int slices = (K + kc - 1) / kc;
+    dgemm_raw(ta, tb, M, N, K, 1.0, A, lda, 0, B, ldb, 0, 0.0, work, N, (int64_t)M * N, slices, kc, 0.0, s);
+    const int64_t mn = (int64_t)M * N;
+    hipLaunchKernelGGL(k_dgemm_slices, dim3(grid_for(mn)), dim3(256), 0, s, work, slices, M, N, alpha, beta, C, ldc);
 }
 
 void launch_df_unpack(const double* qmo, int nq, int nfull, double* bd, hipStream_t s) {
@@ -411,7 +471,7 @@ void launch_oo_fock1(const double* D, const double* h, const int* occ, int no, i
 
 void launch_oo_grad_hess(const OoGradArgs& a, hipStream_t s) {
     if (a.nrot <= 0) return;
-    hipLaunchKernelGGL(k_oo_grad_hess, dim3((a.nrot + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_oo_grad_hess, dim3((a.nrot + 3) / 4), dim3(256), 0, s, a);
 }
 
 void launch_oo_precondition(const double* g, const double* H, int nrot, double r, double* kref, double* kcur,

@@ -13,9 +13,18 @@ namespace v2h {
 // C[b] = alpha op(A[b]) op(B[b]) + beta C[b], row-major, b < batch; op(A) is
 // M x K (A stored K x M with lda when ta), op(B) is K x N (B stored N x K
 // when tb).  Strides sA / sB / sC between batch entries (0: shared).
+// dg is added on the diagonal of every result (m == n).
 void launch_dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda, int64_t sA,
                   const double* B, int64_t ldb, int64_t sB, double beta, double* C, int64_t ldc, int64_t sC,
-                  int batch, hipStream_t s);
+                  int batch, hipStream_t s, double dg = 0.0);
+// One product with K split over up to kMaxSlices workgroup slices (partial
+// products in work, [slices][M][N], then summed in slice order): for a small
+// M x N with a long K.
+constexpr int kMaxSlices = 64;
+int dgemm_split_slices(int M, int N, int K);
+void launch_dgemm_split(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda,
+                        const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* work,
+                        hipStream_t s);
 
 // dense [nq][nfull][nfull] <-> packed lower triangle [nq][nfull (nfull + 1) / 2]
 void launch_df_unpack(const double* qmo, int nq, int nfull, double* bd, hipStream_t s);

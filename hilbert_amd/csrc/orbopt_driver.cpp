@@ -106,7 +106,7 @@ void Batch::oo_setup(int i, const OrbOptArgs& a) {
     if (!S.bd.p) {
         S.bd.alloc((size_t)nq * nn);
         S.tmp.alloc((size_t)nq * nn);
-        for (auto* b : {&S.uscr, &S.hscr, &S.K, &S.U, &S.T1, &S.T2}) b->alloc(nn);
+        for (auto* b : {&S.uscr, &S.hscr, &S.K, &S.U, &S.T1}) b->alloc(nn);
         S.D.alloc(n2);
         S.d.alloc(n2 * n2);
         S.gO.alloc(n2 * n2);
@@ -115,6 +115,7 @@ void Batch::oo_setup(int i, const OrbOptArgs& a) {
         S.Bdiag.alloc((size_t)nfull * nq);
         S.Z.alloc(n2 * nq);
         S.F.alloc((size_t)no * nfull);
+        S.Fw.alloc((size_t)kMaxSlices * no * nfull);
         S.Jd.alloc((size_t)nfull * n2);
         S.Kd.alloc((size_t)nfull * n2);
         S.red.alloc(2);
@@ -129,8 +130,8 @@ void Batch::oo_gradient(int i) {
     launch_oo_gather(S.bd.p, nq, nfull, S.occ.p, no, S.BO.p, S.BFO.p, S.Bdiag.p, stream_);
     launch_dgemm(true, false, n2, n2, nq, 1.0, S.BO.p, n2, 0, S.BO.p, n2, 0, 0.0, S.gO.p, n2, 0, 1, stream_);
     launch_dgemm(false, true, n2, nq, n2, 1.0, S.d.p, n2, 0, S.BO.p, n2, 0, 0.0, S.Z.p, nq, 0, 1, stream_);
-    launch_dgemm(false, true, no, nfull, no * nq, 1.0, S.Z.p, (int64_t)no * nq, 0, S.BFO.p, (int64_t)no * nq, 0, 0.0,
-                 S.F.p, nfull, 0, 1, stream_);
+    launch_dgemm_split(false, true, no, nfull, no * nq, 1.0, S.Z.p, (int64_t)no * nq, S.BFO.p, (int64_t)no * nq, 0.0,
+                       S.F.p, nfull, S.Fw.p, stream_);
     launch_oo_fock1(S.D.p, S.h.p, S.occ.p, no, nfull, S.F.p, stream_);
     launch_dgemm(false, false, nfull, n2, nq, 1.0, S.Bdiag.p, nq, 0, S.BO.p, n2, 0, 0.0, S.Jd.p, n2, 0, 1, stream_);
     launch_dgemm(false, true, no, no, nq, 1.0, S.BFO.p, nq, (int64_t)no * nq, S.BFO.p, nq, (int64_t)no * nq, 0.0,
@@ -170,18 +171,17 @@ double Batch::oo_apply(int i, const double* kappa, double sign) {
     while (sq < 60 && nrm / std::ldexp(1.0, sq) > 0.5) sq++;
     tstart(kKindOrb);
     if (sq > 0) launch_oo_axpy(std::ldexp(1.0, -sq) - 1.0, S.K.p, S.K.p, nn, stream_);
-    // exp(K) = sum_k K^k / k! on the scaled K, then squared back
+    // exp(K) on the scaled K: the Taylor polynomial of degree 14 by Horner's
+    // rule, T <- I + K T / k for k = 14 .. 1 (the identity added in the
+    // product's epilogue), then squared back
     double* U = S.U.p;
     double* T1 = S.T1.p;
-    double* T2 = S.T2.p;
     launch_oo_identity(U, nfull, stream_);
-    launch_oo_identity(T1, nfull, stream_);
     constexpr int kTaylor = 14;
-    for (int k = 1; k <= kTaylor; k++) {
-        launch_dgemm(false, false, nfull, nfull, nfull, 1.0 / k, T1, nfull, 0, S.K.p, nfull, 0, 0.0, T2, nfull, 0, 1,
-                     stream_);
-        launch_oo_axpy(1.0, T2, U, nn, stream_);
-        std::swap(T1, T2);
+    for (int k = kTaylor; k >= 1; k--) {
+        launch_dgemm(false, false, nfull, nfull, nfull, 1.0 / k, S.K.p, nfull, 0, U, nfull, 0, 0.0, T1, nfull, 0, 1,
+                     stream_, 1.0);
+        std::swap(U, T1);
     }
     for (int k = 0; k < sq; k++) {
         launch_dgemm(false, false, nfull, nfull, nfull, 1.0, U, nfull, 0, U, nfull, 0, 0.0, T1, nfull, 0, 1, stream_);
@@ -220,9 +220,10 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
 
     bool eval_G = true, conv_e = false, conv_g = false;
     double r = 1.0, E_init = E_cur, initial_E = E_cur, dE_quad = 0.0, dE = 0.0, gnorm = 0.0;
-    int iters = 0;
+    int iters = 0, evals = 0, applies = 0;
     for (int it = 0; it < a.maxiter; it++) {
         if (eval_G) {
+            evals++;
             oo_gradient(i);
             launch_oo_precondition(S.g.p, S.H.p, S.nrot, r, S.kref.p, S.kcur.p, S.ksave.p, S.red.p, stream_);
             double out[2] = {0.0, 0.0};
@@ -234,6 +235,7 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
             if (it == 0) initial_E = E_init;
         }
         E_cur = oo_apply(i, S.kcur.p, 1.0);
+        applies++;
         dE = E_cur - E_init;
         if (gnorm < a.g_convergence) conv_g = true;
         if (std::fabs(dE) < a.e_convergence) conv_e = true;
@@ -251,8 +253,14 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
         if (!eval_G) launch_oo_update_kappa(S.kref.p, S.nrot, r, S.kcur.p, S.ksave.p, stream_);
     }
     if (dE > 0.0) {
-        if (conv_e || conv_g) E_cur = oo_apply(i, S.kcur.p, -1.0);
-        if (!eval_G) E_cur = oo_apply(i, S.ksave.p, -1.0);
+        if (conv_e || conv_g) {
+            E_cur = oo_apply(i, S.kcur.p, -1.0);
+            applies++;
+        }
+        if (!eval_G) {
+            E_cur = oo_apply(i, S.ksave.p, -1.0);
+            applies++;
+        }
     }
     // the cost vector changed: unless the orbitals are converged (a change
     // below the thresholds), the SDP is not converged for it yet
@@ -265,6 +273,8 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
         res->iterations = iters;
         res->converged = (conv_e && conv_g) ? 1 : 0;
         res->nrot = S.nrot;
+        res->evaluations = evals;
+        res->rotations = applies;
         res->gnorm = gnorm;
         res->dE = E_cur - initial_E;
         res->energy = E_cur;

@@ -80,7 +80,7 @@ struct DfState {
     DBuf<int> occ, pos;
     DBuf<int4> ptab;
     DBuf<int64_t> d1tab;
-    DBuf<double> D, d, BO, BFO, Bdiag, gO, Z, F, Jd, Kd, g, H, kref, kcur, ksave, K, U, T1, T2, red;
+    DBuf<double> D, d, BO, BFO, Bdiag, gO, Z, F, Fw, Jd, Kd, g, H, kref, kcur, ksave, K, U, T1, red;
 };
 
 // Options and results of one orbital-optimisation call (the ORBOPT_* options
@@ -92,7 +92,7 @@ struct OrbOptArgs {
     int active_active = 0;
 };
 struct OrbOptResult {
-    int iterations = 0, converged = 0, nrot = 0;
+    int iterations = 0, converged = 0, nrot = 0, evaluations = 0, rotations = 0;
     double gnorm = 0.0, dE = 0.0, energy = 0.0, efzc = 0.0;
 };
 

@@ -175,6 +175,7 @@ typedef struct v2h_orbopt_options {
 } v2h_orbopt_options;
 typedef struct v2h_orbopt_result {
     int iterations, converged, nrot;
+    int evaluations, rotations; /* gradient evaluations; rotations applied (steps and undos) */
     double gnorm, dE, energy, efzc;
 } v2h_orbopt_result;
 int v2h_batch_optimize_orbitals(v2h_batch *b, int inst, const int *full_irreps, const v2h_orbopt_options *opt,

@@ -228,17 +228,17 @@ def kappa_matrix(kappa, pairs, nfull):
 
 
 def expm(K, order=14):
-    """exp(K) by scaling to ||K||_1 <= 1/2, a Taylor series and squaring."""
+    """exp(K) by scaling to ||K||_1 <= 1/2, the Taylor polynomial of degree
+    14 by Horner's rule and squaring."""
     nrm = np.abs(K).sum(axis=0).max() if K.size else 0.0
     s = 0
     while nrm / (1 << s) > 0.5:
         s += 1
     P = K / float(1 << s)
-    U = np.eye(K.shape[0])
-    T = np.eye(K.shape[0])
-    for k in range(1, order + 1):
-        T = T @ P / k
-        U = U + T
+    eye = np.eye(K.shape[0])
+    U = eye.copy()
+    for k in range(order, 0, -1):
+        U = eye + (P @ U) / k
     for _ in range(s):
         U = U @ U
     return U
