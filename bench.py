@@ -74,10 +74,13 @@ def project_flops(dims):
     return sum(16.0 / 3.0 * float(d) ** 3 for d in dims if d > 0)
 
 
+MU_FREQ = [1000]   # MU_UPDATE_FREQUENCY (hilbert's default 1000)
+
+
 def make_batch(items, maxiter):
     from hilbert_amd.batch import Batch, options
     B = Batch([p for p, _, _ in items],
-              options(e_convergence=E_CONV, r_convergence=R_CONV, maxiter=maxiter))
+              options(e_convergence=E_CONV, r_convergence=R_CONV, maxiter=maxiter, mu_update_frequency=MU_FREQ[0]))
     for i, (_, h, g) in enumerate(items):
         B.set_integrals(i, h, g)
     return B
@@ -540,6 +543,9 @@ def main():
                     help="sdp workload: the N-representability conditions")
     ap.add_argument("--d3", type=int, default=0, help="sdp workload: add the D3 condition (CONSTRAIN_D3)")
     ap.add_argument("--chunk", type=int, default=1000, help="outer iterations between progress lines")
+    ap.add_argument("--mu-update-frequency", type=int, default=1000,
+                    help="sdp workload: MU_UPDATE_FREQUENCY, outer iterations between mu rebalancings "
+                         "(hilbert's default 1000)")
     ap.add_argument("--casscf", type=int, default=1,
                     help="orbopt workload: also run the whole v2RDM-CASSCF loop once (0 = skip)")
     ap.add_argument("--workload", choices=("sdp", "df", "fcidump", "orbopt"), default="sdp",
@@ -547,6 +553,7 @@ def main():
                          "fcidump: the FCIDUMP writer; orbopt: one v2RDM-CASSCF orbital step")
     args = ap.parse_args()
     PROGRESS_CHUNK[0] = max(1, args.chunk)
+    MU_FREQ[0] = args.mu_update_frequency
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
               "communication, and multi-GPU runs are not part of this build.", file=sys.stderr)
@@ -653,6 +660,7 @@ def main():
                                "one batched solve to convergence per step",
                    "batch": args.batch, "nmo": n, "electrons": n, "nirrep": 4,
                    "e_convergence": E_CONV, "r_convergence": R_CONV, "positivity": args.positivity,
+                   "mu_update_frequency": MU_FREQ[0],
                    "constrain_d3": bool(args.d3), "constrain_spin": True, "constrain_sz": True},
         "converged": sum(conv), "attempted": args.batch * args.steps,
         "mean_outer_iterations": mean_oiter,

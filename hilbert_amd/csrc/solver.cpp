@@ -456,8 +456,15 @@ double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, c
         for (int c2 = 0; c2 < ncore; c2++) idx.push_back(tri(core[c], core[c2]));
     const int64_t ntri = (int64_t)nfull * (nfull + 1) / 2;
     if (df_.size() != (size_t)count()) df_.resize(count());
-    df_[i].reset(new DfState());
+    // same shape as the resident state: keep its buffers (the orbital step's
+    // workspace included); the orbital lists may differ, so the rotation
+    // pairs and density tables are rebuilt by the next orbital call
+    const bool reuse = df_[i] && df_[i]->nfull == nfull && df_[i]->nq == nq && df_[i]->n == n &&
+                       df_[i]->ncore == ncore;
+    if (!reuse) df_[i].reset(new DfState());
     DfState& D = *df_[i];
+    D.nrot = -1;
+    D.pair_key.clear();
     D.nfull = nfull;
     D.nq = nq;
     D.n = n;
@@ -465,20 +472,22 @@ double Batch::set_df_integrals(int i, int nfull, const double* h_full, int nq, c
     D.no = ncore + n;
     D.core.assign(core, core + ncore);
     D.active.assign(active, active + n);
-    D.qmo.alloc((size_t)nq * ntri);
+    if (!reuse) D.qmo.alloc((size_t)nq * ntri);
     V2H_HIP(hipMemcpyAsync(D.qmo.p, qmo, sizeof(double) * nq * ntri, hipMemcpyHostToDevice, stream_));
-    D.h.alloc((size_t)nfull * nfull);
+    if (!reuse) D.h.alloc((size_t)nfull * nfull);
     V2H_HIP(hipMemcpyAsync(D.h.p, h_full, sizeof(double) * nfull * nfull, hipMemcpyHostToDevice, stream_));
     D.idx.upload(idx);
     D.act.upload(std::vector<int>(active, active + n));
     D.cor.upload(ncore > 0 ? std::vector<int>(core, core + ncore) : std::vector<int>{0});
     D.m_rows = (int64_t)idx.size();
-    D.rows.alloc(idx.size() * (size_t)nq);
-    D.dvec.alloc(nq);
-    D.gu.alloc((size_t)kDfMaxSplits * nu * nu);
-    D.sq.alloc(std::max<size_t>(1, (size_t)ncore * ncore));
-    D.e.alloc(1);
-    D.uacc.alloc((size_t)nfull * nfull);
+    if (!reuse) {
+        D.rows.alloc(idx.size() * (size_t)nq);
+        D.dvec.alloc(nq);
+        D.gu.alloc((size_t)kDfMaxSplits * nu * nu);
+        D.sq.alloc(std::max<size_t>(1, (size_t)ncore * ncore));
+        D.e.alloc(1);
+        D.uacc.alloc((size_t)nfull * nfull);
+    }
     launch_oo_identity(D.uacc.p, nfull, stream_);
     return run_df_intake(i);
 }
