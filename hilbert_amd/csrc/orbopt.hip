@@ -102,6 +102,36 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, double alpha
     }
 }
 
+// Small products (M, N, K <= kSmallMax): a 16 x 16 output tile per
+// workgroup, one output per thread, with the tile's rows of op(A) and columns
+// of B staged whole in LDS (no K loop of global loads): for the chains of
+// 100 x 100 products of the exponential, which are latency-bound.
+constexpr int kSmallMax = 128;
+
+template <bool TA>
+__global__ __launch_bounds__(256) void k_dgemm_small(int M, int N, int K, double alpha, const double* __restrict__ A,
+                                                     int64_t lda, const double* __restrict__ B, int64_t ldb,
+                                                     double* __restrict__ C, int64_t ldc, double dg) {
+    __shared__ double As[16][kSmallMax + 1];
+    __shared__ double Bs[kSmallMax][17];
+    const int m0 = blockIdx.y * 16, n0 = blockIdx.x * 16;
+    const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+    for (int e = t; e < 16 * K; e += 256) {
+        int m, k;
+        if (!TA) { m = e / K; k = e - m * K; } else { k = e >> 4; m = e & 15; }
+        As[m][k] = (m0 + m < M) ? (TA ? A[(int64_t)k * lda + m0 + m] : A[(int64_t)(m0 + m) * lda + k]) : 0.0;
+    }
+    for (int e = t; e < 16 * K; e += 256) {
+        const int k = e >> 4, n = e & 15;
+        Bs[k][n] = (n0 + n < N) ? B[(int64_t)k * ldb + n0 + n] : 0.0;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int k = 0; k < K; k++) acc = __fma_rn(As[ty][k], Bs[k][tx], acc);
+    const int m = m0 + ty, n = n0 + tx;
+    if (m < M && n < N) C[(int64_t)m * ldc + n] = alpha * acc + (m == n ? dg : 0.0);
+}
+
 // C[m][n] = alpha sum_s W[s][m][n] + beta C[m][n], slices summed in order
 __global__ __launch_bounds__(256) void k_dgemm_slices(const double* __restrict__ W, int slices, int M, int N,
                                                       double alpha, double beta, double* __restrict__ C, int64_t ldc) {
@@ -427,8 +457,20 @@ void launch_dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const dou
 int dgemm_split_slices(int M, int N, int K) {
     const int64_t tiles = (int64_t)((M + kT - 1) / kT) * ((N + kT - 1) / kT);
     int sl = (int)std::min<int64_t>(kMaxSlices, std::max<int64_t>(1, 512 / std::max<int64_t>(1, tiles)));
-    sl = std::min(sl, std::max(1, K / 256));
+    sl = std::min(sl, std::max(1, K / 64));
     return sl;
+}
+
+void launch_dgemm_small(bool ta, int M, int N, int K, double alpha, const double* A, int64_t lda, const double* B,
+                        int64_t ldb, double* C, int64_t ldc, hipStream_t s, double dg) {
+    if (M <= 0 || N <= 0) return;
+    if (M > kSmallMax || N > kSmallMax || K > kSmallMax || K < 1) {
+        launch_dgemm(ta, false, M, N, K, alpha, A, lda, 0, B, ldb, 0, 0.0, C, ldc, 0, 1, s, dg);
+        return;
+    }
+    const dim3 grid((unsigned)((N + 15) / 16), (unsigned)((M + 15) / 16));
+    if (ta) hipLaunchKernelGGL((k_dgemm_small<true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, C, ldc, dg);
+    else hipLaunchKernelGGL((k_dgemm_small<false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, B, ldb, C, ldc, dg);
 }
 
 void launch_dgemm_split(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda,

@@ -21,6 +21,10 @@ void launch_dgemm(bool ta, bool tb, int M, int N, int K, double alpha, const dou
 // products in work, [slices][M][N], then summed in slice order): for a small
 // M x N with a long K.
 constexpr int kMaxSlices = 64;
+// C = alpha op(A) B (+ dg on the diagonal), beta = 0, one product; the LDS
+// kernel when M, N, K <= 128, else launch_dgemm.
+void launch_dgemm_small(bool ta, int M, int N, int K, double alpha, const double* A, int64_t lda, const double* B,
+                        int64_t ldb, double* C, int64_t ldc, hipStream_t s, double dg = 0.0);
 int dgemm_split_slices(int M, int N, int K);
 void launch_dgemm_split(bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int64_t lda,
                         const double* B, int64_t ldb, double beta, double* C, int64_t ldc, double* work,

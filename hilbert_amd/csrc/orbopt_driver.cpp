@@ -116,6 +116,7 @@ void Batch::oo_setup(int i, const OrbOptArgs& a) {
         S.Z.alloc(n2 * nq);
         S.F.alloc((size_t)no * nfull);
         S.Fw.alloc((size_t)kMaxSlices * no * nfull);
+        S.Jw.alloc((size_t)dgemm_split_slices(nfull, (int)n2, nq) * nfull * n2);
         S.Jd.alloc((size_t)nfull * n2);
         S.Kd.alloc((size_t)nfull * n2);
         S.red.alloc(2);
@@ -133,7 +134,7 @@ void Batch::oo_gradient(int i) {
     launch_dgemm_split(false, true, no, nfull, no * nq, 1.0, S.Z.p, (int64_t)no * nq, S.BFO.p, (int64_t)no * nq, 0.0,
                        S.F.p, nfull, S.Fw.p, stream_);
     launch_oo_fock1(S.D.p, S.h.p, S.occ.p, no, nfull, S.F.p, stream_);
-    launch_dgemm(false, false, nfull, n2, nq, 1.0, S.Bdiag.p, nq, 0, S.BO.p, n2, 0, 0.0, S.Jd.p, n2, 0, 1, stream_);
+    launch_dgemm_split(false, false, nfull, n2, nq, 1.0, S.Bdiag.p, nq, S.BO.p, n2, 0.0, S.Jd.p, n2, S.Jw.p, stream_);
     launch_dgemm(false, true, no, no, nq, 1.0, S.BFO.p, nq, (int64_t)no * nq, S.BFO.p, nq, (int64_t)no * nq, 0.0,
                  S.Kd.p, no, n2, nfull, stream_);
     OoGradArgs ga{};
@@ -179,12 +180,11 @@ double Batch::oo_apply(int i, const double* kappa, double sign) {
     launch_oo_identity(U, nfull, stream_);
     constexpr int kTaylor = 14;
     for (int k = kTaylor; k >= 1; k--) {
-        launch_dgemm(false, false, nfull, nfull, nfull, 1.0 / k, S.K.p, nfull, 0, U, nfull, 0, 0.0, T1, nfull, 0, 1,
-                     stream_, 1.0);
+        launch_dgemm_small(false, nfull, nfull, nfull, 1.0 / k, S.K.p, nfull, U, nfull, T1, nfull, stream_, 1.0);
         std::swap(U, T1);
     }
     for (int k = 0; k < sq; k++) {
-        launch_dgemm(false, false, nfull, nfull, nfull, 1.0, U, nfull, 0, U, nfull, 0, 0.0, T1, nfull, 0, 1, stream_);
+        launch_dgemm_small(false, nfull, nfull, nfull, 1.0, U, nfull, U, nfull, T1, nfull, stream_);
         std::swap(U, T1);
     }
     // B^Q <- U^T B^Q U, h <- U^T h U, Uacc <- Uacc U
@@ -192,17 +192,14 @@ double Batch::oo_apply(int i, const double* kappa, double sign) {
                  stream_);
     launch_dgemm(true, false, nfull, nfull, nfull, 1.0, U, nfull, 0, S.tmp.p, nfull, nn, 0.0, S.bd.p, nfull, nn, nq,
                  stream_);
-    launch_dgemm(false, false, nfull, nfull, nfull, 1.0, S.h.p, nfull, 0, U, nfull, 0, 0.0, S.hscr.p, nfull, 0, 1,
-                 stream_);
-    launch_dgemm(true, false, nfull, nfull, nfull, 1.0, U, nfull, 0, S.hscr.p, nfull, 0, 0.0, S.h.p, nfull, 0, 1,
-                 stream_);
-    launch_dgemm(false, false, nfull, nfull, nfull, 1.0, S.uacc.p, nfull, 0, U, nfull, 0, 0.0, S.uscr.p, nfull, 0, 1,
-                 stream_);
+    launch_dgemm_small(false, nfull, nfull, nfull, 1.0, S.h.p, nfull, U, nfull, S.hscr.p, nfull, stream_);
+    launch_dgemm_small(true, nfull, nfull, nfull, 1.0, U, nfull, S.hscr.p, nfull, S.h.p, nfull, stream_);
+    launch_dgemm_small(false, nfull, nfull, nfull, 1.0, S.uacc.p, nfull, U, nfull, S.uscr.p, nfull, stream_);
     std::swap(S.uacc.p, S.uscr.p);
     launch_df_pack(S.bd.p, nq, nfull, S.qmo.p, stream_);
     tstop(kKindOrb);
     V2H_HIP(hipGetLastError());
-    run_df_intake(i);
+    run_df_intake(i, false);  // ||c|| once, at the end of the call
     return instance_energy(i);
 }
 
@@ -215,7 +212,7 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
     launch_df_unpack(S.qmo.p, S.nq, S.nfull, S.bd.p, stream_);
     launch_oo_density(S.tabs, x_.p + I.poff, S.D.p, S.d.p, stream_);
     for (auto* b : {&S.kref, &S.kcur, &S.ksave}) launch_fill(b->p, nr, 0.0, stream_);
-    run_df_intake(i);
+    run_df_intake(i, false);
     double E_cur = instance_energy(i);
 
     bool eval_G = true, conv_e = false, conv_g = false;
@@ -262,8 +259,10 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
             applies++;
         }
     }
-    // the cost vector changed: unless the orbitals are converged (a change
-    // below the thresholds), the SDP is not converged for it yet
+    // the cost vector changed: its norm, and unless the orbitals are
+    // converged (a change below the thresholds), the SDP is not converged
+    // for it yet
+    refresh_norms(i);
     if (!(conv_e && conv_g)) {
         sync_state_to_host();
         state_[i].converged = 0;

@@ -498,7 +498,7 @@ DfState& Batch::df_state(int i) {
     return *df_[i];
 }
 
-double Batch::run_df_intake(int i) {
+double Batch::run_df_intake(int i, bool norms) {
     const InstDev& I = inst_[i];
     const Structure& S = *structs_[inst_sid_[i]];
     DfState& D = df_state(i);
@@ -529,7 +529,7 @@ double Batch::run_df_intake(int i) {
     V2H_HIP(hipGetLastError());
     V2H_HIP(hipMemcpyAsync(&D.efzc, D.e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
     V2H_HIP(hipStreamSynchronize(stream_));
-    refresh_norms(i);
+    if (norms) refresh_norms(i);
     return D.efzc;
 }
 

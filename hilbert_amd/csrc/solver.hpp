@@ -80,7 +80,7 @@ struct DfState {
     DBuf<int> occ, pos;
     DBuf<int4> ptab;
     DBuf<int64_t> d1tab;
-    DBuf<double> D, d, BO, BFO, Bdiag, gO, Z, F, Fw, Jd, Kd, g, H, kref, kcur, ksave, K, U, T1, red;
+    DBuf<double> D, d, BO, BFO, Bdiag, gO, Z, F, Fw, Jw, Jd, Kd, g, H, kref, kcur, ksave, K, U, T1, red;
 };
 
 // Options and results of one orbital-optimisation call (the ORBOPT_* options
@@ -201,7 +201,7 @@ class Batch {
     void sync_state_to_device();
     void refresh_norms(int i);
     DfState& df_state(int i);
-    double run_df_intake(int i);          // resident DF integrals -> c, v1, E_fzc
+    double run_df_intake(int i, bool norms = true);  // resident DF integrals -> c, v1, E_fzc (and ||c||)
     double instance_energy(int i);        // c.x + E_fzc
     void oo_setup(int i, const OrbOptArgs& a);
     void oo_gradient(int i);
