@@ -440,7 +440,7 @@ def bench_orbopt(args):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "note": "GEMM flops of the call (transforms of 400 slices dominate) over the HIP-event "
-                             "time of its device work outside the intakes; k_dgemm is a vector-FMA kernel"},
+                             "time of its device work outside the intakes (k_dgemm_mfma on the FP64 matrix cores, k_dgemm_small)"},
         "cpu_baseline": cpu,
         "host": host_info(),
     }

@@ -30,6 +30,10 @@ namespace {
 
 constexpr int kT = 64;    // output tile
 constexpr int kKc = 16;   // K chunk staged in LDS
+#ifndef V2H_DGEMM_MFMA
+#define V2H_DGEMM_MFMA 1
+#endif
+constexpr bool kUseMfma = V2H_DGEMM_MFMA != 0;  // tiled products on the matrix cores
 
 // kc > 0: split K; batch entry bz covers k in [bz kc, min(K, (bz + 1) kc)) and
 // writes its partial product to C + bz sC (summed by k_dgemm_slices).
@@ -100,6 +104,79 @@ __global__ __launch_bounds__(256) void k_dgemm(int M, int N, int K, double alpha
             *c = v;
         }
     }
+}
+
+// The same product on the FP64 matrix cores: four wavefronts per 64 x 64
+// tile, each a 32 x 32 quarter as 2 x 2 MFMA 16 x 16 x 4 tiles, operands read
+// from the same LDS staging.  v_mfma_f64_16x16x4f64 lane layout (checked on
+// gfx950): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
+// D[4 r + l / 16][l % 16] in its register r.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_dgemm_mfma(int M, int N, int K, double alpha, const double* __restrict__ A,
+                                                    int64_t lda, int64_t sA, const double* __restrict__ B,
+                                                    int64_t ldb, int64_t sB, double beta, double* __restrict__ C,
+                                                    int64_t ldc, int64_t sC, int kc, double dg) {
+    __shared__ double As[kKc][kT + 1];
+    __shared__ double Bs[kKc][kT + 1];
+    const int64_t bz = blockIdx.z;
+    A += bz * sA;
+    B += bz * sB;
+    C += bz * sC;
+    const int kbeg = kc > 0 ? (int)bz * kc : 0;
+    const int kend = kc > 0 ? min(K, kbeg + kc) : K;
+    const int m0 = blockIdx.y * kT, n0 = blockIdx.x * kT;
+    const int t = threadIdx.x, l = t & 63, w = t >> 6;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const int li = l & 15, lk = l >> 4;
+    v4d acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kbeg; k0 < kend; k0 += kKc) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int idx = t + 256 * e;
+            int m, k;
+            if (!TA) { m = idx >> 4; k = idx & 15; } else { k = idx >> 6; m = idx & 63; }
+            double v = 0.0;
+            if (m0 + m < M && k0 + k < kend)
+                v = TA ? A[(int64_t)(k0 + k) * lda + (m0 + m)] : A[(int64_t)(m0 + m) * lda + (k0 + k)];
+            As[k][m] = v;
+            int n, k2;
+            if (!TB) { k2 = idx >> 6; n = idx & 63; } else { n = idx >> 4; k2 = idx & 15; }
+            double u = 0.0;
+            if (n0 + n < N && k0 + k2 < kend)
+                u = TB ? B[(int64_t)(n0 + n) * ldb + (k0 + k2)] : B[(int64_t)(k0 + k2) * ldb + (n0 + n)];
+            Bs[k2][n] = u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kKc; kk += 4) {
+            const double a0 = As[kk + lk][wm + li], a1 = As[kk + lk][wm + 16 + li];
+            const double b0 = Bs[kk + lk][wn + li], b1 = Bs[kk + lk][wn + 16 + li];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * i + 4 * r + lk, n = n0 + wn + 16 * j + li;
+                if (m >= M || n >= N) continue;
+                double* c = C + (int64_t)m * ldc + n;
+                double v = beta != 0.0 ? __fma_rn(alpha, acc[i][j][r], beta * *c) : alpha * acc[i][j][r];
+                if (m == n) v += dg;
+                *c = v;
+            }
 }
 
 // Small products (M, N, K <= kSmallMax): a 16 x 16 output tile per
@@ -440,6 +517,13 @@ void dgemm_raw(bool ta, bool tb, int M, int N, int K, double alpha, const double
         throw std::invalid_argument("launch_dgemm: shape " + std::to_string(M) + " x " + std::to_string(N) +
                                     " x batch " + std::to_string(batch) + " exceeds the grid");
     const dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)batch);
+    if (kUseMfma) {
+        if (!ta && !tb) hipLaunchKernelGGL((k_dgemm_mfma<false, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+        else if (ta && !tb) hipLaunchKernelGGL((k_dgemm_mfma<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+        else if (!ta && tb) hipLaunchKernelGGL((k_dgemm_mfma<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+        else hipLaunchKernelGGL((k_dgemm_mfma<true, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
+        return;
+    }
     if (!ta && !tb) hipLaunchKernelGGL((k_dgemm<false, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
     else if (ta && !tb) hipLaunchKernelGGL((k_dgemm<true, false>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);
     else if (!ta && tb) hipLaunchKernelGGL((k_dgemm<false, true>), grid, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, kc, dg);

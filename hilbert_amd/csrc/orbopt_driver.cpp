@@ -117,6 +117,8 @@ void Batch::oo_setup(int i, const OrbOptArgs& a) {
         S.F.alloc((size_t)no * nfull);
         S.Fw.alloc((size_t)kMaxSlices * no * nfull);
         S.Jw.alloc((size_t)dgemm_split_slices(nfull, (int)n2, nq) * nfull * n2);
+        S.Gw.alloc((size_t)dgemm_split_slices((int)n2, (int)n2, nq) * n2 * n2);
+        S.Zw.alloc((size_t)dgemm_split_slices((int)n2, nq, (int)n2) * n2 * nq);
         S.Jd.alloc((size_t)nfull * n2);
         S.Kd.alloc((size_t)nfull * n2);
         S.red.alloc(2);
@@ -129,8 +131,8 @@ void Batch::oo_gradient(int i) {
     const int nfull = S.nfull, no = S.no, nq = S.nq, n2 = S.no * S.no;
     tstart(kKindOrb);
     launch_oo_gather(S.bd.p, nq, nfull, S.occ.p, no, S.BO.p, S.BFO.p, S.Bdiag.p, stream_);
-    launch_dgemm(true, false, n2, n2, nq, 1.0, S.BO.p, n2, 0, S.BO.p, n2, 0, 0.0, S.gO.p, n2, 0, 1, stream_);
-    launch_dgemm(false, true, n2, nq, n2, 1.0, S.d.p, n2, 0, S.BO.p, n2, 0, 0.0, S.Z.p, nq, 0, 1, stream_);
+    launch_dgemm_split(true, false, n2, n2, nq, 1.0, S.BO.p, n2, S.BO.p, n2, 0.0, S.gO.p, n2, S.Gw.p, stream_);
+    launch_dgemm_split(false, true, n2, nq, n2, 1.0, S.d.p, n2, S.BO.p, n2, 0.0, S.Z.p, nq, S.Zw.p, stream_);
     launch_dgemm_split(false, true, no, nfull, no * nq, 1.0, S.Z.p, (int64_t)no * nq, S.BFO.p, (int64_t)no * nq, 0.0,
                        S.F.p, nfull, S.Fw.p, stream_);
     launch_oo_fock1(S.D.p, S.h.p, S.occ.p, no, nfull, S.F.p, stream_);

@@ -80,7 +80,7 @@ struct DfState {
     DBuf<int> occ, pos;
     DBuf<int4> ptab;
     DBuf<int64_t> d1tab;
-    DBuf<double> D, d, BO, BFO, Bdiag, gO, Z, F, Fw, Jw, Jd, Kd, g, H, kref, kcur, ksave, K, U, T1, red;
+    DBuf<double> D, d, BO, BFO, Bdiag, gO, Gw, Z, Zw, F, Fw, Jw, Jd, Kd, g, H, kref, kcur, ksave, K, U, T1, red;
 };
 
 // Options and results of one orbital-optimisation call (the ORBOPT_* options
