@@ -121,7 +121,8 @@ void Batch::oo_setup(int i, const OrbOptArgs& a) {
         S.Zw.alloc((size_t)dgemm_split_slices((int)n2, nq, (int)n2) * n2 * nq);
         S.Jd.alloc((size_t)nfull * n2);
         S.Kd.alloc((size_t)nfull * n2);
-        S.red.alloc(2);
+        S.red.alloc(3);
+        if (!S.pin) V2H_HIP(hipHostMalloc((void**)&S.pin, 8 * sizeof(double), hipHostMallocDefault));
     }
     for (auto* b : {&S.g, &S.H, &S.kref, &S.kcur, &S.ksave}) b->alloc(nr);
 }
@@ -164,11 +165,13 @@ double Batch::oo_apply(int i, const double* kappa, double sign) {
     const int nfull = S.nfull, nq = S.nq;
     const int64_t nn = (int64_t)nfull * nfull;
     tstart(kKindOrb);
-    launch_oo_kappa_matrix(kappa, S.pairs.p, S.nrot, sign, nfull, S.K.p, S.red.p, stream_);
+    launch_oo_kappa_matrix(kappa, S.pairs.p, S.nrot, sign, nfull, S.K.p, S.red.p + 2, stream_);
     tstop(kKindOrb);
-    double nrm = 0.0;
-    V2H_HIP(hipMemcpyAsync(&nrm, S.red.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    // one sync: ||K||_1, and with it the step model copied after the last
+    // preconditioning
+    V2H_HIP(hipMemcpyAsync(S.pin + 2, S.red.p + 2, sizeof(double), hipMemcpyDeviceToHost, stream_));
     V2H_HIP(hipStreamSynchronize(stream_));
+    const double nrm = S.pin[2];
     if (!std::isfinite(nrm)) throw std::runtime_error("optimize_orbitals: non-finite rotation");
     int sq = 0;
     while (sq < 60 && nrm / std::ldexp(1.0, sq) > 0.5) sq++;
@@ -201,8 +204,16 @@ double Batch::oo_apply(int i, const double* kappa, double sign) {
     launch_df_pack(S.bd.p, nq, nfull, S.qmo.p, stream_);
     tstop(kKindOrb);
     V2H_HIP(hipGetLastError());
-    run_df_intake(i, false);  // ||c|| once, at the end of the call
-    return instance_energy(i);
+    // the intake (||c|| once, at the end of the call) and E = c.x + E_fzc,
+    // read back together
+    launch_df_intake_only(i);
+    const InstDev& I = inst_[i];
+    launch_dot1(c_.p + I.poff, x_.p + I.poff, n_primal(i), scratch_.p, stream_);
+    V2H_HIP(hipMemcpyAsync(S.pin + 3, S.e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipMemcpyAsync(S.pin + 4, scratch_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
+    S.efzc = S.pin[3];
+    return S.pin[4] + S.efzc;
 }
 
 void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
@@ -225,15 +236,16 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
             evals++;
             oo_gradient(i);
             launch_oo_precondition(S.g.p, S.H.p, S.nrot, r, S.kref.p, S.kcur.p, S.ksave.p, S.red.p, stream_);
-            double out[2] = {0.0, 0.0};
-            V2H_HIP(hipMemcpyAsync(out, S.red.p, sizeof(out), hipMemcpyDeviceToHost, stream_));
-            V2H_HIP(hipStreamSynchronize(stream_));
-            gnorm = std::sqrt(out[0]);
-            dE_quad = out[1];
+            // read back with the next sync (in oo_apply, before anything uses them)
+            V2H_HIP(hipMemcpyAsync(S.pin, S.red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
             E_init = E_cur;
             if (it == 0) initial_E = E_init;
         }
         E_cur = oo_apply(i, S.kcur.p, 1.0);
+        if (eval_G) {
+            gnorm = std::sqrt(S.pin[0]);
+            dE_quad = S.pin[1];
+        }
         applies++;
         dE = E_cur - E_init;
         if (gnorm < a.g_convergence) conv_g = true;

@@ -499,6 +499,15 @@ DfState& Batch::df_state(int i) {
 }
 
 double Batch::run_df_intake(int i, bool norms) {
+    launch_df_intake_only(i);
+    DfState& D = df_state(i);
+    V2H_HIP(hipMemcpyAsync(&D.efzc, D.e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+    V2H_HIP(hipStreamSynchronize(stream_));
+    if (norms) refresh_norms(i);
+    return D.efzc;
+}
+
+void Batch::launch_df_intake_only(int i) {
     const InstDev& I = inst_[i];
     const Structure& S = *structs_[inst_sid_[i]];
     DfState& D = df_state(i);
@@ -527,10 +536,6 @@ double Batch::run_df_intake(int i, bool norms) {
     launch_oo_sub(c_.p + I.poff, z_.p + I.poff, v1_.p + I.poff, S.n_primal, stream_);  // v1 = c - z
     tstop(kKindDf);
     V2H_HIP(hipGetLastError());
-    V2H_HIP(hipMemcpyAsync(&D.efzc, D.e.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
-    V2H_HIP(hipStreamSynchronize(stream_));
-    if (norms) refresh_norms(i);
-    return D.efzc;
 }
 
 double Batch::instance_energy(int i) {

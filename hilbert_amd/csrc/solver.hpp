@@ -64,10 +64,17 @@ struct FcidumpArgs {
 // full-space slices, h, the accumulated orbital rotation and the scratch of
 // the intake and of the orbital step.
 struct DfState {
+    DfState() = default;
+    DfState(const DfState&) = delete;
+    DfState& operator=(const DfState&) = delete;
+    ~DfState() { if (pin) (void)hipHostFree(pin); }
     int nfull = 0, nq = 0, n = 0, ncore = 0, no = 0;
     std::vector<int> core, active;
     double efzc = 0.0;
     int64_t m_rows = 0;
+    // pinned read-back slots of the orbital step: |g|^2 and the model change,
+    // ||K||_1, E_fzc, c.x (copied asynchronously, read after one sync)
+    double* pin = nullptr;
     DBuf<double> qmo, h, bd, tmp, uacc, uscr, hscr;
     DBuf<int64_t> idx;
     DBuf<int> act, cor;
@@ -202,6 +209,7 @@ class Batch {
     void refresh_norms(int i);
     DfState& df_state(int i);
     double run_df_intake(int i, bool norms = true);  // resident DF integrals -> c, v1, E_fzc (and ||c||)
+    void launch_df_intake_only(int i);    // the same launches, no read-back
     double instance_energy(int i);        // c.x + E_fzc
     void oo_setup(int i, const OrbOptArgs& a);
     void oo_gradient(int i);
