@@ -5,7 +5,9 @@
   their largest element (C1, symmetry-blocked, active-active rotations).
 * v2h_batch_optimize_orbitals: one call from the same primal vector follows
   the oracle's steps (same iteration count and convergence flag; gradient
-  norm, dE, energy, accumulated rotation and rotated integrals to 1e-10).
+  norm, dE, energy, accumulated rotation and rotated integrals to 1e-10),
+  with and without active-active rotations; in a batch the call touches only
+  its instance and equals the same instance alone bit for bit.
 * The full v2RDM-CASSCF loop (solve_casscf) on two-electron problems, where
   DQG is exact, reaches the brute-force CASSCF energy within 1e-6 Eh, two
   instances batched together.
@@ -67,13 +69,13 @@ def test_gradient_and_hessian_match_oracle(case):
     assert np.abs(H - Ho).max() <= 1e-11 * max(1.0, np.abs(Ho).max())
 
 
-@pytest.mark.parametrize("case", ["c1", "sym"])
+@pytest.mark.parametrize("case", ["c1", "c1_actact", "sym"])
 def test_one_call_matches_oracle(case):
     c = _instance(case, 43)
     B = Batch([c["prob"]], options())
     B.set_df_integrals(0, c["h"], c["qmo"], c["core"], c["active"])
     B.set_guess(0, c["x"])
-    r = B.optimize_orbitals(0, c["irr"], maxiter=6)
+    r = B.optimize_orbitals(0, c["irr"], maxiter=6, active_active=c["aa"])
     U, h_gpu, q_gpu = B.df_orbitals(0)
     B.close()
     h2, B2, Uo, info = oo.optimize_orbitals(c["h"], oo.unpack_df(c["qmo"], c["nfull"]), c["D"], c["d"], c["occ"],
@@ -87,6 +89,32 @@ def test_one_call_matches_oracle(case):
     assert np.abs(U.T @ U - np.eye(c["nfull"])).max() <= 1e-12
     assert np.abs(h_gpu - h2).max() <= 1e-10
     assert np.abs(q_gpu - oo.pack_df(B2)).max() <= 1e-10
+
+
+def test_orbital_call_in_a_batch_equals_alone():
+    """Instance 1 of a two-instance batch takes the same orbital step as the
+    same instance alone, bit for bit, and instance 0 is left untouched."""
+    a, b = _instance("sym", 51), _instance("c1", 53)
+    B2 = Batch([b["prob"], a["prob"]], options())
+    B1 = Batch([a["prob"]], options())
+    for B, k in ((B2, 1), (B1, 0)):
+        B.set_df_integrals(k, a["h"], a["qmo"], a["core"], a["active"])
+        B.set_guess(k, a["x"])
+    B2.set_df_integrals(0, b["h"], b["qmo"], b["core"], b["active"])
+    B2.set_guess(0, b["x"])
+    c0 = B2.c(0).copy()
+    r2 = B2.optimize_orbitals(1, a["irr"], maxiter=5)
+    r1 = B1.optimize_orbitals(0, a["irr"], maxiter=5)
+    U2, h2, q2 = B2.df_orbitals(1)
+    U1, h1, q1 = B1.df_orbitals(0)
+    assert r1 == r2
+    assert np.array_equal(U1, U2) and np.array_equal(h1, h2) and np.array_equal(q1, q2)
+    assert np.array_equal(B1.c(0), B2.c(1))
+    assert np.array_equal(B2.c(0), c0)
+    U0, _, _ = B2.df_orbitals(0)
+    assert np.array_equal(U0, np.eye(b["nfull"]))
+    B1.close()
+    B2.close()
 
 
 def test_casscf_two_electrons_batched():
