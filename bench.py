@@ -6,7 +6,9 @@ with C2v irrep labels drawn at random and synthetic molecular-like integrals
 (naux = 48), every instance with its own seed.  Integrals and tables are
 resident in HBM before the timed region; the timed region is reset + solve.
 
-Printed (one JSON line): value = converged SDPs/s of the batch; the single
+Printed (one JSON line): value = converged SDPs/s of the batch, where an SDP
+counts when the stopping test holds and its energy is within 1e-6 Eh of the
+oracle's (per-instance fixture tests/golden/c2v16_oracle.json); the single
 instance rate and the CPU oracle rate are reported beside it; `roofline` is
 the dominant kernel's achieved rate: the mean work of its launches (each
 launch processes the instances still iterating) over the mean duration of its
@@ -220,6 +222,39 @@ def energy_check(item, maxiter, gpu_energy, cpu):
 
 
 PROGRESS_CHUNK = [1000]
+
+
+def load_fixture(nmo, batch, positivity, d3):
+    """Per-instance oracle energies of the benchmark batch (tests/golden/,
+    written by tests/golden/make_golden.py --c2v16 at 1e-9 / 1e-8): {index:
+    energy}, or {} when the workload has no fixture."""
+    if nmo != 16 or positivity != "DQG" or d3:
+        return {}
+    path = os.path.join(ROOT, "tests", "golden", "c2v16_oracle.json")
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return {}
+    return {o["index"]: o["energy_primal"] for o in data["instances"] if o["index"] < batch and o["converged"]}
+
+
+def check_energies(B, ref, tol=1e-6):
+    """(instances counted as converged, max |E - E_oracle|, failing indices):
+    an instance counts when the solver's stopping test holds and, where the
+    oracle fixture has it, its energy is within tol of the oracle's."""
+    ok, worst, bad = 0, 0.0, []
+    for i in range(B.count):
+        st = B.status(i)
+        good = bool(st["converged"])
+        if i in ref:
+            d = abs(st["energy_primal"] - ref[i])
+            worst = max(worst, d)
+            good = good and d <= tol
+        if good:
+            ok += 1
+        else:
+            bad.append(i)
+    return ok, worst, bad
 
 
 def solve_with_progress(B, label, chunk=None):
@@ -543,6 +578,7 @@ def main():
                     help="sdp workload: the N-representability conditions")
     ap.add_argument("--d3", type=int, default=0, help="sdp workload: add the D3 condition (CONSTRAIN_D3)")
     ap.add_argument("--chunk", type=int, default=1000, help="outer iterations between progress lines")
+    ap.add_argument("--energies", default="", help="sdp workload: write the per-instance status of step 0 here")
     ap.add_argument("--mu-update-frequency", type=int, default=1000,
                     help="sdp workload: MU_UPDATE_FREQUENCY, outer iterations between mu rebalancings "
                          "(hilbert's default 1000)")
@@ -594,13 +630,22 @@ def main():
         solve_with_progress(B, f"warmup {w}")
     B.reset_kernel_times()
 
-    step_times, conv, oiters, counts = [], [], [], [[0, 0] for _ in range(args.batch)]
+    ref = load_fixture(args.nmo, args.batch, args.positivity, args.d3)
+    step_times, conv, conv_solver, oiters, counts = [], [], [], [], [[0, 0] for _ in range(args.batch)]
+    worst, failing = 0.0, set()
     for k in range(args.steps):
         reset_all(B)
         t0 = time.perf_counter()
         nconv = solve_with_progress(B, f"step {k}")
         step_times.append(time.perf_counter() - t0)
-        conv.append(nconv)
+        conv_solver.append(nconv)
+        ok, w, bad = check_energies(B, ref)
+        conv.append(ok)
+        worst = max(worst, w)
+        failing.update(bad)
+        if k == 0 and args.energies:
+            with open(args.energies, "w") as f:
+                json.dump([{"index": i, **B.status(i)} for i in range(B.count)], f)
         oiters.append([B.status(i)["oiter"] for i in range(B.count)])
         for i in range(B.count):
             st = B.status(i)
@@ -663,6 +708,10 @@ def main():
                    "mu_update_frequency": MU_FREQ[0],
                    "constrain_d3": bool(args.d3), "constrain_spin": True, "constrain_sz": True},
         "converged": sum(conv), "attempted": args.batch * args.steps,
+        "converged_check": {"stopping_test": sum(conv_solver), "within_1e-6_of_oracle": sum(conv),
+                            "instances_with_oracle_energy": len(ref), "max_abs_dE_vs_oracle": worst,
+                            "failing_instances": sorted(failing),
+                            "oracle": "tests/golden/c2v16_oracle.json: oracle BPSDP at 1e-9 / 1e-8 per instance"},
         "mean_outer_iterations": mean_oiter,
         "mean_cg_per_outer": sum(s["iiter"] for s in st) / max(1, sum(s["oiter"] for s in st)),
         "single_gpu": single,

@@ -52,8 +52,11 @@ class Status(C.Structure):
                 ("active", C.c_int), ("status_bits", C.c_int), ("last_cg_iterations", C.c_int)]
 
 
+# v2h_monitor_fn: (instance, oiter, iiter, E_p, E_d, mu, eps_p, eps_d, data).
 MONITOR = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                       C.c_double, C.c_double, C.c_void_p)
+# v2h_sdp_monitor_fn, libsdp's monitor: (print_level, oiter, iiter, E_p, E_d, mu, eps_p, eps_d, data).
+SDP_MONITOR = MONITOR
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -115,7 +118,7 @@ SIGNATURES = {
     "v2h_sdp_create": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(SdpOptions), C.c_int, C.POINTER(_vp)]),
     "v2h_sdp_destroy": (C.c_int, [_vp]),
     "v2h_sdp_set_options": (C.c_int, [_vp, C.POINTER(SdpOptions)]),
-    "v2h_sdp_solve": (C.c_int, [_vp, _dp, _dp, _dp, _ip, C.c_int, C.c_int, _vp, _vp, _vp, _vp,
+    "v2h_sdp_solve": (C.c_int, [_vp, _dp, _dp, _dp, _ip, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp,
                                 C.POINTER(C.c_int)]),
     "v2h_sdp_status": (C.c_int, [_vp, C.POINTER(Status)]),
     "v2h_sdp_get_y": (C.c_int, [_vp, _dp]),

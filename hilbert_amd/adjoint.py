@@ -14,9 +14,13 @@ array as given (an 8-fold-unique (pq|rs) collects the sum over its images).
     E.backward()                                           # h.grad, g.grad
 
 ``v2rdm_energies`` solves several instances as one batch and returns one
-energy per instance.
+energy per instance.  An instance that does not converge within MAXITER (or
+whose state turns non-finite) returns NaN, with a RuntimeWarning: its c . x
+is not the optimum and x . dc/dtheta not its gradient.
 """
 from __future__ import annotations
+
+import warnings
 
 import numpy as np
 import torch
@@ -57,10 +61,26 @@ class _Energies(torch.autograd.Function):
                 keep["x"] = [B.x(i) for i in range(B.count)]
         finally:
             B.close()
+        # The envelope gradient x* . dc/dtheta is dE/dtheta only at a converged
+        # optimum: an instance that stopped at MAXITER or with a non-finite
+        # state (status_bits) gets a NaN energy and NaN gradients, with a
+        # warning, and leaves the other instances of the batch untouched.
+        energies = []
+        bad = []
+        for i, s in enumerate(st):
+            if s["converged"] and not s["status_bits"]:
+                energies.append(s["energy_primal"])
+            else:
+                bad.append(i)
+                energies.append(float("nan"))
+                grads[i] = tuple(np.full_like(d, np.nan) for d in grads[i])
+        if bad:
+            warnings.warn("v2RDM SDP not converged (MAXITER reached or non-finite state) for instance(s) %s: "
+                          "energy and gradient set to NaN" % bad, RuntimeWarning, stacklevel=3)
         ctx.grads = grads
         ctx.shapes = [(t.shape, t.dtype, t.device) for t in hg]
         dev = hg[0].device if hg else "cpu"
-        return torch.tensor([s["energy_primal"] for s in st], dtype=torch.float64, device=dev)
+        return torch.tensor(energies, dtype=torch.float64, device=dev)
 
     @staticmethod
     def backward(ctx, gE):

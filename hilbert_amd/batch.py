@@ -192,10 +192,13 @@ class Batch:
                     continue
                 last[i] = self.optimize_orbitals(i, irr[i], **orbopt)
                 macro[i] += 1
+                # orbopt_converged_ is set by the first call that passes the
+                # call-level test and never reset (v2rdm_solver.cc:3271-3274);
+                # the loop ends once it is set and the SDP chunk before the
+                # call converged (the while test of v2rdm_solver.cc:1361)
                 if last[i]["converged"]:
                     oo_conv[i] = True
-                if last[i]["converged"] and st["converged"]:
-                    # a converged orbital call leaves the SDP marked converged
+                if oo_conv[i] and st["converged"]:
                     done[i] = sdp_conv[i] = True
             if progress:
                 progress(max(macro), last)

@@ -561,9 +561,10 @@ int v2h_sdp_set_options(v2h_sdp* s, const v2h_sdp_options* opt) {
 }
 
 int v2h_sdp_solve(v2h_sdp* s, double* x, const double* b, const double* c, const int* block_dims, int n_blocks,
-                  int maxiter, v2h_map_fn Au, v2h_map_fn ATu, v2h_monitor_fn mon, void* data, int* converged) {
+                  int maxiter, v2h_map_fn Au, v2h_map_fn ATu, v2h_sdp_monitor_fn mon, int print_level, void* data,
+                  int* converged) {
     return guarded([&] {
-        int cv = S(s).solve(x, b, c, block_dims, n_blocks, maxiter, Au, ATu, mon, data);
+        int cv = S(s).solve(x, b, c, block_dims, n_blocks, maxiter, Au, ATu, mon, print_level, data);
         if (converged) *converged = cv;
     });
 }

@@ -70,6 +70,16 @@ struct InstState {
 
 enum StatusBits : int { kStatusJacobiNotConverged = 1, kStatusNonFinite = 2 };
 
+// CG stopping rule of each outer iteration: relative residual below
+// kCgTolFactor x the larger of the last primal and dual errors, capped at
+// 1e-2 first and then raised to at least CG_CONVERGENCE.  Shared by the batch
+// kernels and the caller-map solver.
+#ifndef V2H_CG_TOL_FACTOR
+#define V2H_CG_TOL_FACTOR 0.003
+#endif
+constexpr double kCgTolFactor = V2H_CG_TOL_FACTOR;
+constexpr double kCgTolCap = 1e-2;
+
 struct SolverOptions {
     double e_convergence, r_convergence, cg_convergence, mu0;
     int cg_maxiter, maxiter, mu_update_frequency;

@@ -25,14 +25,6 @@
 
 namespace v2h {
 
-// CG stopping rule of each outer iteration: relative residual below
-// kCgTolFactor x the larger of the last primal and dual errors (at most 1e-2,
-// at least CG_CONVERGENCE).
-#ifndef V2H_CG_TOL_FACTOR
-#define V2H_CG_TOL_FACTOR 0.003
-#endif
-constexpr double kCgTolFactor = V2H_CG_TOL_FACTOR;
-
 namespace {
 
 constexpr int kWG = 256;         // gather workgroup: 4 wavefronts
@@ -454,7 +446,7 @@ __global__ __launch_bounds__(kWG) void k_pre_reduce(BatchDev B, UnitsDev U) {
         st.pending = 0;
         double e = st.oiter == 0 ? 1.0 : fmax(st.eps_p, st.eps_d);
         double tol = kCgTolFactor * e;
-        if (tol > 1e-2) tol = 1e-2;
+        if (tol > kCgTolCap) tol = kCgTolCap;
         if (tol < o.cg_convergence) tol = o.cg_convergence;
         double mu = st.mu;
         double rhs2 = tot[0] + 2.0 * mu * tot[1] + mu * mu * tot[2];

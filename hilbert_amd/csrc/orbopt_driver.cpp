@@ -273,18 +273,21 @@ void Batch::optimize_orbitals(int i, const OrbOptArgs& a, OrbOptResult* res) {
             applies++;
         }
     }
-    // the cost vector changed: its norm, and unless the orbitals are
-    // converged (a change below the thresholds), the SDP is not converged
-    // for it yet
+    // The call-level test of RotateOrbitals (v2rdm_solver.cc:3271-3274): the
+    // net energy change of the whole call and the last gradient norm, not the
+    // optimiser's per-step flags.
+    const bool call_converged = std::fabs(E_cur - initial_E) < a.e_convergence && gnorm < a.g_convergence;
+    // the cost vector changed: its norm, and unless the call converged the
+    // SDP is not converged for it yet
     refresh_norms(i);
-    if (!(conv_e && conv_g)) {
+    if (!call_converged) {
         sync_state_to_host();
         state_[i].converged = 0;
         sync_state_to_device();
     }
     if (res) {
         res->iterations = iters;
-        res->converged = (conv_e && conv_g) ? 1 : 0;
+        res->converged = call_converged ? 1 : 0;
         res->nrot = S.nrot;
         res->evaluations = evals;
         res->rotations = applies;

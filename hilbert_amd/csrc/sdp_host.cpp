@@ -73,7 +73,7 @@ void CallbackSdp::project(double* x) {
 }
 
 int CallbackSdp::solve(double* x, const double* b, const double* c, const int* dims, int nblocks, int maxiter,
-                       MapFn Au, MapFn ATu, MonitorFn mon, void* data) {
+                       MapFn Au, MapFn ATu, SdpMonitorFn mon, int print_level, void* data) {
     if (!x || !b || !c || !dims || nblocks <= 0 || !Au || !ATu) throw std::invalid_argument("null argument");
     V2H_HIP(hipSetDevice(device_));
     int64_t tot = 0;
@@ -116,7 +116,8 @@ int CallbackSdp::solve(double* x, const double* b, const double* c, const int* d
         for (int64_t i = 0; i < nd_; i++) r[i] = p[i] = rhs[i] - w[i];
         double rr = dot(r, r);
         double e = oiter == 0 ? 1.0 : std::max(eps_p, eps_d);
-        double tol = std::min(std::max(0.003 * e, opt_.cg_convergence), 1e-2);
+        double tol = std::min(kCgTolFactor * e, kCgTolCap);
+        if (tol < opt_.cg_convergence) tol = opt_.cg_convergence;
         const double thr = tol * tol * dot(rhs, rhs);
         int k = 0;
         while (k < opt_.cg_maxiter && rr > thr) {
@@ -160,7 +161,7 @@ int CallbackSdp::solve(double* x, const double* b, const double* c, const int* d
         oiter++;
         if (opt_.mu_update_frequency > 0 && oiter % opt_.mu_update_frequency == 0 && eps_d > 0.0 && eps_p > 0.0)
             mu_ *= eps_p / eps_d;
-        if (mon) mon(0, (int)oiter, k, energy_primal, energy_dual, mu_, eps_p, eps_d, data);
+        if (mon) mon(print_level, (int)oiter, k, energy_primal, energy_dual, mu_, eps_p, eps_d, data);
         if (std::fabs(energy_primal - energy_dual) < opt_.e_convergence && eps_p < opt_.r_convergence &&
             eps_d < opt_.r_convergence) {
             converged = 1;

@@ -20,8 +20,10 @@
 namespace v2h {
 
 using MapFn = void (*)(double* out, double* in, void* data);
-using MonitorFn = void (*)(int instance, int oiter, int iiter, double energy_primal, double energy_dual, double mu,
-                           double primal_error, double dual_error, void* data);
+// libsdp's SDPProgressMonitorFunction (v2rdm_solver.cc:115-124): print_level
+// first, as sdp_->solve passes it through (v2rdm_solver.cc:1340).
+using SdpMonitorFn = void (*)(int print_level, int oiter, int iiter, double energy_primal, double energy_dual,
+                              double mu, double primal_error, double dual_error, void* data);
 
 class CallbackSdp {
   public:
@@ -32,9 +34,11 @@ class CallbackSdp {
 
     void set_options(const SolverOptions& o) { opt_ = o; }
     // Up to maxiter outer iterations from x (overwritten with the result);
-    // y, z and mu persist across calls.  Returns 1 when converged.
+    // y, z and mu persist across calls.  mon, if set, is called after every
+    // outer iteration with print_level passed through.  Returns 1 when
+    // converged.
     int solve(double* x, const double* b, const double* c, const int* dims, int nblocks, int maxiter, MapFn Au,
-              MapFn ATu, MonitorFn mon, void* data);
+              MapFn ATu, SdpMonitorFn mon, int print_level, void* data);
 
     int64_t n_primal() const { return np_; }
     int64_t n_dual() const { return nd_; }

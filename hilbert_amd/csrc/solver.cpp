@@ -413,6 +413,12 @@ void Batch::refresh_norms(int i) {
     V2H_HIP(hipStreamSynchronize(stream_));
     sync_state_to_host();
     state_[i].cnorm = cc > 0 ? std::sqrt(cc) : 1.0;
+    // A non-finite cost (NaN or infinite integrals) fails this instance
+    // before any kernel iterates it; the rest of the batch is unaffected.
+    if (!std::isfinite(cc)) {
+        state_[i].status |= kStatusNonFinite;
+        state_[i].active = 0;
+    }
     sync_state_to_device();
 }
 
@@ -758,10 +764,16 @@ void Batch::get_b(int i, double* b) const {
     std::copy(v.begin(), v.end(), b);
 }
 void Batch::set_dual(int i, const double* y) {
+    const Structure& S = *structs_[inst_sid_[i]];
+    const InstDev& I = inst_[i];
     DBuf<double> t;
     t.alloc(n_dual(i));
     h2d(t.p, y, n_dual(i), stream_);
-    launch_copy_strided(&cg_.p[inst_[i].doff].y, 4, t.p, 1, n_dual(i), stream_);
+    launch_copy_strided(&cg_.p[I.doff].y, 4, t.p, 1, n_dual(i), stream_);
+    // the next outer iteration starts from A (A^T y): keep aty = A^T y with y
+    launch_fill(aty_.p + I.poff, S.n_primal, 0.0, stream_);
+    launch_spmv_plain(S.dev().AT, unitsAT_.p, hwgAT_begin_[i], hwgAT_begin_[i + 1] - hwgAT_begin_[i], t.p,
+                      aty_.p + I.poff, stream_);
     V2H_HIP(hipStreamSynchronize(stream_));
 }
 void Batch::set_slack(int i, const double* z) {
@@ -772,6 +784,14 @@ void Batch::set_slack(int i, const double* z) {
     get_cost(i, c.data());
     for (int64_t k = 0; k < n_primal(i); k++) c[k] -= z[k];
     h2d(v1_.p + I.poff, c.data(), n_primal(i), stream_);
+}
+void Batch::set_options(const SolverOptions& o) {
+    const bool tighter = o.e_convergence < opt_.e_convergence || o.r_convergence < opt_.r_convergence;
+    opt_ = o;
+    if (!tighter || !constructed_) return;
+    sync_state_to_host();
+    for (auto& st : state_) st.converged = 0;
+    sync_state_to_device();
 }
 double Batch::get_mu(int i) {
     sync_state_to_host();

@@ -113,7 +113,10 @@ class Batch {
     int64_t n_primal(int i) const { return structs_[inst_sid_[i]]->n_primal; }
     int64_t n_dual(int i) const { return structs_[inst_sid_[i]]->n_dual; }
     const SolverOptions& options() const { return opt_; }
-    void set_options(const SolverOptions& o) { opt_ = o; }
+    // New options; tighter E_CONVERGENCE or R_CONVERGENCE re-open the
+    // instances that converged under the old ones, so that the next solve
+    // continues them from their state to the new thresholds.
+    void set_options(const SolverOptions& o);
 
     // Hamiltonian in the user's orbital order: h[n*n], g[n^4] chemists' (pq|rs).
     void set_integrals(int i, const double* h, const double* g);

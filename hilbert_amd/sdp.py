@@ -31,8 +31,13 @@ class BPSDPSolver:
     def __del__(self):
         self.close()
 
-    def solve(self, x, b, c, block_dims, maxiter, Au, ATu, monitor=None) -> bool:
-        """Au(u) -> A u (n_dual), ATu(y) -> A^T y (n_primal); x is updated in place."""
+    def solve(self, x, b, c, block_dims, maxiter, Au, ATu, monitor=None, print_level=1, data=None) -> bool:
+        """Au(u) -> A u (n_dual), ATu(y) -> A^T y (n_primal); x is updated in place.
+
+        monitor(print_level, oiter, iiter, E_p, E_d, mu, eps_p, eps_d, data)
+        is libsdp's progress monitor (v2rdm_solver.cc:115-124), called after
+        every outer iteration with print_level and data passed through, as
+        sdp_->solve(..., bpsdp_monitor, 1, this) does (v2rdm_solver.cc:1340)."""
         npr, nd = self.n_primal, self.n_dual
 
         def wrap(fn, n_in, n_out):
@@ -43,13 +48,14 @@ class BPSDPSolver:
             return MAP(cb)
 
         cau, catu = wrap(Au, npr, nd), wrap(ATu, nd, npr)
-        mon = N.MONITOR(lambda *a: monitor(*a[:-1])) if monitor else None
+        mon = N.SDP_MONITOR(lambda *a: monitor(*a[:-1], data)) if monitor else None
         conv = C.c_int(0)
         N.check(N.lib().v2h_sdp_solve(self._h, x, np.ascontiguousarray(b, dtype=np.float64),
                                       np.ascontiguousarray(c, dtype=np.float64),
                                       np.ascontiguousarray(block_dims, dtype=np.int32), len(block_dims),
                                       int(maxiter), C.cast(cau, C.c_void_p), C.cast(catu, C.c_void_p),
-                                      C.cast(mon, C.c_void_p) if mon else None, None, C.byref(conv)))
+                                      C.cast(mon, C.c_void_p) if mon else None, int(print_level), None,
+                                      C.byref(conv)))
         return bool(conv.value)
 
     def status(self) -> dict:

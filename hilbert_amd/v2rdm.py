@@ -23,6 +23,7 @@ the option instead of being ignored.
 """
 from __future__ import annotations
 
+import warnings
 from dataclasses import dataclass
 
 import numpy as np
@@ -159,6 +160,11 @@ class v2RDMHelper:
         st = self._batch.status(0)
         self.status = st
         self.energy = st["energy_primal"]
+        if not st["converged"] or st["status_bits"]:
+            warnings.warn("v2RDM SDP did not converge (%d outer iterations, MAXITER %d, status bits %d): "
+                          "the energy is that of the last iterate" % (st["oiter"], int(self.options["MAXITER"]),
+                                                                      st["status_bits"]),
+                          RuntimeWarning, stacklevel=2)
         return self.energy
 
     def _need(self):

@@ -98,6 +98,9 @@ int v2h_batch_create(const v2h_problem_desc *descs, int count, const v2h_sdp_opt
                      v2h_batch **out);
 int v2h_batch_destroy(v2h_batch *b);
 int v2h_batch_count(const v2h_batch *b, int *count);
+/* Replace the options.  A tighter E_CONVERGENCE or R_CONVERGENCE re-opens the
+ * instances that had converged, so that the next v2h_batch_solve continues
+ * them from their current x, y, z and mu to the new thresholds. */
 int v2h_batch_set_options(v2h_batch *b, const v2h_sdp_options *opt);
 
 /* Sizes (determine_n_primal / determine_n_dual) and the PSD block list
@@ -163,8 +166,11 @@ int v2h_batch_write_fcidump(v2h_batch *b, int inst, int nfull, const int *full_i
  * (RepackIntegrals).  The SDP state (y, z, mu) is kept; unless the call
  * converged, the instance is marked unconverged.  full_irreps [nfull] (NULL: C1) label the full space; the
  * active orbitals must carry the instance's irreps.  res (may be NULL):
- * iterations, converged (|dE| < e_convergence and |g| < g_convergence, the
- * test v2rdm_solver.cc:3271-3274 applies to the call), rotation count,
+ * iterations, converged (the call-level test of v2rdm_solver.cc:3271-3274:
+ * |energy change over the whole call| < e_convergence and the last gradient
+ * norm < g_convergence; unless it holds the instance is marked unconverged;
+ * the caller keeps it as a flag that is never reset, as orbopt_converged_ is),
+ * rotation count,
  * gradient norm at the last evaluation, dE over the call, the energy c.x +
  * E_fzc afterwards and E_fzc. */
 typedef struct v2h_orbopt_options {
@@ -279,6 +285,12 @@ int v2h_batch_cost_adjoint(v2h_batch *b, int inst, const double *xbar, double *d
  * (sizes in block_dims, row-major and contiguous in x) runs on the GPU.
  * y, z and mu persist across solve calls, as libsdp's do. */
 typedef void (*v2h_map_fn)(double *out, double *in, void *data);
+/* libsdp's SDPProgressMonitorFunction (v2rdm_solver.cc:115-124), called after
+ * every outer iteration with the print_level given to v2h_sdp_solve, so that
+ * hilbert's bpsdp_monitor (which prints only for print_level > 0) passes
+ * through unchanged; iiter = CG steps of that outer iteration. */
+typedef void (*v2h_sdp_monitor_fn)(int print_level, int oiter, int iiter, double energy_primal, double energy_dual,
+                                   double mu, double primal_error, double dual_error, void *data);
 typedef struct v2h_sdp v2h_sdp;
 int v2h_sdp_create(int64_t n_primal, int64_t n_dual, const v2h_sdp_options *opt, int device, v2h_sdp **out);
 int v2h_sdp_destroy(v2h_sdp *s);
@@ -286,7 +298,8 @@ int v2h_sdp_set_options(v2h_sdp *s, const v2h_sdp_options *opt);
 /* Up to maxiter outer iterations from x, which receives the result;
  * *converged = libsdp is_converged() (v2rdm_solver.cc:1361). */
 int v2h_sdp_solve(v2h_sdp *s, double *x, const double *b, const double *c, const int *block_dims, int n_blocks,
-                  int maxiter, v2h_map_fn Au, v2h_map_fn ATu, v2h_monitor_fn mon, void *data, int *converged);
+                  int maxiter, v2h_map_fn Au, v2h_map_fn ATu, v2h_sdp_monitor_fn mon, int print_level, void *data,
+                  int *converged);
 int v2h_sdp_status(const v2h_sdp *s, v2h_status *st);
 int v2h_sdp_get_y(const v2h_sdp *s, double *y);   /* libsdp get_y (checkpoint.cc:58-64) */
 int v2h_sdp_get_z(const v2h_sdp *s, double *z);   /* libsdp get_z */

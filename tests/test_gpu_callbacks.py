@@ -44,3 +44,32 @@ def test_callback_solve_matches_oracle(name):
     assert st["eps_p"] < R and st["eps_d"] < R
     assert np.abs(P.Au(x) - P.b()).max() < 1e-4
     S.close()
+
+
+def test_monitor_gets_libsdp_arguments():
+    """hilbert's bpsdp_monitor prints only for print_level > 0
+    (v2rdm_solver.cc:115-124) and receives the print_level given to solve
+    (v2rdm_solver.cc:1340): a monitor gated the same way logs one line per
+    outer iteration at print_level 1 and none at print_level 0."""
+    P, h, g = _case("hubbard6")
+    c = P.pack_c(h, g)
+    S = BPSDPSolver(P.n_primal, P.n_dual, options(maxiter=50000))
+    lines, seen = [], []
+
+    def bpsdp_monitor(print_level, oiter, iiter, ep, ed, mu, eps_p, eps_d, data):
+        seen.append((print_level, data))
+        if print_level > 0:
+            lines.append((oiter, iiter, ep, ed, mu, eps_p, eps_d))
+
+    x = P.guess()
+    S.solve(x, P.b(), c, P.dims, 7, P.Au, P.ATu, monitor=bpsdp_monitor, print_level=1, data="solver")
+    assert [l[0] for l in lines] == list(range(1, 8))
+    assert all(s == (1, "solver") for s in seen)
+    st = S.status()
+    oi, ii, ep, ed, mu, eps_p, eps_d = lines[-1]
+    assert (ep, ed, mu, eps_p, eps_d) == (st["energy_primal"], st["energy_dual"], st["mu"], st["eps_p"], st["eps_d"])
+    assert sum(l[1] for l in lines) == st["iiter"] and ii >= 0
+    n = len(lines)
+    S.solve(x, P.b(), c, P.dims, 5, P.Au, P.ATu, monitor=bpsdp_monitor, print_level=0)
+    assert len(lines) == n and len(seen) == 12 and seen[-1][0] == 0
+    S.close()

@@ -90,3 +90,30 @@ def test_checkpoint_restores_a_solve():
         assert abs(B.status(i)["energy_primal"] - ref.status(i)["energy_primal"]) < 1e-6
     for h in (ref, A, B):
         h.close()
+
+
+def test_restored_state_continues_the_iteration():
+    """x, y, z and mu copied into a fresh handle continue the same iteration:
+    the next outer iteration from the restored state equals the next outer
+    iteration of the original handle (A^T y is rebuilt with y, so the CG's
+    first residual is that of the copied state; only the projection's
+    eigenvalue warm start differs, which moves the last bits).  CG_CONVERGENCE
+    1e-2 gives both handles the same CG tolerance (a fresh handle starts from
+    errors of 1)."""
+    items = _cases()
+    A = _batch(items, maxiter=100000, cg_convergence=1e-2)
+    A.solve(150)
+    B = _batch(items, maxiter=100000, cg_convergence=1e-2)
+    for i in range(len(items)):
+        B.set_guess(i, A.x(i))
+        B.set_y(i, A.y(i))
+        B.set_z(i, A.z(i))
+        B.set_mu(i, A.mu(i))
+    A.solve(1)
+    B.solve(1)
+    for i in range(len(items)):
+        ya, yb = A.y(i), B.y(i)
+        assert np.abs(ya - yb).max() <= 1e-9 * max(1.0, np.abs(ya).max()), i
+        assert np.abs(A.x(i) - B.x(i)).max() <= 1e-9, i
+    A.close()
+    B.close()
