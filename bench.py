@@ -65,6 +65,8 @@ def bytes_per_instance(kind, n_p, n_d):
                 "post": 4 * n_p + n_d}[kind]   # read y, x, c; write A^T y, U
 
 
+# with the refinement on, the 65..256 classes time only their fallback
+# launches, so their work per launch is an upper bound
 PROJ_CLASSES = (("project_jacobi16", 0, 16), ("project_tri64", 17, 64), ("project_tri128", 65, 128),
                 ("project_tri192", 129, 192), ("project_tri224", 193, 224), ("project_tri256", 225, 256),
                 ("project_jacobi_big", 257, 1 << 30))
@@ -94,6 +96,12 @@ def reset_all(B):
         B.reset(i)
 
 
+def gradients(B):
+    """dE/dh and dE/d(pq|rs) of every instance from its converged x (the
+    adjoint of the cost packing, v2h_batch_cost_adjoint), copied to the host."""
+    return [B.cost_adjoint(i) for i in range(B.count)]
+
+
 def load_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
@@ -118,7 +126,7 @@ def executions(kind, oiter, iiter):
     return oiter  # post and the projection classes
 
 
-def roofline(B, kt, counts, workload, dom=None):
+def roofline(B, kt, counts, workload, dom=None, proj=None):
     """Roofline dict of kernel kind `dom` (default: the dominant kernel, the
     largest total time in the timed region).  Work per launch is the mean
     over the kernel's launches of the work of the instances each launch
@@ -132,6 +140,18 @@ def roofline(B, kt, counts, workload, dom=None):
     ms, cnt = kt[dom]
     per_launch_ms = ms / max(cnt, 1)
     traffic = load_traffic(dom, workload)
+    if dom == "project_refine":
+        # one "launch" = the refinement launches of one outer iteration; its
+        # work = the nominal projection flops of the 65..256 blocks it
+        # accepted (the share of block projections it did, from the counters)
+        share = proj["refined"] / max(1, proj["refined"] + proj["fallback"]) if proj else 1.0
+        work = share * sum(executions(dom, o, it) * project_flops([d for d in B.block_dims(i) if 65 <= d <= 256])
+                           for i, (o, it) in enumerate(counts)) / max(cnt, 1)
+        achieved = work / (per_launch_ms * 1e-3) / 1e12
+        return dom, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "k_ref_* (refinement)",
+                     "launches": cnt, "per_launch_ms": per_launch_ms, "work_per_launch": work,
+                     "work": "(16/3) d^3 FP64 flops per accepted 65..256 block per outer iteration"}
     if dom.startswith("project"):
         lo, hi = [(a, b) for k, a, b in PROJ_CLASSES if k == dom][0]
         work = sum(executions(dom, o, it) * project_flops([d for d in B.block_dims(i) if lo <= d <= hi])
@@ -224,13 +244,16 @@ def energy_check(item, maxiter, gpu_energy, cpu):
 PROGRESS_CHUNK = [1000]
 
 
-def load_fixture(nmo, batch, positivity, d3):
+FIXTURES = {"sdp": "c2v16_oracle.json", "d2h64": "batch64_d2h_all_oracle.json", "big16": "big16_oracle.json"}
+
+
+def load_fixture(workload, nmo, batch, positivity, d3):
     """Per-instance oracle energies of the benchmark batch (tests/golden/,
-    written by tests/golden/make_golden.py --c2v16 at 1e-9 / 1e-8): {index:
-    energy}, or {} when the workload has no fixture."""
-    if nmo != 16 or positivity != "DQG" or d3:
+    written by tests/golden/make_golden.py --c2v16 / --batch64 / --big at
+    1e-9 / 1e-8): {index: energy}, or {} when the workload has no fixture."""
+    if positivity != "DQG" or d3 or (workload == "sdp" and nmo != 16):
         return {}
-    path = os.path.join(ROOT, "tests", "golden", "c2v16_oracle.json")
+    path = os.path.join(ROOT, "tests", "golden", FIXTURES[workload])
     try:
         data = json.load(open(path))
     except (OSError, ValueError):
@@ -584,9 +607,14 @@ def main():
                          "(hilbert's default 1000)")
     ap.add_argument("--casscf", type=int, default=1,
                     help="orbopt workload: also run the whole v2RDM-CASSCF loop once (0 = skip)")
-    ap.add_argument("--workload", choices=("sdp", "df", "fcidump", "orbopt"), default="sdp",
-                    help="sdp: the batched (16e,16o) solve; df: the density-fitted integral intake; "
-                         "fcidump: the FCIDUMP writer; orbopt: one v2RDM-CASSCF orbital step")
+    ap.add_argument("--workload", choices=("sdp", "d2h64", "big16", "df", "fcidump", "orbopt"), default="sdp",
+                    help="sdp: the batched (16e,16o) solve; d2h64: the 64 (6e,6o)-(12e,12o) D2h SDPs; big16: "
+                         "the 16 (20e,20o)-(24e,24o) D2h SDPs with their adjoint gradients; df: the "
+                         "density-fitted integral intake; fcidump: the FCIDUMP writer; orbopt: one v2RDM-CASSCF "
+                         "orbital step")
+    ap.add_argument("--projection", choices=("refined", "tridiagonal"), default="refined",
+                    help="PSD blocks of 65..256 rows: warm-started refinement on the FP64 matrix cores with the "
+                         "tridiagonal fallback, or the tridiagonal eigensolver alone")
     args = ap.parse_args()
     PROGRESS_CHUNK[0] = max(1, args.chunk)
     MU_FREQ[0] = args.mu_update_frequency
@@ -618,25 +646,38 @@ def main():
     if args.steps is None:
         args.steps = 1
 
-    items = configs.c2v(args.nmo, args.batch)
+    if args.workload == "d2h64":
+        items = configs.batch64_d2h(args.batch)
+    elif args.workload == "big16":
+        args.batch = min(args.batch, 16)
+        items = configs.batch16_big(args.batch)
+    else:
+        items = configs.c2v(args.nmo, args.batch)
     t1 = args.positivity in ("DQGT1", "DQGT1T2")
     t2 = args.positivity in ("DQGT2", "DQGT1T2")
     for prob, _, _ in items:
         prob.t1, prob.t2, prob.d3 = t1, t2, bool(args.d3)
     B = make_batch(items, args.maxiter)
+    B.set_projection(B.PROJECT_REFINED if args.projection == "refined" else B.PROJECT_TRIDIAGONAL)
     B.set_profiling(True)
+    grads = args.workload == "big16"
     for w in range(args.warmup):
         reset_all(B)
         solve_with_progress(B, f"warmup {w}")
+        if grads:
+            gradients(B)
     B.reset_kernel_times()
+    proj0 = B.projection_stats()
 
-    ref = load_fixture(args.nmo, args.batch, args.positivity, args.d3)
+    ref = load_fixture(args.workload, args.nmo, args.batch, args.positivity, args.d3)
     step_times, conv, conv_solver, oiters, counts = [], [], [], [], [[0, 0] for _ in range(args.batch)]
     worst, failing = 0.0, set()
     for k in range(args.steps):
         reset_all(B)
         t0 = time.perf_counter()
         nconv = solve_with_progress(B, f"step {k}")
+        if grads:
+            gradients(B)
         step_times.append(time.perf_counter() - t0)
         conv_solver.append(nconv)
         ok, w, bad = check_energies(B, ref)
@@ -658,10 +699,16 @@ def main():
     st = [B.status(i) for i in range(B.count)]
     n = args.nmo
     cond = args.positivity + ("+D3" if args.d3 else "")
-    workload = f"c2v{n}x{args.batch}" + ("" if cond == "DQG" else "_" + cond.lower().replace("+", ""))
-    dom, roof = roofline(B, kt, counts, workload)
+    workload = {"sdp": f"c2v{n}x{args.batch}", "d2h64": f"d2h{args.batch}",
+                "big16": f"big{args.batch}"}[args.workload] + ("" if cond == "DQG" else "_" + cond.lower().replace("+", ""))
+    proj1 = B.projection_stats()
+    proj = {"mode": args.projection, "refined": proj1["refined"] - proj0["refined"],
+            "fallback": proj1["fallback"] - proj0["fallback"]}
+    dom, roof = roofline(B, kt, counts, workload, proj=proj)
     # the HBM-bound gather kernel of the CG step, whichever kernel dominates
     roof_cg = roofline(B, kt, counts, workload, "cg_a")[1] if dom != "cg_a" else None
+    roof_ref = (roofline(B, kt, counts, workload, "project_refine", proj=proj)[1]
+                if dom != "project_refine" and kt.get("project_refine", (0, 0))[1] else None)
     energy0 = B.status(0)["energy_primal"]
     B.close()
 
@@ -696,14 +743,26 @@ def main():
     if args.energy_check:
         echeck = energy_check(items[0], 200000, energy0, cpu)
 
+    if args.workload == "sdp":
+        metric = f"converged v2RDM SDPs/s (ΔE≤1e-6 Eh), batched vs single GPU vs CPU, ({n}e,{n}o)"
+        wl = (f"{args.batch} x ({n}e,{n}o) {cond} SDPs, C2v labels at random, naux {3 * n}, "
+              "one batched solve to convergence per step")
+        cfg = {"nmo": n, "electrons": n, "nirrep": 4}
+    else:
+        ns = [p.nmo for p, _, _ in items]
+        lo, hi = min(ns), max(ns)
+        metric = (f"converged v2RDM SDPs/s (ΔE≤1e-6 Eh), {args.batch} x ({lo}e,{lo}o)-({hi}e,{hi}o) D2h"
+                  + (" with adjoint gradients" if grads else ""))
+        wl = (f"{args.batch} x ({lo}e,{lo}o)-({hi}e,{hi}o) {cond} SDPs, D2h labels at random, "
+              f"naux {'3 n' if grads else 30}, one batched solve to convergence per step"
+              + (" and dE/dh, dE/d(pq|rs) of every instance" if grads else ""))
+        cfg = {"nmo": [lo, hi], "electrons": [lo, hi], "nirrep": 8}
     out = {
-        "metric": f"converged v2RDM SDPs/s (ΔE≤1e-6 Eh), batched vs single GPU vs CPU, ({n}e,{n}o)",
+        "metric": metric,
         "value": value, "unit": "SDPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.batch} x ({n}e,{n}o) {cond} SDPs, C2v labels at random, naux {3 * n}, "
-                               "one batched solve to convergence per step",
-                   "batch": args.batch, "nmo": n, "electrons": n, "nirrep": 4,
+        "config": {"workload": wl, "batch": args.batch, **cfg, "projection": args.projection,
                    "e_convergence": E_CONV, "r_convergence": R_CONV, "positivity": args.positivity,
                    "mu_update_frequency": MU_FREQ[0],
                    "constrain_d3": bool(args.d3), "constrain_spin": True, "constrain_sz": True},
@@ -711,7 +770,9 @@ def main():
         "converged_check": {"stopping_test": sum(conv_solver), "within_1e-6_of_oracle": sum(conv),
                             "instances_with_oracle_energy": len(ref), "max_abs_dE_vs_oracle": worst,
                             "failing_instances": sorted(failing),
-                            "oracle": "tests/golden/c2v16_oracle.json: oracle BPSDP at 1e-9 / 1e-8 per instance"},
+                            "oracle": f"tests/golden/{FIXTURES[args.workload]}: oracle BPSDP at 1e-9 / 1e-8 "
+                                      "per instance"},
+        "projection": proj,
         "mean_outer_iterations": mean_oiter,
         "mean_cg_per_outer": sum(s["iiter"] for s in st) / max(1, sum(s["oiter"] for s in st)),
         "single_gpu": single,
@@ -721,6 +782,7 @@ def main():
         "energy_check": echeck,
         "roofline": roof,
         "roofline_cg_a": roof_cg,
+        "roofline_refine": roof_ref,
         "kernel_ms": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
         "kernel_launches": {k: v[1] for k, v in kt.items() if v[1]},
         "host": host_info(),

@@ -286,7 +286,26 @@ class Batch:
 
     KERNEL_KINDS = ("pre", "cg_at", "cg_a", "post", "project_jacobi16", "project_tri64", "project_tri128",
                     "project_tri192", "project_tri224", "project_tri256", "project_jacobi_big", "df_intake",
-                    "pair_gram", "orbopt")
+                    "pair_gram", "orbopt", "project_refine", "project_seed")
+
+    PROJECT_TRIDIAGONAL, PROJECT_REFINED = 0, 1
+
+    def set_projection(self, mode):
+        """PROJECT_REFINED (default): blocks of 65..256 rows refine the
+        previous iteration's eigenvectors on the FP64 matrix cores and fall
+        back to the tridiagonal eigensolver; PROJECT_TRIDIAGONAL: always the
+        tridiagonal eigensolver."""
+        N.check(N.lib().v2h_batch_set_projection(self._h, int(mode)))
+
+    def projection_stats(self) -> dict:
+        st = np.zeros(5, dtype=np.int64)
+        N.check(N.lib().v2h_batch_projection_stats(self._h, st))
+        return dict(zip(("refined", "fallback", "rejected_divergence", "rejected_correction",
+                         "rejected_clusters"), (int(v) for v in st)))
+
+    def set_refine_params(self, cluster_rel=1e-8, accept_e=3e-8, accept_cpl=1e-13):
+        N.check(N.lib().v2h_batch_set_refine_params(self._h, float(cluster_rel), float(accept_e),
+                                                    float(accept_cpl)))
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))

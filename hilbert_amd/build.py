@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libv2rdm_hip.so")
 SOURCES = ["layout.cpp", "constraints.cpp", "sell.cpp", "rdm.cpp", "projector.cpp", "solver.cpp", "sdp_host.cpp",
            "fcidump.cpp", "orbopt_driver.cpp", "abi.cpp",
            "kernels.hip",
-           "eig_tri.hip",
+           "eig_tri.hip", "refine.hip",
            "df.hip", "orbopt.hip"]
 ARCH = os.environ.get("V2H_OFFLOAD_ARCH", "gfx950")
 

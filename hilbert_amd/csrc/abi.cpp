@@ -507,6 +507,34 @@ int v2h_batch_launch_stats(v2h_batch* b, int64_t* st) {
     });
 }
 
+int v2h_batch_set_projection(v2h_batch* b, int mode) {
+    return guarded([&] {
+        if (mode != V2H_PROJECT_TRIDIAGONAL && mode != V2H_PROJECT_REFINED)
+            throw std::invalid_argument("unknown projection mode");
+        B(b).set_refine(mode == V2H_PROJECT_REFINED);
+    });
+}
+
+int v2h_batch_projection_stats(v2h_batch* b, int64_t* st) {
+    return guarded([&] {
+        check_ptr(st);
+        B(b).projection_stats(st);
+    });
+}
+
+int v2h_batch_set_refine_params(v2h_batch* b, double cluster_rel, double accept_e, double accept_cpl) {
+    return guarded([&] {
+        if (!(cluster_rel >= 0.0 && cluster_rel < 1e-2) || !(accept_e > 0.0 && accept_e <= 1e-3) ||
+            !(accept_cpl > 0.0 && accept_cpl <= 1e-6))
+            throw std::invalid_argument("refinement thresholds out of range");
+        v2h::RefParams p;
+        p.cluster_rel = cluster_rel;
+        p.accept_e = accept_e;
+        p.accept_cpl = accept_cpl;
+        B(b).set_refine_params(p);
+    });
+}
+
 int v2h_batch_cost_adjoint(v2h_batch* b, int inst, const double* xbar, double* dh, double* dg) {
     return guarded([&] {
         check_inst(B(b), inst);

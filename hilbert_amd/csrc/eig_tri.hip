@@ -583,7 +583,8 @@ constexpr int kTriWavesPerEU = NT == 512 ? 4 : 1;
 
 template <int NT, int RI, int CJ, int MODE>
 __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev B, const ProjItem* items,
-                                                                      double* work, double* lamst) {
+                                                                      double* work, double* lamst,
+                                                                      const int* skip) {
     constexpr bool GM = MODE == kTileGlobal;
     constexpr int PR = NT / 32;
     constexpr int DMAX = PR * RI;
@@ -604,6 +605,7 @@ __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev
     const ProjItem it = items[blockIdx.x];
     InstState& st = B.st[it.inst];
     if (!st.active) return;
+    if (skip && skip[it.gblock]) return;  // projected by the warm-started refinement
     const InstDev I = B.inst[it.inst];
     const int d = it.d;
     const int t = threadIdx.x, tr = t >> 5, tc = t & 31, lane = t & 63;
@@ -816,6 +818,7 @@ __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev
         if (t == 0) {
             ee[k] = beta;
             tau[k] = tk;
+            hv[(size_t)k * d + k] = tk;  // tau of reflector k, read back when the eigenvectors are formed
         }
         if (tk != 0.0) {  // uniform: every wavefront computed the same sums
             const int tn = 64 * wv_s + lane_now();
@@ -859,6 +862,7 @@ __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev
         if (t == 0) {
             ee[k] = beta;
             tau[k] = tk;
+            hv[(size_t)k * d + k] = tk;  // tau of reflector k, read back when the eigenvectors are formed
         }
         // v_k = (0.., 1, col[k+2..] sc), zero when no reflection is needed
         auto vk = [&](int i) {
@@ -917,6 +921,8 @@ __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev
     double tnorm;
     bool zero_block;
     const int ps = tri_eigen<NT>(d, dd, ee, e2, lamv, xs, red, lprev, Sv, fac, &st.status, tnorm, zero_block);
+    if (zero_block)  // T = 0: any orthonormal basis is an eigenbasis; keep one for the eigenvector formation
+        for (int e = t; e < d * d; e += NT) Sv[e] = (e / d == e % d) ? 1.0 : 0.0;
 
     // ---------------------------------------------------------------- T+ in registers
     const bool use_pos = zero_block || (d - ps) <= ps;
@@ -1321,26 +1327,26 @@ int project_tri_max_d(int cls) {
 }
 
 void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int nitems, double* work, double* lam,
-                        hipStream_t s) {
+                        const int* skip, hipStream_t s) {
     if (nitems <= 0) return;
     switch (cls) {
         case 0:
             hipLaunchKernelGGL(k_project_wave, dim3(nitems), dim3(kW), 0, s, B, items, work, lam);
             break;
         case 1:
-            hipLaunchKernelGGL((k_project_tri<512, 8, 4, kTileReg>), dim3(nitems), dim3(512), 0, s, B, items, work, lam);
+            hipLaunchKernelGGL((k_project_tri<512, 8, 4, kTileReg>), dim3(nitems), dim3(512), 0, s, B, items, work, lam, skip);
             break;
         case 2:
             hipLaunchKernelGGL((k_project_tri<768, 8, 6, kTileReg>), dim3(nitems), dim3(768), 0, s, B, items, work,
-                               lam);
+                               lam, skip);
             break;
         case 3:
             hipLaunchKernelGGL((k_project_tri<1024, 7, 7, kTileSplit>), dim3(nitems), dim3(1024), 0, s, B, items,
-                               work, lam);
+                               work, lam, skip);
             break;
         case 4:
             hipLaunchKernelGGL((k_project_tri<1024, 8, 8, kTileGlobal>), dim3(nitems), dim3(1024), 0, s, B, items,
-                               work, lam);
+                               work, lam, skip);
             break;
         default:
             break;

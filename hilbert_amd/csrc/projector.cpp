@@ -37,6 +37,7 @@ void Projector::build(const std::vector<std::vector<int>>& dims, std::vector<Ins
     int gblock = 0;
     int64_t rec = 0, mat = 0, work = 0, lam = 0;
     const int count = (int)dims.size();
+    std::vector<Refiner::Item> ritems;
     for (auto& c : cls_) c.items.clear();
     lam_begin_.assign(count + 1, 0);
     for (int i = 0; i < count; i++) {
@@ -62,6 +63,7 @@ void Projector::build(const std::vector<std::vector<int>>& dims, std::vector<Ins
                 work += tri_work(kClassTri[k], dim);
                 it.lam_off = lam;
                 lam += dim + 1;
+                if (refined_class(k)) ritems.push_back(Refiner::Item{i, dim, it.gblock, boff, it.work_off});
             } else {
                 int dp = dim + (dim & 1);
                 it.rec_off = rec;
@@ -102,18 +104,22 @@ void Projector::build(const std::vector<std::vector<int>>& dims, std::vector<Ins
     work_.alloc(std::max<int64_t>(work, 1));
     lam_.alloc(std::max<int64_t>(lam, 1));
     launch_fill(lam_.p, std::max<int64_t>(lam, 1), std::nan(""), s);
+    refiner_.build(ritems, count, gblock, s);
+    refiner_.set_work_base(work_.p);
 }
 
 void Projector::forget(int i, hipStream_t s) {
     if (lam_begin_[i + 1] > lam_begin_[i])
         launch_fill(lam_.p + lam_begin_[i], lam_begin_[i + 1] - lam_begin_[i], std::nan(""), s);
+    refiner_.forget(i, s);
 }
 
 void Projector::launch(int k, const BatchDev& B, hipStream_t s) const {
     const Class& c = cls_[k];
     if (c.items.empty()) return;
     if (c.tri >= 0)
-        launch_project_tri(c.tri, B, c.d_items.p, (int)c.items.size(), work_.p, lam_.p, s);
+        launch_project_tri(c.tri, B, c.d_items.p, (int)c.items.size(), work_.p, lam_.p,
+                           refined_class(k) && refine_on() ? refiner_.ok() : nullptr, s);
     else
         launch_project(c.nt, c.global, B, c.d_items.p, (int)c.items.size(), c.max_d, rec_.p, max_sweeps_, gmat_.p, s);
 }

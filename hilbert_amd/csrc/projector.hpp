@@ -10,6 +10,7 @@
 
 #include "common.hpp"
 #include "dbuf.hpp"
+#include "refine.hpp"
 
 namespace v2h {
 
@@ -25,9 +26,22 @@ class Projector {
     // forget instance i's eigenvalues (the warm start of the tridiagonal path)
     void forget(int i, hipStream_t s);
     bool has(int k) const { return !cls_[k].items.empty(); }
+    // k_project_tri of class k skips the blocks the refinement projected when
+    // the refinement is on
     void launch(int k, const BatchDev& B, hipStream_t s) const;
     int64_t nblocks() const { return nblocks_; }
     double* proj_part() const { return proj_part_.p; }
+
+    // Warm-started refinement (refine.hpp) for the tridiagonal classes of
+    // 65..256 rows: on by default.
+    static bool refined_class(int k) { return k >= 2 && k <= 5; }
+    bool refine_on() const { return refine_on_ && !refiner_.empty(); }
+    void set_refine(bool on) { refine_on_ = on; }
+    void refine_run(const BatchDev& B, hipStream_t s) const { refiner_.run(B, s); }
+    void refine_seed(const BatchDev& B, hipStream_t s) const { refiner_.seed(B, s); }
+    void refine_stats(int64_t* out) const { refiner_.stats(out); }
+    void set_refine_params(const RefParams& p) { refiner_.set_params(p); }
+    const RefParams& refine_params() const { return refiner_.params(); }
 
   private:
     struct Class {
@@ -42,6 +56,8 @@ class Projector {
     std::vector<int64_t> lam_begin_;  // [count + 1] eigenvalue-store range of each instance
     int64_t nblocks_ = 0;
     int max_sweeps_ = 24;
+    Refiner refiner_;
+    bool refine_on_ = true;
 };
 
 }  // namespace v2h

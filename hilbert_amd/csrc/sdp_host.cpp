@@ -62,8 +62,12 @@ void CallbackSdp::project(double* x) {
     B.v1 = d_v1_.p;
     B.c = d_c_.p;
     B.proj_part = proj_.proj_part();
+    // the warm-started refinement first; the tridiagonal classes take the
+    // blocks it did not accept, whose eigenvectors then seed the next call
+    if (proj_.refine_on()) proj_.refine_run(B, stream_);
     for (int k = 0; k < Projector::kClasses; k++)
         if (proj_.has(k)) proj_.launch(k, B, stream_);
+    if (proj_.refine_on()) proj_.refine_seed(B, stream_);
     V2H_HIP(hipGetLastError());
     V2H_HIP(hipMemcpyAsync(x, d_x_.p, bytes, hipMemcpyDeviceToHost, stream_));
     V2H_HIP(hipMemcpyAsync(z_.data(), d_z_.p, bytes, hipMemcpyDeviceToHost, stream_));

@@ -104,6 +104,7 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
         V2H_HIP(hipEventCreateWithFlags(&ev_proj_[k], hipEventDisableTiming));
     }
     V2H_HIP(hipEventCreateWithFlags(&ev_post_, hipEventDisableTiming));
+    V2H_HIP(hipEventCreateWithFlags(&ev_ref_, hipEventDisableTiming));
 
     std::map<std::string, int> sid_of;
     for (const auto& d : descs) {
@@ -195,6 +196,7 @@ Batch::~Batch() {
         if (ev_proj_[k]) (void)hipEventDestroy(ev_proj_[k]);
     }
     if (ev_post_) (void)hipEventDestroy(ev_post_);
+    if (ev_ref_) (void)hipEventDestroy(ev_ref_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -687,24 +689,7 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         launches_++;
         // the size classes are independent: largest first, on the streams of
         // kClassLane, joined back into stream_
-        V2H_HIP(hipEventRecord(ev_post_, stream_));
-        bool used[kSideStreams + 1] = {};
-        for (int k = kProjClasses - 1; k >= 0; k--) {
-            if (!proj_.has(k)) continue;
-            const int kind = 4 + k, ln = kClassLane[k];
-            hipStream_t ps = ln == 0 ? stream_ : pstream_[ln - 1];
-            if (ln > 0 && !used[ln]) V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
-            used[ln] = true;
-            tstart(kind, ps);
-            proj_.launch(k, B, ps);
-            tstop(kind, ps);
-            launches_++;
-        }
-        for (int ln = 1; ln <= kSideStreams; ln++)
-            if (used[ln]) {
-                V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
-                V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[ln - 1], 0));
-            }
+        launch_projection(B);
         V2H_HIP(hipGetLastError());
     }
     select(false);
@@ -721,6 +706,72 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     int nconv = 0;
     for (const auto& st : state_) nconv += st.converged;
     return nconv;
+}
+
+// The PSD projection of one outer iteration, after k_post on stream_.  The
+// warm-started refinement of the 65..256 blocks runs on side lane kRefLane;
+// the tridiagonal classes then take the blocks it did not accept (their
+// launches wait for it), every other class starts right after k_post; the
+// eigenvectors of the fallback blocks are formed once every fallback class
+// has finished, and stream_ joins all lanes.
+void Batch::launch_projection(const BatchDev& B) {
+    constexpr int rl = kRefLane;
+    V2H_HIP(hipEventRecord(ev_post_, stream_));
+    bool used[kSideStreams + 1] = {};
+    const bool refine = proj_.refine_on();
+    if (refine) {
+        hipStream_t ps = pstream_[rl - 1];
+        V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
+        used[rl] = true;
+        tstart(kKindRefine, ps);
+        proj_.refine_run(B, ps);
+        tstop(kKindRefine, ps);
+        launches_ += 15;
+        V2H_HIP(hipEventRecord(ev_ref_, ps));
+    }
+    for (int k = kProjClasses - 1; k >= 0; k--) {
+        if (!proj_.has(k)) continue;
+        const int kind = 4 + k, ln = kClassLane[k];
+        hipStream_t ps = ln == 0 ? stream_ : pstream_[ln - 1];
+        if (refine && Projector::refined_class(k)) {
+            if (ln != rl) V2H_HIP(hipStreamWaitEvent(ps, ev_ref_, 0));
+        } else if (ln > 0 && !used[ln]) {
+            V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
+        }
+        used[ln] = true;
+        tstart(kind, ps);
+        proj_.launch(k, B, ps);
+        tstop(kind, ps);
+        launches_++;
+    }
+    if (refine) {
+        hipStream_t ps = pstream_[rl - 1];
+        for (int ln = 1; ln <= kSideStreams; ln++)
+            if (ln != rl && used[ln]) {
+                V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
+                V2H_HIP(hipStreamWaitEvent(ps, ev_proj_[ln - 1], 0));
+            }
+        tstart(kKindSeed, ps);
+        proj_.refine_seed(B, ps);
+        tstop(kKindSeed, ps);
+        launches_++;
+        V2H_HIP(hipEventRecord(ev_proj_[rl - 1], ps));
+        V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[rl - 1], 0));
+        return;
+    }
+    for (int ln = 1; ln <= kSideStreams; ln++)
+        if (used[ln]) {
+            V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
+            V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[ln - 1], 0));
+        }
+}
+
+void Batch::set_refine(bool on) {
+    V2H_HIP(hipSetDevice(device_));
+    if (on && !proj_.refine_on())
+        for (int i = 0; i < count(); i++) proj_.forget(i, stream_);  // eigenvectors from before are stale
+    proj_.set_refine(on);
+    V2H_HIP(hipStreamSynchronize(stream_));
 }
 
 InstStatus Batch::status(int i) {

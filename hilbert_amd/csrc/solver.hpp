@@ -174,12 +174,18 @@ class Batch {
     // global memory); 11: the device part of one set_df_integrals (its
     // contraction kernels and the cost packing); 12: the pair Gram matrix of
     // write_fcidump; 13: the device work of optimize_orbitals outside the
-    // intake (densities, Fock and Hessian products, exponential, transform).
+    // intake (densities, Fock and Hessian products, exponential, transform);
+    // 14: the warm-started refinement of the 65..256 blocks (all its launches
+    // of one outer iteration); 15: the eigenvector formation of the blocks
+    // that fell back to the tridiagonal path.  Classes 2..5 then time only
+    // their fallback launches.
     static constexpr int kProjClasses = Projector::kClasses;
     static constexpr int kKindDf = 4 + kProjClasses;
     static constexpr int kKindGram = kKindDf + 1;
     static constexpr int kKindOrb = kKindGram + 1;
-    static constexpr int kKinds = kKindOrb + 1;
+    static constexpr int kKindRefine = kKindOrb + 1;
+    static constexpr int kKindSeed = kKindRefine + 1;
+    static constexpr int kKinds = kKindSeed + 1;
     // The projection classes run concurrently on stream_ (lane 0) and three
     // side streams: with the runtime's default of four hardware queues per
     // process, every stream of a batch keeps a queue of its own, so classes
@@ -188,6 +194,15 @@ class Batch {
     // to balance the per-iteration class times of the (16e,16o) workloads.
     static constexpr int kSideStreams = 3;
     static constexpr int kClassLane[kProjClasses] = {0, 0, 2, 3, 1, 1, 2};
+    static constexpr int kRefLane = 1;  // the warm-started refinement and the eigenvector formation
+    // Refined projection of the 65..256 blocks on (default) or off (every
+    // block takes the tridiagonal eigensolver).
+    void set_refine(bool on);
+    bool refine() const { return proj_.refine_on(); }
+    // [block projections by the refinement, by the fallback, rejections for
+    // divergence, correction size, coupled clusters]
+    void projection_stats(int64_t* out) const { proj_.refine_stats(out); }
+    void set_refine_params(const RefParams& p) { proj_.set_refine_params(p); }
     void set_profiling(bool on) { profile_ = on; }
     // ms/counts: all launches; full_ms/full_counts: launches made while every
     // instance of the batch was active (and, for CG kernels, still iterating)
@@ -210,6 +225,7 @@ class Batch {
     void sync_state_to_host();
     void sync_state_to_device();
     void refresh_norms(int i);
+    void launch_projection(const BatchDev& B);
     DfState& df_state(int i);
     double run_df_intake(int i, bool norms = true);  // resident DF integrals -> c, v1, E_fzc (and ||c||)
     void launch_df_intake_only(int i);    // the same launches, no read-back
@@ -244,7 +260,7 @@ class Batch {
     Projector proj_;
     // side streams of the projection, joined back into stream_ by events
     hipStream_t pstream_[kSideStreams] = {};
-    hipEvent_t ev_post_ = nullptr, ev_proj_[kSideStreams] = {};
+    hipEvent_t ev_post_ = nullptr, ev_proj_[kSideStreams] = {}, ev_ref_ = nullptr;
 
     struct KTimer {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;

@@ -259,7 +259,11 @@ int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
  * contraction kernel to the packed c), 12 the pair Gram matrix of
  * v2h_batch_write_fcidump, 13 the device work of v2h_batch_optimize_orbitals
  * besides the intake (densities, Fock and Hessian products, exponential,
- * integral transform).  Cumulative since the
+ * integral transform), 14 the warm-started refinement of the blocks of
+ * 65..256 rows (all its launches of one outer iteration), 15 the eigenvector
+ * formation of the blocks that took the tridiagonal fallback (with the
+ * refinement on, kinds 6..9 time only the fallback launches).  Arrays of 16
+ * entries.  Cumulative since the
  * last reset.  full_ms / full_counts (may be NULL) cover only the launches
  * made while every instance was running (for CG kernels: still iterating)
  * at the previous host poll, i.e. launches that processed the whole batch. */
@@ -269,6 +273,31 @@ int v2h_batch_reset_kernel_times(v2h_batch *b);
 /* Launch statistics: [kernel launches so far, A gather workgroups,
  * A^T gather workgroups, projection blocks] */
 int v2h_batch_launch_stats(v2h_batch *b, int64_t *stats);
+
+/* How the PSD blocks of 65..256 rows are projected (the eigendecomposition
+ * libsdp applies to every block of dimensions_, v2rdm_solver.cc:1340, block
+ * list v2rdm_solver.cc:3729-3846):
+ *   V2H_PROJECT_REFINED (default): the eigenvectors of the previous outer
+ *     iteration are refined by FP64 matrix-core products (iterative
+ *     refinement of Ogita and Aishima, at most three steps, accepted when the
+ *     last correction is below 3e-8); blocks without eigenvectors yet, or
+ *     whose refinement is not accepted, take the tridiagonal eigensolver,
+ *     whose eigenvectors seed the next iteration;
+ *   V2H_PROJECT_TRIDIAGONAL: every block takes the tridiagonal eigensolver.
+ * Smaller and larger blocks are not affected. */
+#define V2H_PROJECT_TRIDIAGONAL 0
+#define V2H_PROJECT_REFINED 1
+int v2h_batch_set_projection(v2h_batch *b, int mode);
+/* Over the blocks of 65..256 rows since the batch was created: [block
+ * projections done by the refinement, by the tridiagonal fallback, refinements
+ * rejected for a diverging or non-finite correction, for a correction still
+ * above accept_e after three steps, for clusters left coupled]. */
+int v2h_batch_projection_stats(v2h_batch *b, int64_t *stats);
+/* Thresholds of the refinement: eigenvalue estimates closer than cluster_rel
+ * x max|l| form a cluster (default 1e-8); a step is accepted when its largest
+ * correction is at most accept_e (3e-8) and the couplings left inside
+ * clusters at most accept_cpl x max|l| (1e-13). */
+int v2h_batch_set_refine_params(v2h_batch *b, double cluster_rel, double accept_e, double accept_cpl);
 
 /* Adjoint of the cost packing: dE/dh[nmo*nmo], dE/dg[nmo^4] (caller order)
  * for E = c . xbar; xbar = NULL uses the current primal solution x. */
