@@ -326,3 +326,42 @@ def optimize_orbitals(h, B, D, d, occ, pairs, g_convergence=1e-4, e_convergence=
     info = dict(iterations=iters, gnorm=gnorm, dE=final_E - initial_E,
                 converged=bool(converged_e and converged_g), energy=final_E, trace=trace)
     return h, B, Uacc, info
+
+
+def call_converged(info, g_convergence=1e-4, e_convergence=1e-8):
+    """The call-level test of RotateOrbitals (v2rdm_solver.cc:3271-3274): the
+    energy change of the whole optimize_orbitals call and its last gradient
+    norm, not the optimiser's per-step flags."""
+    return abs(info["dE"]) < e_convergence and info["gnorm"] < g_convergence
+
+
+def casscf_loop(run, h, B, core, active, pairs, orbopt_frequency=500, g_convergence=1e-4, e_convergence=1e-8,
+                maxiter=10, max_macro=1000):
+    """The v2RDM-CASSCF loop of compute_energy (v2rdm_solver.cc:1330-1361):
+    do { sdp_->solve(ORBOPT_FREQUENCY iterations); RotateOrbitals() } while
+    (!orbopt_converged_ || !sdp_->is_converged()), with orbopt_converged_ set
+    by the first call that passes the call-level test and never reset.
+
+    run: an SdpRun of the active-space problem (its x is the starting point).
+    Returns dict(energy = c . x + E_fzc after the last orbital call, macro =
+    orbital calls, sdp_converged, orbopt_converged, h, B, oiter)."""
+    from .pyoracle import df_integrals
+    P = run.P
+    occ = list(core) + list(active)
+    oo_conv = False
+    macro = 0
+    sdp_conv = False
+    info = None
+    while macro < max_macro:
+        ha, ga, efzc = df_integrals(h, pack_df(B), core, active)
+        sdp_conv = run.solve(P.pack_c(ha, ga), orbopt_frequency)
+        D1, d = spatial_rdms(P, run.x)
+        D, dd = occupied_rdms(D1, d, len(core))
+        h, B, _, info = optimize_orbitals(h, B, D, dd, occ, pairs, g_convergence, e_convergence, maxiter)
+        macro += 1
+        if call_converged(info, g_convergence, e_convergence):
+            oo_conv = True
+        if oo_conv and sdp_conv:
+            break
+    return dict(energy=info["energy"], macro=macro, sdp_converged=sdp_conv, orbopt_converged=oo_conv, h=h, B=B,
+                oiter=run.status()["oiter"])

@@ -321,6 +321,10 @@ class OracleProblem:
             raise ValueError("tpdm export is C1 only")
         return out.reshape(n * n, n * n)
 
+    def run(self, x=None, **opts):
+        """A BPSDP run whose y, z and mu persist across chunks (see SdpRun)."""
+        return SdpRun(self, x, **opts)
+
     # ---- solve --------------------------------------------------------------
     def solve(self, c, x=None, e_convergence=1e-6, r_convergence=1e-5, cg_convergence=1e-9,
               cg_maxiter=10000, maxiter=20000, mu_update_frequency=1000, mu0=1.0,
@@ -355,3 +359,41 @@ class OracleProblem:
         return {"x": x, "y": y, "z": z, "converged": bool(conv), "mu": sc[0],
                 "energy_primal": sc[1], "energy_dual": sc[2], "eps_p": sc[3], "eps_d": sc[4],
                 "cg_time": sc[5], "proj_time": sc[6], "oiter": int(cnt[0]), "iiter": int(cnt[1])}
+
+
+class SdpRun:
+    """BPSDP whose y, z and mu persist across solve() chunks, as libsdp keeps
+    them between sdp_->solve calls (checkpoint.cc:58-120), for loops that
+    change c between chunks (the v2RDM-CASSCF loop, v2rdm_solver.cc:1330-1361).
+    Every chunk iterates at least once and re-derives the norms of b and c."""
+
+    def __init__(self, P, x=None, e_convergence=1e-6, r_convergence=1e-5, cg_convergence=1e-9,
+                 cg_maxiter=10000, maxiter=20000, mu_update_frequency=1000, mu0=1.0, nthreads=None):
+        self.P = P
+        self.x = np.array(P.guess() if x is None else x, dtype=np.float64, copy=True)
+        self.b = P.b()
+        self.o = SdpOptions(e_convergence, r_convergence, cg_convergence, cg_maxiter, maxiter,
+                            mu_update_frequency, mu0, nthreads or (os.cpu_count() or 1))
+        self.s = lib().orc_sdp_state_new(P.n_primal, P.n_dual)
+        self.converged = False
+
+    def solve(self, c, maxiter):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self.converged = bool(lib().orc_solve_dqg(self.s, C.byref(self.o), self.P._p, self.x, self.b, c,
+                                                  int(maxiter), dsyevd_pointer(), None, 1))
+        return self.converged
+
+    def status(self):
+        sc = np.zeros(7)
+        cnt = np.zeros(3, dtype=np.int64)
+        lib().orc_sdp_state_scalars(self.s, sc, cnt)
+        return {"mu": sc[0], "energy_primal": sc[1], "energy_dual": sc[2], "eps_p": sc[3], "eps_d": sc[4],
+                "oiter": int(cnt[0]), "iiter": int(cnt[1]), "converged": self.converged}
+
+    def close(self):
+        if getattr(self, "s", None):
+            lib().orc_sdp_state_delete(self.s)
+            self.s = None
+
+    def __del__(self):
+        self.close()

@@ -127,10 +127,22 @@ def test_casscf_two_electrons_batched():
     for i, (h, qmo, core, active) in enumerate(models):
         e_bf = _brute_force_casscf(h, qmo, core, active)
         U, _, _ = B.df_orbitals(i)
+        # the oracle's restatement of the macro loop with the same settings
+        nfull = h.shape[0]
+        run = OracleProblem(2, 1, 1).run(e_convergence=1e-8, r_convergence=1e-7, maxiter=40000, nthreads=2)
+        ref = oo.casscf_loop(run, h, oo.unpack_df(qmo, nfull), core, active, oo.rotation_pairs(nfull, core, active),
+                             orbopt_frequency=200)
+        run.close()
         print("instance", i, "v2RDM-CASSCF", res[i]["energy"], "brute force", e_bf, "macro", res[i]["macro"],
-              "oiter", res[i]["oiter"], flush=True)
+              "oiter", res[i]["oiter"], "| oracle loop", ref["energy"], "macro", ref["macro"], "oiter", ref["oiter"],
+              flush=True)
         assert res[i]["orbopt_converged"] and res[i]["sdp_converged"]
+        assert ref["orbopt_converged"] and ref["sdp_converged"]
         assert abs(res[i]["energy"] - e_bf) < 1e-6
+        assert abs(res[i]["energy"] - ref["energy"]) < 1e-6
+        # the two BPSDP trajectories differ in their last bits, so the macro
+        # iteration at which both flags first hold may differ by a little
+        assert abs(res[i]["macro"] - ref["macro"]) <= 2
         assert np.abs(U.T @ U - np.eye(h.shape[0])).max() < 1e-12
     B.close()
 

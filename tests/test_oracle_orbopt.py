@@ -205,3 +205,30 @@ def test_two_electron_casscf_equals_brute_force(nfull, ncore):
     print("E(fixed orbitals)", e0, "CASSCF brute force", e_bf, "oracle v2RDM-CASSCF", e_v2)
     assert e_bf < e0 - 1e-4          # the orbitals do relax
     assert abs(e_v2 - e_bf) < 1e-7
+
+
+@pytest.mark.parametrize("nfull,ncore", [(4, 0), (5, 1)])
+def test_casscf_loop_reference_rule_reaches_brute_force(nfull, ncore):
+    """The restated macro loop (do { solve; RotateOrbitals } while
+    (!orbopt_converged_ || !is_converged()), orbopt_converged_ set by the
+    call-level test and never reset, v2rdm_solver.cc:1330-1361, 3271-3274)
+    ends with both flags set at the brute-force CASSCF energy (two electrons:
+    DQG is exact)."""
+    h, qmo, core, active = _cas_model(nfull, ncore, 31 + nfull)
+    B = oo.unpack_df(qmo, nfull)
+    pairs = oo.rotation_pairs(nfull, core, active)
+    run = OracleProblem(2, 1, 1).run(e_convergence=1e-9, r_convergence=1e-8, maxiter=400000, nthreads=2)
+    res = oo.casscf_loop(run, h, B, core, active, pairs, orbopt_frequency=200)
+    run.close()
+    e_bf = _brute_force_casscf(h, qmo, core, active)
+    print("macro", res["macro"], "oiter", res["oiter"], "E", res["energy"], "brute force", e_bf)
+    assert res["orbopt_converged"] and res["sdp_converged"] and res["macro"] < 1000
+    assert abs(res["energy"] - e_bf) < 1e-6
+
+
+def test_call_level_test_uses_the_whole_call():
+    """A call whose per-step flags hold but whose net change or last gradient
+    does not pass is not converged at the call level, and the reverse."""
+    assert oo.call_converged({"dE": -5e-9, "gnorm": 5e-5})
+    assert not oo.call_converged({"dE": -5e-9, "gnorm": 2e-4})
+    assert not oo.call_converged({"dE": -2e-8, "gnorm": 5e-5})
