@@ -142,6 +142,8 @@ SIGNATURES = {
     "v2h_structure_opdm": (C.c_int, [_vp, _dp, C.c_char_p, _dp]),
     "v2h_structure_tpdm": (C.c_int, [_vp, _dp, _dp]),
     "v2h_structure_hf_guess": (C.c_int, [_vp, _dp]),
+    "v2h_structure_write_rdm_files": (C.c_int, [_vp, _dp, C.c_int, _ip, _ip, C.c_char_p, C.c_int, _lp]),
+    "v2h_batch_write_rdm_files": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, C.c_char_p, C.c_int, _lp]),
 }
 
 _lib = None
@@ -167,6 +169,23 @@ def check(rc: int):
     if rc != V2H_OK:
         msg = lib().v2h_last_error().decode(errors="replace")
         raise V2hError(f"{ERRORS.get(rc, rc)}: {msg}")
+
+
+RDM_FILE_COUNTS = ("d1a", "d1b", "d2aa", "d2bb", "d2ab", "d2", "aa.iwl", "bb.iwl", "ab.iwl")
+# which files: hilbert's options that call each writer (v2rdm_solver.cc:1474-1509)
+RDM_TPDM_ACTIVE, RDM_TPDM_FULL, RDM_TPDM_SPIN_FREE, RDM_OPDM_FULL, RDM_TPDM_IWL = 1, 2, 4, 8, 16
+RDM_ALL_FULL = RDM_TPDM_FULL | RDM_TPDM_SPIN_FREE | RDM_OPDM_FULL | RDM_TPDM_IWL
+
+
+def write_rdm_files(fn, head, prefix, active, core, which):
+    """Call a *_write_rdm_files entry point; returns {file suffix: records}
+    for the files written."""
+    core = np.ascontiguousarray(core if core is not None else [], dtype=np.int32)
+    active = np.ascontiguousarray(active, dtype=np.int32)
+    counts = np.zeros(len(RDM_FILE_COUNTS), dtype=np.int64)
+    core_arg = core if len(core) else np.zeros(1, dtype=np.int32)
+    check(fn(*head, len(core), core_arg, active, os.fsencode(prefix), int(which), counts))
+    return {k: int(v) for k, v in zip(RDM_FILE_COUNTS, counts) if v >= 0}
 
 
 def make_desc(nmo, nalpha, nbeta, irreps=None, nirrep=None, multiplicity=0, q2=True, g2=True, spin=True,

@@ -303,7 +303,7 @@ class Batch:
         return dict(zip(("refined", "fallback", "rejected_divergence", "rejected_correction",
                          "rejected_clusters"), (int(v) for v in st)))
 
-    def set_refine_params(self, cluster_rel=1e-8, accept_e=3e-8, accept_cpl=1e-13):
+    def set_refine_params(self, cluster_rel=1e-8, accept_e=2e-7, accept_cpl=1e-13):
         N.check(N.lib().v2h_batch_set_refine_params(self._h, float(cluster_rel), float(accept_e),
                                                     float(accept_cpl)))
 
@@ -406,6 +406,19 @@ class Batch:
     def tpdm_sparse(self, i, type):
         """(indices [m, 4], values [m]) in the order of hilbert's get_tpdm_sparse."""
         return self._sparse(N.lib().v2h_batch_get_tpdm_sparse, i, type, 4)
+
+    def write_rdm_files(self, i, prefix, active, core=(), which=N.RDM_ALL_FULL):
+        """Instance i's RDM files for downstream codes (hilbert's
+        WriteActiveTPDM, WriteTPDM, WriteTPDMSpinFree, WriteOPDM and
+        WriteTPDM_IWL; write_tpdm.cc:40-595, write_opdm.cc:40,
+        write_tpdm_iwl.cc:45): <prefix>.d2aa .d2bb .d2ab .d2 .d1a .d1b (keyed
+        records) and <prefix>.aa.iwl .bb.iwl .ab.iwl (IWL buffers), chosen by
+        `which` (N.RDM_* flags).  `active[u]` is the full-space label of
+        active orbital u, `core` the labels of the doubly occupied core
+        orbitals.  Returns the record count of each file written."""
+        if len(active) != self.problems[i].nmo:
+            raise ValueError("one full-space label per active orbital needed")
+        return N.write_rdm_files(N.lib().v2h_batch_write_rdm_files, (self._h, i), prefix, active, core, which)
 
     def Au(self, i, u):
         out = np.zeros(self.n_dual[i])

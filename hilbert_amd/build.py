@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libv2rdm_hip.so")
 SOURCES = ["layout.cpp", "constraints.cpp", "sell.cpp", "rdm.cpp", "projector.cpp", "solver.cpp", "sdp_host.cpp",
-           "fcidump.cpp", "orbopt_driver.cpp", "abi.cpp",
+           "fcidump.cpp", "writers.cpp", "orbopt_driver.cpp", "abi.cpp",
            "kernels.hip",
            "eig_tri.hip", "refine.hip",
            "df.hip", "orbopt.hip"]

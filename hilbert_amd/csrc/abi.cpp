@@ -15,6 +15,7 @@
 #include "rdm.hpp"
 #include "sdp_host.hpp"
 #include "solver.hpp"
+#include "writers.hpp"
 
 struct v2h_batch {
     std::unique_ptr<v2h::Batch> impl;
@@ -479,6 +480,31 @@ int v2h_batch_Au(v2h_batch* b, int inst, const double* u, double* out) {
     });
 }
 
+namespace {
+v2h::FullSpace full_space(const v2h::Layout& L, int ncore, const int* core, const int* active) {
+    if (ncore < 0) throw std::invalid_argument("ncore must be non-negative");
+    if (ncore > 0 && !core) throw std::invalid_argument("null core list");
+    if (!active) throw std::invalid_argument("null active list");
+    v2h::FullSpace f;
+    f.core.assign(core, core + ncore);
+    f.active.assign(active, active + L.n);
+    return f;
+}
+}  // namespace
+
+int v2h_batch_write_rdm_files(v2h_batch* b, int inst, int ncore, const int* core, const int* active,
+                              const char* prefix, int which, int64_t* counts) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(prefix);
+        check_ptr(counts);
+        const v2h::Layout& L = B(b).layout(inst);
+        std::vector<double> x(B(b).n_primal(inst));
+        B(b).get_primal(inst, x.data());
+        v2h::write_rdm_files(L, x.data(), full_space(L, ncore, core, active), prefix, which, counts);
+    });
+}
+
 int v2h_batch_ATu(v2h_batch* b, int inst, const double* y, double* out) {
     return guarded([&] {
         check_inst(B(b), inst);
@@ -776,6 +802,17 @@ int v2h_structure_opdm(const v2h_structure* s, const double* x, const char* type
         check_ptr(type);
         check_ptr(out);
         v2h::opdm_dense(Sref(s).L, x, type, out);
+    });
+}
+
+int v2h_structure_write_rdm_files(const v2h_structure* s, const double* x, int ncore, const int* core,
+                                  const int* active, const char* prefix, int which, int64_t* counts) {
+    return guarded([&] {
+        check_ptr(x);
+        check_ptr(prefix);
+        check_ptr(counts);
+        const v2h::Layout& L = Sref(s).L;
+        v2h::write_rdm_files(L, x, full_space(L, ncore, core, active), prefix, which, counts);
     });
 }
 

@@ -584,7 +584,7 @@ constexpr int kTriWavesPerEU = NT == 512 ? 4 : 1;
 template <int NT, int RI, int CJ, int MODE>
 __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev B, const ProjItem* items,
                                                                       double* work, double* lamst,
-                                                                      const int* skip) {
+                                                                      const int* route, int want) {
     constexpr bool GM = MODE == kTileGlobal;
     constexpr int PR = NT / 32;
     constexpr int DMAX = PR * RI;
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev
     const ProjItem it = items[blockIdx.x];
     InstState& st = B.st[it.inst];
     if (!st.active) return;
-    if (skip && skip[it.gblock]) return;  // projected by the warm-started refinement
+    if (route && route[it.gblock] != want) return;  // not this launch's share of the refined classes
     const InstDev I = B.inst[it.inst];
     const int d = it.d;
     const int t = threadIdx.x, tr = t >> 5, tc = t & 31, lane = t & 63;
@@ -1327,26 +1327,26 @@ int project_tri_max_d(int cls) {
 }
 
 void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int nitems, double* work, double* lam,
-                        const int* skip, hipStream_t s) {
+                        const int* route, int want, hipStream_t s) {
     if (nitems <= 0) return;
     switch (cls) {
         case 0:
             hipLaunchKernelGGL(k_project_wave, dim3(nitems), dim3(kW), 0, s, B, items, work, lam);
             break;
         case 1:
-            hipLaunchKernelGGL((k_project_tri<512, 8, 4, kTileReg>), dim3(nitems), dim3(512), 0, s, B, items, work, lam, skip);
+            hipLaunchKernelGGL((k_project_tri<512, 8, 4, kTileReg>), dim3(nitems), dim3(512), 0, s, B, items, work, lam, route, want);
             break;
         case 2:
             hipLaunchKernelGGL((k_project_tri<768, 8, 6, kTileReg>), dim3(nitems), dim3(768), 0, s, B, items, work,
-                               lam, skip);
+                               lam, route, want);
             break;
         case 3:
             hipLaunchKernelGGL((k_project_tri<1024, 7, 7, kTileSplit>), dim3(nitems), dim3(1024), 0, s, B, items,
-                               work, lam, skip);
+                               work, lam, route, want);
             break;
         case 4:
             hipLaunchKernelGGL((k_project_tri<1024, 8, 8, kTileGlobal>), dim3(nitems), dim3(1024), 0, s, B, items,
-                               work, lam, skip);
+                               work, lam, route, want);
             break;
         default:
             break;

@@ -26,12 +26,12 @@ void set_project_lds_limit(size_t bytes);
 // Tridiagonal projection (eig_tri.hip), classes 0..4 for blocks up to
 // 64 / 128 / 192 / 224 / 256.  work: 7 d^2 doubles per block (class 4: plus
 // 1024 x 64 for the tile); lam: d + 1 doubles per block, kept between calls
-// (NaN = no previous eigenvalues).  skip (may be null, classes 1-4): blocks
-// with skip[global block] != 0 are left alone.  Each reflector's tau is kept
-// on the diagonal of the reflector matrix (row k, column k).
+// (NaN = no previous eigenvalues).  route (may be null, classes 1-4): only
+// blocks with route[global block] == want are projected.  Each reflector's tau
+// is kept on the diagonal of the reflector matrix (row k, column k).
 int project_tri_max_d(int cls);
 void launch_project_tri(int cls, const BatchDev& B, const ProjItem* items, int nitems, double* work, double* lam,
-                        const int* skip, hipStream_t s);
+                        const int* route, int want, hipStream_t s);
 
 void launch_fill(double* p, int64_t n, double v, hipStream_t s);
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);

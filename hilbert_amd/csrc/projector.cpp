@@ -114,12 +114,12 @@ void Projector::forget(int i, hipStream_t s) {
     refiner_.forget(i, s);
 }
 
-void Projector::launch(int k, const BatchDev& B, hipStream_t s) const {
+void Projector::launch(int k, const BatchDev& B, hipStream_t s, int route) const {
     const Class& c = cls_[k];
     if (c.items.empty()) return;
     if (c.tri >= 0)
         launch_project_tri(c.tri, B, c.d_items.p, (int)c.items.size(), work_.p, lam_.p,
-                           refined_class(k) && refine_on() ? refiner_.ok() : nullptr, s);
+                           route >= 0 && refined_class(k) && refine_on() ? refiner_.route() : nullptr, route, s);
     else
         launch_project(c.nt, c.global, B, c.d_items.p, (int)c.items.size(), c.max_d, rec_.p, max_sweeps_, gmat_.p, s);
 }

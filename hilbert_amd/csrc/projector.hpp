@@ -26,9 +26,11 @@ class Projector {
     // forget instance i's eigenvalues (the warm start of the tridiagonal path)
     void forget(int i, hipStream_t s);
     bool has(int k) const { return !cls_[k].items.empty(); }
-    // k_project_tri of class k skips the blocks the refinement projected when
-    // the refinement is on
-    void launch(int k, const BatchDev& B, hipStream_t s) const;
+    // The launch of class k.  With the refinement on, a refined class (2..5)
+    // is launched twice per outer iteration: route kRouteFallback (blocks known
+    // to need the tridiagonal eigensolver when the iteration starts, alongside
+    // the refinement) and kRouteRejected (after it); route < 0: every block.
+    void launch(int k, const BatchDev& B, hipStream_t s, int route = -1) const;
     int64_t nblocks() const { return nblocks_; }
     double* proj_part() const { return proj_part_.p; }
 
@@ -37,7 +39,8 @@ class Projector {
     static bool refined_class(int k) { return k >= 2 && k <= 5; }
     bool refine_on() const { return refine_on_ && !refiner_.empty(); }
     void set_refine(bool on) { refine_on_ = on; }
-    void refine_run(const BatchDev& B, hipStream_t s) const { refiner_.run(B, s); }
+    void refine_begin(const BatchDev& B, hipStream_t s) const { refiner_.begin(B, s); }
+    void refine_run(const BatchDev& B, hipStream_t s) const { refiner_.refine(B, s); }
     void refine_seed(const BatchDev& B, hipStream_t s) const { refiner_.seed(B, s); }
     void refine_stats(int64_t* out) const { refiner_.stats(out); }
     void set_refine_params(const RefParams& p) { refiner_.set_params(p); }

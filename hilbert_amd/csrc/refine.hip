@@ -97,12 +97,15 @@ __device__ void ref_tile(const BatchDev& B, const RefDev& R, const Prod& p, cons
     const int wm = (w >> 1) * 16, wn = (w & 1) * 16;
     const int li = l & 15, lk = l >> 4;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < d; k0 += kKc) {
+    // the loads of a chunk go to registers first, so that the next chunk's
+    // loads are in flight during this chunk's products
+    constexpr int kPer = (kT * kKc) / kNT;
+    double ra[kPer], rb[kPer];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < (kT * kKc) / kNT; e++) {
+        for (int e = 0; e < kPer; e++) {
             const int idx = t + kNT * e;
-            // A chunk: rows m0.., columns k0.. of op(A); consecutive threads
-            // read consecutive addresses of the stored matrix
+            // consecutive threads read consecutive addresses of the stored matrix
             int m, kk;
             if (p.am == kAT) {
                 m = idx & 31;
@@ -117,7 +120,7 @@ __device__ void ref_tile(const BatchDev& B, const RefDev& R, const Prod& p, cons
                 av = p.am == kAT ? A[(int64_t)gk * d + gm] : A[(int64_t)gm * d + gk];
                 if (p.am == kAScaled) av *= fmax(sc[gk], 0.0) * inv_mu;
             }
-            As[kk][m] = av;
+            ra[e] = av;
             int n, kb;
             if (p.bm == kBT) {
                 kb = idx & 31;
@@ -127,17 +130,38 @@ __device__ void ref_tile(const BatchDev& B, const RefDev& R, const Prod& p, cons
                 kb = idx >> 5;
             }
             const int gn = n0 + n, gkb = k0 + kb;
-            double bv = 0.0;
-            if (gn < d && gkb < d) bv = p.bm == kBT ? Bm[(int64_t)gn * d + gkb] : Bm[(int64_t)gkb * d + gn];
-            Bs[kb][n] = bv;
+            rb[e] = (gn < d && gkb < d) ? (p.bm == kBT ? Bm[(int64_t)gn * d + gkb] : Bm[(int64_t)gkb * d + gn]) : 0.0;
         }
-        __syncthreads();
+    };
+    auto stage = [&]() {
 #pragma unroll
-        for (int kk = 0; kk < kKc; kk += 4) {
-            const double a = As[kk + lk][wm + li];
-            const double bb = Bs[kk + lk][wn + li];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+        for (int e = 0; e < kPer; e++) {
+            const int idx = t + kNT * e;
+            if (p.am == kAT)
+                As[idx >> 5][idx & 31] = ra[e];
+            else
+                As[idx & 31][idx >> 5] = ra[e];
+            if (p.bm == kBT)
+                Bs[idx & 31][idx >> 5] = rb[e];
+            else
+                Bs[idx >> 5][idx & 31] = rb[e];
         }
+    };
+    // a wavefront whose 16 x 16 part lies outside the block only helps load
+    const bool live = (m0 + wm < d) && (n0 + wn < d);
+    fetch(0);
+    for (int k0 = 0; k0 < d; k0 += kKc) {
+        stage();
+        __syncthreads();
+        if (k0 + kKc < d) fetch(k0 + kKc);
+        // k steps of 4 up to the end of the matrix (zero-filled beyond d)
+        const int kend = min(kKc, (d - k0 + 3) & ~3);
+        if (live)
+            for (int kk = 0; kk < kend; kk += 4) {
+                const double a = As[kk + lk][wm + li];
+                const double bb = Bs[kk + lk][wn + li];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
+            }
         __syncthreads();
     }
     if (p.epi != kEpiFinal) {
@@ -189,134 +213,160 @@ __device__ void ref_tile(const BatchDev& B, const RefDev& R, const Prod& p, cons
     }
 }
 
-// Persistent over the tile jobs: workgroup g takes jobs g, g + grid, ...;
-// blockIdx.y selects the product.  Tiles of blocks the gate excludes cost one
-// state read, not a workgroup dispatch.
+// One workgroup per tile job; blockIdx.y selects the product.  The jobs of a
+// block are contiguous and workgroups are dispatched round-robin over the 8
+// XCDs, so job j = the XCD-major remap of the workgroup index: the tiles of one
+// block run on one XCD and share its L2.
+__device__ __forceinline__ int xcd_job(int bid, int n) {
+    const int per = n >> 3, rem = n & 7;
+    const int x = bid & 7, loc = bid >> 3;
+    return x * per + (x < rem ? x : rem) + loc;
+}
+
 __global__ __launch_bounds__(kNT) void k_ref_gemm(BatchDev B, RefDev R, Prod p0, Prod p1, int gate, int k) {
     __shared__ double As[kKc][kT + 1];  // As[k][m]
     __shared__ double Bs[kKc][kT + 1];  // Bs[k][n]
     __shared__ double red[4];
-    const Prod p = blockIdx.y == 0 ? p0 : p1;
-    for (int j = blockIdx.x; j < R.njobs; j += gridDim.x) {
-        const RefJob job = R.jobs[j];
-        const RefState& rs = R.rs[job.blk];
-        if (!gate_ok(rs, gate, k)) continue;  // uniform over the workgroup
-        const RefBlock b = R.blk[job.blk];
-        ref_tile(B, R, p, job, b, rs, As, Bs, red);
-        __syncthreads();  // the LDS of this tile is free for the next
-    }
+    const RefJob job = R.jobs[xcd_job(blockIdx.x, gridDim.x)];
+    const RefState& rs = R.rs[job.blk];
+    if (!gate_ok(rs, gate, k)) return;
+    const RefBlock b = R.blk[job.blk];
+    ref_tile(B, R, blockIdx.y == 0 ? p0 : p1, job, b, rs, As, Bs, red);
 }
 
 // ---------------------------------------------------------------- per block
-// Start of an outer iteration: sym(U) into its slot, phase, ok flag.
-__global__ __launch_bounds__(kNT) void k_ref_begin(BatchDev B, RefDev R) {
-    __shared__ double tt[32][33];
-    const RefBlock b = R.blk[blockIdx.x];
-    RefState& rs = R.rs[blockIdx.x];
+// Start of an outer iteration, one thread per block: phase, backoff, route.
+__global__ void k_ref_state(BatchDev B, RefDev R) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R.nblk) return;
+    const RefBlock b = R.blk[i];
+    RefState& rs = R.rs[i];
     const bool active = B.st[b.inst].active != 0;
     const int wait = rs.wait;
     const bool run = active && rs.vvalid && wait == 0;
-    __syncthreads();  // every thread has read the state before thread 0 changes it
-    if (threadIdx.x == 0) {
-        R.ok[b.gblock] = 0;
-        rs.phase = run ? 1 : 0;
-        if (active && wait > 0) rs.wait = wait - 1;
-    }
-    if (!run) return;
-    // sym(U) tile by tile: the transposed tile goes through LDS, so that both
-    // reads are coalesced
-    const int d = b.d, t = threadIdx.x, tc = t & 31, tr = t >> 5;
-    const double* U = B.u + B.inst[b.inst].poff + b.off;
-    double* Us = slot(R, b, kSlotU);
-    for (int bi = 0; bi < b.nt; bi++)
-        for (int bj = 0; bj < b.nt; bj++) {
-            // tt = tile (bj, bi) of U, transposed on the way out
-            for (int r = tr; r < 32; r += kNT / 32) {
-                const int gi = bj * 32 + r, gj = bi * 32 + tc;
-                tt[r][tc] = (gi < d && gj < d) ? U[(int64_t)gi * d + gj] : 0.0;
-            }
-            __syncthreads();
-            for (int r = tr; r < 32; r += kNT / 32) {
-                const int gi = bi * 32 + r, gj = bj * 32 + tc;
-                if (gi < d && gj < d) Us[(int64_t)gi * d + gj] = 0.5 * (U[(int64_t)gi * d + gj] + tt[tc][r]);
-            }
-            __syncthreads();
-        }
+    rs.phase = run ? 1 : 0;
+    if (active && wait > 0) rs.wait = wait - 1;
+    R.route[b.gblock] = run ? kRouteRefining : kRouteFallback;
 }
 
-// Eigenvalue estimates, corrections E (over G) and acceptance of step k.
-__global__ __launch_bounds__(kNT) void k_ref_decide(RefDev R, RefParams P, int k, int xout) {
-    __shared__ double ls[256];
+// sym(U) of one 32 x 32 tile of a refining block; the transposed tile goes
+// through LDS, so that both reads are coalesced.
+__global__ __launch_bounds__(kNT) void k_ref_sym(BatchDev B, RefDev R) {
+    __shared__ double tt[32][33];
+    const RefJob job = R.jobs[xcd_job(blockIdx.x, gridDim.x)];
+    if (R.rs[job.blk].phase != 1) return;
+    const RefBlock b = R.blk[job.blk];
+    const int d = b.d, tc = threadIdx.x & 31, tr = threadIdx.x >> 5;
+    const double* U = B.u + B.inst[b.inst].poff + b.off;
+    double* Us = slot(R, b, kSlotU);
+    for (int r = tr; r < 32; r += kNT / 32) {  // tile (tj, ti) of U
+        const int gi = job.tj * 32 + r, gj = job.ti * 32 + tc;
+        tt[r][tc] = (gi < d && gj < d) ? U[(int64_t)gi * d + gj] : 0.0;
+    }
+    __syncthreads();
+    for (int r = tr; r < 32; r += kNT / 32) {
+        const int gi = job.ti * 32 + r, gj = job.tj * 32 + tc;
+        if (gi < d && gj < d) Us[(int64_t)gi * d + gj] = 0.5 * (U[(int64_t)gi * d + gj] + tt[tc][r]);
+    }
+}
+
+// Corrections E of one 32 x 32 tile from S = X^T U X and G = X^T X, with the
+// eigenvalue estimates l_i = s_ii / g_ii of the block; E goes to slot K (free
+// once S is formed; G and S stay intact while other tiles read their
+// diagonals); the tile's largest correction (separated pairs) and largest
+// coupling left inside clusters go to its partial slots; diagonal tiles store
+// their l.
+__global__ __launch_bounds__(kNT) void k_ref_etile(RefDev R, RefParams P) {
+    __shared__ double li_s[32], lj_s[32];
     __shared__ double red[4];
-    const RefBlock b = R.blk[blockIdx.x];
-    RefState& rs = R.rs[blockIdx.x];
-    if (rs.phase != 1) return;
+    const RefJob job = R.jobs[xcd_job(blockIdx.x, gridDim.x)];
+    if (R.rs[job.blk].phase != 1) return;
+    const RefBlock b = R.blk[job.blk];
     const int d = b.d, t = threadIdx.x;
     const double* S = slot(R, b, kSlotS);
-    double* G = slot(R, b, kSlotG);
-    double* lam = lam_of(R, b);
+    const double* G = slot(R, b, kSlotG);
+    double* E = slot(R, b, kSlotK);
+    // block scale max |l|: every tile reads the d diagonal pairs itself
     double scale = 0.0;
-    for (int i = t; i < d; i += kNT) {
-        const double v = S[(int64_t)i * d + i] / G[(int64_t)i * d + i];  // s_ii / (1 - r_ii)
-        ls[i] = v;
-        scale = nmax(scale, fabs(v));
-    }
-    scale = wg_nmax(scale, red);  // also orders the ls stores before their reads
+    for (int i = t; i < d; i += kNT) scale = nmax(scale, fabs(S[(int64_t)i * d + i] / G[(int64_t)i * d + i]));
+    scale = wg_nmax(scale, red);
     const double om = P.cluster_rel * scale;
+    const int i0 = job.ti * 32, j0 = job.tj * 32;
+    if (t < 32) {
+        const int i = i0 + t, j = j0 + t;
+        li_s[t] = i < d ? S[(int64_t)i * d + i] / G[(int64_t)i * d + i] : 0.0;
+        lj_s[t] = j < d ? S[(int64_t)j * d + j] / G[(int64_t)j * d + j] : 0.0;
+        if (job.ti == job.tj && i < d) lam_of(R, b)[i] = li_s[t];
+    }
+    __syncthreads();
     double me = 0.0, mc = 0.0;
-    // thread t owns column t (d <= 256 = kNT); rows in order
-    if (t < d) {
-        const int j = t;
-        const double lj = ls[j];
-        for (int i = 0; i < d; i++) {
-            const int64_t e = (int64_t)i * d + j;
-            const double r = (i == j ? 1.0 : 0.0) - G[e];
-            double ev;
-            if (i == j) {
-                ev = 0.5 * r;
+    const int c = t & 31;
+    const int j = j0 + c;
+    for (int r = t >> 5; r < 32; r += kNT / 32) {
+        const int i = i0 + r;
+        if (i >= d || j >= d) continue;
+        const int64_t e = (int64_t)i * d + j;
+        const double rr = (i == j ? 1.0 : 0.0) - G[e];
+        double ev;
+        if (i == j) {
+            ev = 0.5 * rr;
+        } else {
+            const double dl = lj_s[c] - li_s[r];
+            const double sv = S[e];
+            if (fabs(dl) > om) {
+                ev = (sv + lj_s[c] * rr) / dl;
+                me = nmax(me, fabs(ev));
             } else {
-                const double dl = lj - ls[i];
-                const double s = S[e];
-                if (fabs(dl) > om) {
-                    ev = (s + lj * r) / dl;
-                    me = nmax(me, fabs(ev));
-                } else {
-                    ev = 0.5 * r;
-                    mc = nmax(mc, fabs(s));
-                }
+                ev = 0.5 * rr;
+                mc = nmax(mc, fabs(sv));
             }
-            G[e] = ev;
         }
+        E[e] = ev;
     }
     me = wg_nmax(me, red);
     mc = wg_nmax(mc, red);
-    for (int i = t; i < d; i += kNT) lam[i] = ls[i];
     if (t == 0) {
-        const double cpl = scale > 0.0 ? mc / scale : mc;
-        rs.max_e = me;
-        rs.cpl = cpl;
-        bool reject = false;
-        if (!(me <= 0.25) || !(cpl == cpl) || !(scale == scale) || isinf(scale)) {
-            reject = true;
-            rs.rej_div += 1;
-        } else if (me <= P.accept_e && cpl <= P.accept_cpl) {
-            rs.phase = 2;
-            rs.cstep = k;
-            rs.xfinal = xout;
-            rs.backoff = 0;
-        } else if (k >= 3) {
-            reject = true;
-            if (me > P.accept_e)
-                rs.rej_e += 1;
-            else
-                rs.rej_c += 1;
-        }
-        if (reject) {
-            rs.phase = 3;
-            const int bo = min(2 * rs.backoff + 2, kMaxBackoff);
-            rs.backoff = bo;
-            rs.wait = bo;
-        }
+        R.part[b.part + 2 * job.tile] = me;
+        R.part[b.part + 2 * job.tile + 1] = scale > 0.0 ? mc / scale : mc;
+    }
+}
+
+// Acceptance of step k from the tile maxima, one thread per block.
+__global__ void k_ref_judge(RefDev R, RefParams P, int k, int xout) {
+    const int bi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi >= R.nblk) return;
+    RefState& rs = R.rs[bi];
+    if (rs.phase != 1) return;
+    const RefBlock b = R.blk[bi];
+    double me = 0.0, cpl = 0.0;
+    for (int q = 0; q < b.nt * b.nt; q++) {
+        me = nmax(me, R.part[b.part + 2 * q]);
+        cpl = nmax(cpl, R.part[b.part + 2 * q + 1]);
+    }
+    rs.max_e = me;
+    rs.cpl = cpl;
+    bool reject = false;
+    if (!(me <= 0.25) || !(cpl == cpl)) {
+        reject = true;
+        rs.rej_div += 1;
+    } else if (me <= P.accept_e && cpl <= P.accept_cpl) {
+        rs.phase = 2;
+        rs.cstep = k;
+        rs.xfinal = xout;
+        rs.backoff = 0;
+    } else if (k >= 3) {
+        reject = true;
+        if (me > P.accept_e)
+            rs.rej_e += 1;
+        else
+            rs.rej_c += 1;
+    }
+    if (reject) {
+        rs.phase = 3;
+        const int bo = min(2 * rs.backoff + 2, kMaxBackoff);
+        rs.backoff = bo;
+        rs.wait = bo;
+        R.route[b.gblock] = kRouteRejected;
     }
 }
 
@@ -341,7 +391,7 @@ __global__ __launch_bounds__(kNT) void k_ref_close(BatchDev B, RefDev R) {
         B.proj_part[2 * (size_t)b.gblock] = ex;
         B.proj_part[2 * (size_t)b.gblock + 1] = dr;
         B.st[b.inst].pending = 1;
-        R.ok[b.gblock] = 1;
+        R.route[b.gblock] = kRouteAccepted;
         rs.refined += 1;
     }
 }
@@ -359,7 +409,8 @@ __global__ __launch_bounds__(kNT) void k_ref_form_v(BatchDev B, RefDev R) {
     const RefBlock b = R.blk[blockIdx.x];
     RefState& rs = R.rs[blockIdx.x];
     // blocks that took the fallback and try the refinement next iteration
-    if (!B.st[b.inst].active || R.ok[b.gblock] || rs.wait > 0) return;
+    const int route = R.route[b.gblock];
+    if (!B.st[b.inst].active || (route != kRouteFallback && route != kRouteRejected) || rs.wait > 0) return;
     const int d = b.d, t = threadIdx.x;
     const int64_t dd = (int64_t)d * d;
     const double* hv = R.tri_work + b.tri;  // row k: reflector k (v[k + 1] = 1), tau_k at [k][k]
@@ -442,12 +493,14 @@ __global__ void k_ref_forget(RefDev R, int b0, int b1) {
 
 }  // namespace
 
-// Workgroups of a persistent product launch: enough to fill every CU a few
-// times over, never more than there are tiles.
-int gemm_grid(const RefDev& R) { return std::max(1, std::min(R.njobs, 256 * 5)); }
+int gemm_grid(const RefDev& R) { return R.njobs; }
 
-void launch_ref_begin(const BatchDev& B, const RefDev& R, hipStream_t s) {
-    if (R.nblk > 0) hipLaunchKernelGGL(k_ref_begin, dim3(R.nblk), dim3(kNT), 0, s, B, R);
+void launch_ref_state(const BatchDev& B, const RefDev& R, hipStream_t s) {
+    if (R.nblk > 0) hipLaunchKernelGGL(k_ref_state, dim3((R.nblk + 255) / 256), dim3(256), 0, s, B, R);
+}
+
+void launch_ref_sym(const BatchDev& B, const RefDev& R, hipStream_t s) {
+    if (R.nblk > 0) hipLaunchKernelGGL(k_ref_sym, dim3(R.njobs), dim3(kNT), 0, s, B, R);
 }
 
 void launch_ref_step(const BatchDev& B, const RefDev& R, const RefParams& P, int k, hipStream_t s) {
@@ -462,9 +515,10 @@ void launch_ref_step(const BatchDev& B, const RefDev& R, const RefParams& P, int
     // S = X^T K
     const Prod ps{xin, kSlotK, kSlotS, -1, kAT, kBN, kEpiStore, 0};
     hipLaunchKernelGGL(k_ref_gemm, dim3(g, 1), dim3(kNT), 0, s, B, R, ps, ps, (int)kGateRefining, k);
-    hipLaunchKernelGGL(k_ref_decide, dim3(R.nblk), dim3(kNT), 0, s, R, P, k, xout);
-    // X_out = X + X E
-    const Prod pu{xin, kSlotG, xout, xin, kAN, kBN, kEpiAdd, 0};
+    hipLaunchKernelGGL(k_ref_etile, dim3(R.njobs), dim3(kNT), 0, s, R, P);
+    hipLaunchKernelGGL(k_ref_judge, dim3((R.nblk + 255) / 256), dim3(256), 0, s, R, P, k, xout);
+    // X_out = X + X E (E in slot K)
+    const Prod pu{xin, kSlotK, xout, xin, kAN, kBN, kEpiAdd, 0};
     hipLaunchKernelGGL(k_ref_gemm, dim3(g, 1), dim3(kNT), 0, s, B, R, pu, pu, (int)kGateUpdate, k);
 }
 
@@ -514,8 +568,8 @@ void Refiner::build(const std::vector<Item>& items, int count, int nblocks_total
     d_jobs_.upload(jobs);
     d_rs_.alloc(std::max<size_t>(blk_.size(), 1));
     V2H_HIP(hipMemsetAsync(d_rs_.p, 0, sizeof(RefState) * d_rs_.n, s));
-    ok_.alloc(std::max(nblocks_total, 1));
-    V2H_HIP(hipMemsetAsync(ok_.p, 0, sizeof(int) * ok_.n, s));
+    route_.alloc(std::max(nblocks_total, 1));
+    V2H_HIP(hipMemsetAsync(route_.p, 0, sizeof(int) * route_.n, s));
     work_.alloc(std::max<int64_t>(w, 1));
     part_.alloc(std::max<int64_t>(part, 1));
     V2H_HIP(hipStreamSynchronize(s));
@@ -526,7 +580,7 @@ RefDev Refiner::dev() const {
     R.blk = d_blk_.p;
     R.jobs = d_jobs_.p;
     R.rs = d_rs_.p;
-    R.ok = ok_.p;
+    R.route = route_.p;
     R.work = work_.p;
     R.part = part_.p;
     R.tri_work = tri_work_;
@@ -535,10 +589,14 @@ RefDev Refiner::dev() const {
     return R;
 }
 
-void Refiner::run(const BatchDev& B, hipStream_t s) const {
+void Refiner::begin(const BatchDev& B, hipStream_t s) const {
+    if (!blk_.empty()) launch_ref_state(B, dev(), s);
+}
+
+void Refiner::refine(const BatchDev& B, hipStream_t s) const {
     if (blk_.empty()) return;
     const RefDev R = dev();
-    launch_ref_begin(B, R, s);
+    launch_ref_sym(B, R, s);
     for (int k = 1; k <= 3; k++) launch_ref_step(B, R, params_, k, s);
     launch_ref_final(B, R, s);
 }

@@ -67,24 +67,33 @@ struct RefDev {
     const RefBlock* blk;
     const RefJob* jobs;
     RefState* rs;
-    int* ok;            // [global block]: 1 when this iteration's projection came from the refinement
+    int* route;         // [global block]: how this iteration projects the block (kRoute*)
     double* work;
     double* part;
     const double* tri_work;
     int nblk, njobs;
 };
 
+// Route of a block of the refined classes in the current outer iteration:
+// the tridiagonal fallback known at its start (no eigenvectors, backoff),
+// refining, rejected during the refinement (fallback after it), accepted.
+enum Route : int { kRouteFallback = 0, kRouteRefining = 1, kRouteRejected = 2, kRouteAccepted = 3 };
+
 // Thresholds of the refinement.
-// A step whose largest correction is eps leaves off-diagonal couplings of
-// about eps^2 max|l| and an orthogonality error of about eps^2, so U+ is then
-// accurate to about eps^2 relative (1e-12 at the default).
+// A step whose largest correction is eps leaves off-diagonal couplings and an
+// orthogonality error of order eps^2 (measured: U+ within 7.5e-11 relative of
+// LAPACK at eps = 1e-6 on (16e,16o) blocks, so the default keeps it near
+// 3e-12).
 struct RefParams {
     double cluster_rel = 1e-8;   // |l_i - l_j| <= cluster_rel max|l|: one cluster
-    double accept_e = 1e-6;      // accept a step whose largest correction is below this
+    double accept_e = 2e-7;      // accept a step whose largest correction is below this
     double accept_cpl = 1e-13;   // ... and whose clusters are decoupled to this (relative to max|l|)
 };
 
-void launch_ref_begin(const BatchDev& B, const RefDev& R, hipStream_t s);
+// Start of an outer iteration: phases, backoff and routes of every block.
+void launch_ref_state(const BatchDev& B, const RefDev& R, hipStream_t s);
+// sym(U) of the refining blocks.
+void launch_ref_sym(const BatchDev& B, const RefDev& R, hipStream_t s);
 // One refinement step k (1..3): X_in = V (k = 1), Xa (k = 2), Xb (k = 3).
 void launch_ref_step(const BatchDev& B, const RefDev& R, const RefParams& P, int k, hipStream_t s);
 // x = X diag(l+) X^T / mu with z, v1, partial sums; the block's projection sums.
@@ -102,12 +111,16 @@ class Refiner {
     };
     void build(const std::vector<Item>& items, int count, int nblocks_total, hipStream_t s);
     bool empty() const { return blk_.empty(); }
-    // The pipeline of one outer iteration (before the tridiagonal fallback).
-    void run(const BatchDev& B, hipStream_t s) const;
+    // One outer iteration: begin() sets every block's route (the fallback
+    // launches for kRouteFallback may start right after it); refine() runs
+    // the refinement and routes its rejections to kRouteRejected.
+    void begin(const BatchDev& B, hipStream_t s) const;
+    void refine(const BatchDev& B, hipStream_t s) const;
+    static constexpr int kLaunches = 18;  // kernel launches of begin() + refine()
     // After the fallback: eigenvectors of the blocks it projected.
     void seed(const BatchDev& B, hipStream_t s) const;
     void forget(int inst, hipStream_t s) const;
-    const int* ok() const { return ok_.p; }
+    const int* route() const { return route_.p; }
     // [block projections by the refinement, by the fallback, rejections for
     // divergence, for the correction size, for coupled clusters]
     static constexpr int kStats = 5;
@@ -125,7 +138,7 @@ class Refiner {
     DBuf<RefBlock> d_blk_;
     DBuf<RefJob> d_jobs_;
     DBuf<RefState> d_rs_;
-    DBuf<int> ok_;
+    DBuf<int> route_;
     DBuf<double> work_, part_;
     const double* tri_work_ = nullptr;
 };

@@ -64,10 +64,21 @@ void CallbackSdp::project(double* x) {
     B.proj_part = proj_.proj_part();
     // the warm-started refinement first; the tridiagonal classes take the
     // blocks it did not accept, whose eigenvectors then seed the next call
-    if (proj_.refine_on()) proj_.refine_run(B, stream_);
-    for (int k = 0; k < Projector::kClasses; k++)
-        if (proj_.has(k)) proj_.launch(k, B, stream_);
-    if (proj_.refine_on()) proj_.refine_seed(B, stream_);
+    const bool refine = proj_.refine_on();
+    if (refine) {
+        proj_.refine_begin(B, stream_);
+        proj_.refine_run(B, stream_);
+    }
+    for (int k = 0; k < Projector::kClasses; k++) {
+        if (!proj_.has(k)) continue;
+        if (refine && Projector::refined_class(k)) {
+            proj_.launch(k, B, stream_, kRouteFallback);
+            proj_.launch(k, B, stream_, kRouteRejected);
+        } else {
+            proj_.launch(k, B, stream_);
+        }
+    }
+    if (refine) proj_.refine_seed(B, stream_);
     V2H_HIP(hipGetLastError());
     V2H_HIP(hipMemcpyAsync(x, d_x_.p, bytes, hipMemcpyDeviceToHost, stream_));
     V2H_HIP(hipMemcpyAsync(z_.data(), d_z_.p, bytes, hipMemcpyDeviceToHost, stream_));

@@ -105,6 +105,8 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     }
     V2H_HIP(hipEventCreateWithFlags(&ev_post_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_ref_, hipEventDisableTiming));
+    V2H_HIP(hipEventCreateWithFlags(&ev_state_, hipEventDisableTiming));
+    V2H_HIP(hipEventCreateWithFlags(&ev_seed_, hipEventDisableTiming));
 
     std::map<std::string, int> sid_of;
     for (const auto& d : descs) {
@@ -197,6 +199,8 @@ Batch::~Batch() {
     }
     if (ev_post_) (void)hipEventDestroy(ev_post_);
     if (ev_ref_) (void)hipEventDestroy(ev_ref_);
+    if (ev_state_) (void)hipEventDestroy(ev_state_);
+    if (ev_seed_) (void)hipEventDestroy(ev_seed_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -697,6 +701,7 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     launch_pre(B, units_a(), stream_);  // book-keeping of the last iteration
     tstop(0);
     launches_++;
+    join_seed();
     V2H_HIP(hipGetLastError());
     sync_state_to_host();
     if (mon)
@@ -708,62 +713,86 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     return nconv;
 }
 
-// The PSD projection of one outer iteration, after k_post on stream_.  The
-// warm-started refinement of the 65..256 blocks runs on side lane kRefLane;
-// the tridiagonal classes then take the blocks it did not accept (their
-// launches wait for it), every other class starts right after k_post; the
-// eigenvectors of the fallback blocks are formed once every fallback class
-// has finished, and stream_ joins all lanes.
+// The PSD projection of one outer iteration, after k_post on stream_.
+//
+// Without the refinement every size class starts right after k_post on its
+// lane (kClassLane) and stream_ joins the lanes.  With it, lane kRefLane runs
+// the warm-started refinement of the 65..256 classes: its first kernel sets
+// each block's route, after which the tridiagonal launches of the blocks
+// known to need the fallback start on lanes 2 and 3 (kFallbackLane) beside
+// the refinement; after the refinement the same lanes take the blocks it
+// rejected.  stream_ joins every lane that writes x and z; the eigenvectors
+// of the fallback blocks (needed only by the next iteration's refinement) are
+// formed on kRefLane after the fallback lanes, off the critical path.
 void Batch::launch_projection(const BatchDev& B) {
     constexpr int rl = kRefLane;
     V2H_HIP(hipEventRecord(ev_post_, stream_));
     bool used[kSideStreams + 1] = {};
     const bool refine = proj_.refine_on();
+    hipStream_t rs = pstream_[rl - 1];
     if (refine) {
-        hipStream_t ps = pstream_[rl - 1];
-        V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
+        V2H_HIP(hipStreamWaitEvent(rs, ev_post_, 0));
         used[rl] = true;
-        tstart(kKindRefine, ps);
-        proj_.refine_run(B, ps);
-        tstop(kKindRefine, ps);
-        launches_ += 15;
-        V2H_HIP(hipEventRecord(ev_ref_, ps));
+        tstart(kKindRefine, rs);
+        proj_.refine_begin(B, rs);
+        V2H_HIP(hipEventRecord(ev_state_, rs));
+        proj_.refine_run(B, rs);
+        tstop(kKindRefine, rs);
+        launches_ += Refiner::kLaunches;
+        V2H_HIP(hipEventRecord(ev_ref_, rs));
     }
+    auto lane_stream = [&](int ln) { return ln == 0 ? stream_ : pstream_[ln - 1]; };
     for (int k = kProjClasses - 1; k >= 0; k--) {
-        if (!proj_.has(k)) continue;
-        const int kind = 4 + k, ln = kClassLane[k];
-        hipStream_t ps = ln == 0 ? stream_ : pstream_[ln - 1];
-        if (refine && Projector::refined_class(k)) {
-            if (ln != rl) V2H_HIP(hipStreamWaitEvent(ps, ev_ref_, 0));
-        } else if (ln > 0 && !used[ln]) {
-            V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
-        }
+        if (!proj_.has(k) || (refine && Projector::refined_class(k))) continue;
+        const int ln = kClassLane[k];
+        hipStream_t ps = lane_stream(ln);
+        if (ln > 0 && !used[ln]) V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
         used[ln] = true;
-        tstart(kind, ps);
+        tstart(4 + k, ps);
         proj_.launch(k, B, ps);
-        tstop(kind, ps);
+        tstop(4 + k, ps);
         launches_++;
     }
     if (refine) {
-        hipStream_t ps = pstream_[rl - 1];
-        for (int ln = 1; ln <= kSideStreams; ln++)
-            if (ln != rl && used[ln]) {
-                V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
-                V2H_HIP(hipStreamWaitEvent(ps, ev_proj_[ln - 1], 0));
+        for (int pass = 0; pass < 2; pass++) {  // fallback known at the start, then the rejections
+            bool waited[kSideStreams + 1] = {};
+            for (int k = kProjClasses - 1; k >= 0; k--) {
+                if (!proj_.has(k) || !Projector::refined_class(k)) continue;
+                const int ln = kFallbackLane[k];
+                hipStream_t ps = lane_stream(ln);
+                if (!waited[ln]) V2H_HIP(hipStreamWaitEvent(ps, pass == 0 ? ev_state_ : ev_ref_, 0));
+                waited[ln] = used[ln] = true;
+                tstart(4 + k, ps);
+                proj_.launch(k, B, ps, pass == 0 ? kRouteFallback : kRouteRejected);
+                tstop(4 + k, ps);
+                launches_++;
             }
-        tstart(kKindSeed, ps);
-        proj_.refine_seed(B, ps);
-        tstop(kKindSeed, ps);
-        launches_++;
-        V2H_HIP(hipEventRecord(ev_proj_[rl - 1], ps));
-        V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[rl - 1], 0));
-        return;
+        }
     }
     for (int ln = 1; ln <= kSideStreams; ln++)
         if (used[ln]) {
             V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
             V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[ln - 1], 0));
         }
+    if (refine) {
+        for (int ln = 1; ln <= kSideStreams; ln++)
+            if (ln != rl && used[ln]) V2H_HIP(hipStreamWaitEvent(rs, ev_proj_[ln - 1], 0));
+        tstart(kKindSeed, rs);
+        proj_.refine_seed(B, rs);
+        tstop(kKindSeed, rs);
+        launches_++;
+        seed_pending_ = true;
+    }
+}
+
+// The eigenvector formation of the last iteration runs on kRefLane, which
+// stream_ does not wait for; join it before anything else touches the
+// projection state.
+void Batch::join_seed() {
+    if (!seed_pending_) return;
+    V2H_HIP(hipEventRecord(ev_seed_, pstream_[kRefLane - 1]));
+    V2H_HIP(hipStreamWaitEvent(stream_, ev_seed_, 0));
+    seed_pending_ = false;
 }
 
 void Batch::set_refine(bool on) {
@@ -862,6 +891,13 @@ void Batch::apply_A(int i, const double* u, double* out) {
     launch_fill(o, S.n_dual, 0.0, stream_);
     launch_spmv_plain(S.dev().A, unitsA_.p, hwgA_begin_[i], hwgA_begin_[i + 1] - hwgA_begin_[i], in, o, stream_);
     d2h(out, o, S.n_dual, stream_);
+}
+
+void Batch::apply_A_device(int i, const double* in_dev, double* out_dev) {
+    const Structure& S = *structs_[inst_sid_[i]];
+    launch_fill(out_dev, S.n_dual, 0.0, stream_);
+    launch_spmv_plain(S.dev().A, unitsA_.p, hwgA_begin_[i], hwgA_begin_[i + 1] - hwgA_begin_[i], in_dev, out_dev,
+                      stream_);
 }
 
 void Batch::apply_AT(int i, const double* y, double* out) {

@@ -162,6 +162,10 @@ class Batch {
 
     // Maps on host vectors (copy in, apply on the device, copy out).
     void apply_A(int i, const double* u, double* out);
+    // out_dev[n_dual] = A in_dev[n_primal] on the batch stream (device
+    // pointers; out_dev is cleared first); no allocation, no synchronisation.
+    void apply_A_device(int i, const double* in_dev, double* out_dev);
+    int device() const { return device_; }
     void apply_AT(int i, const double* y, double* out);
     // dE/dsrc for src = [h; g] (user order) from a primal vector (default x).
     void cost_adjoint(int i, const double* xbar, double* dh, double* dg);
@@ -195,6 +199,8 @@ class Batch {
     static constexpr int kSideStreams = 3;
     static constexpr int kClassLane[kProjClasses] = {0, 0, 2, 3, 1, 1, 2};
     static constexpr int kRefLane = 1;  // the warm-started refinement and the eigenvector formation
+    // lanes of the tridiagonal fallback of the refined classes (2..5)
+    static constexpr int kFallbackLane[kProjClasses] = {0, 0, 2, 3, 3, 2, 2};
     // Refined projection of the 65..256 blocks on (default) or off (every
     // block takes the tridiagonal eigensolver).
     void set_refine(bool on);
@@ -226,6 +232,7 @@ class Batch {
     void sync_state_to_device();
     void refresh_norms(int i);
     void launch_projection(const BatchDev& B);
+    void join_seed();
     DfState& df_state(int i);
     double run_df_intake(int i, bool norms = true);  // resident DF integrals -> c, v1, E_fzc (and ||c||)
     void launch_df_intake_only(int i);    // the same launches, no read-back
@@ -260,7 +267,9 @@ class Batch {
     Projector proj_;
     // side streams of the projection, joined back into stream_ by events
     hipStream_t pstream_[kSideStreams] = {};
-    hipEvent_t ev_post_ = nullptr, ev_proj_[kSideStreams] = {}, ev_ref_ = nullptr;
+    hipEvent_t ev_post_ = nullptr, ev_proj_[kSideStreams] = {}, ev_ref_ = nullptr, ev_state_ = nullptr,
+               ev_seed_ = nullptr;
+    bool seed_pending_ = false;
 
     struct KTimer {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;

@@ -88,6 +88,16 @@ class Structure:
         N.check(N.lib().v2h_structure_tpdm(self._h, np.ascontiguousarray(x, dtype=np.float64), out))
         return out.reshape(n, n, n, n)
 
+    def write_rdm_files(self, x, prefix, active, core=(), which=N.RDM_ALL_FULL):
+        """The RDM files of primal vector x (host code; see
+        Batch.write_rdm_files).  Returns the record count of each file."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.size != self.n_primal:
+            raise ValueError("x has the wrong length")
+        if len(active) != self.nmo:
+            raise ValueError("one full-space label per active orbital needed")
+        return N.write_rdm_files(N.lib().v2h_structure_write_rdm_files, (self._h, x), prefix, active, core, which)
+
     def hf_guess(self):
         """TPDM_GUESS HF primal vector (v2h_structure_hf_guess)."""
         x = np.zeros(self.n_primal)

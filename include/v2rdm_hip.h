@@ -251,7 +251,7 @@ int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
 
 /* HIP-event timing of the kernels launched by v2h_batch_solve, recorded on
- * the library's stream.  ms[14] and counts[14] per kernel kind: 0 iteration
+ * the library's stream.  ms[16] and counts[16] per kernel kind: 0 iteration
  * start (A [c - z, x, A^T y]), 1 CG A^T p, 2 CG A q, 3 A^T y and U, 4..10 PSD
  * projection classes: Jacobi for blocks up to 16, tridiagonal for blocks up to
  * 64 / 128 / 192 / 224 / 256, Jacobi beyond 256 (matrix in global memory);
@@ -262,9 +262,8 @@ int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
  * integral transform), 14 the warm-started refinement of the blocks of
  * 65..256 rows (all its launches of one outer iteration), 15 the eigenvector
  * formation of the blocks that took the tridiagonal fallback (with the
- * refinement on, kinds 6..9 time only the fallback launches).  Arrays of 16
- * entries.  Cumulative since the
- * last reset.  full_ms / full_counts (may be NULL) cover only the launches
+ * refinement on, kinds 6..9 time only the fallback launches).  Cumulative
+ * since the last reset.  full_ms / full_counts (may be NULL) cover only the launches
  * made while every instance was running (for CG kernels: still iterating)
  * at the previous host poll, i.e. launches that processed the whole batch. */
 int v2h_batch_set_profiling(v2h_batch *b, int on);
@@ -280,7 +279,7 @@ int v2h_batch_launch_stats(v2h_batch *b, int64_t *stats);
  *   V2H_PROJECT_REFINED (default): the eigenvectors of the previous outer
  *     iteration are refined by FP64 matrix-core products (iterative
  *     refinement of Ogita and Aishima, at most three steps, accepted when the
- *     last correction is below 3e-8); blocks without eigenvectors yet, or
+ *     last correction is below 2e-7); blocks without eigenvectors yet, or
  *     whose refinement is not accepted, take the tridiagonal eigensolver,
  *     whose eigenvectors seed the next iteration;
  *   V2H_PROJECT_TRIDIAGONAL: every block takes the tridiagonal eigensolver.
@@ -295,13 +294,46 @@ int v2h_batch_set_projection(v2h_batch *b, int mode);
 int v2h_batch_projection_stats(v2h_batch *b, int64_t *stats);
 /* Thresholds of the refinement: eigenvalue estimates closer than cluster_rel
  * x max|l| form a cluster (default 1e-8); a step is accepted when its largest
- * correction is at most accept_e (3e-8) and the couplings left inside
+ * correction is at most accept_e (2e-7) and the couplings left inside
  * clusters at most accept_cpl x max|l| (1e-13). */
 int v2h_batch_set_refine_params(v2h_batch *b, double cluster_rel, double accept_e, double accept_cpl);
 
 /* Adjoint of the cost packing: dE/dh[nmo*nmo], dE/dg[nmo^4] (caller order)
  * for E = c . xbar; xbar = NULL uses the current primal solution x. */
 int v2h_batch_cost_adjoint(v2h_batch *b, int inst, const double *xbar, double *dh, double *dg);
+
+/* hilbert's RDM files for downstream codes, from instance inst's current x:
+ * the PSIO records of WriteActiveTPDM / WriteTPDM / WriteTPDMSpinFree /
+ * WriteOPDM (write_tpdm.cc:521-595, 40-300, 312-519, write_opdm.cc:40-119)
+ * and the IWL TPDM of WriteTPDM_IWL (write_tpdm_iwl.cc:45-200), chosen by
+ * `which` (V2H_RDM_* flags, the options that make hilbert call each writer,
+ * v2rdm_solver.cc:1474-1509) and written in hilbert's order.  Orbital labels
+ * are full-space indices: active[u] for active orbital u (caller order),
+ * core[] the doubly occupied core orbitals in hilbert's order (by irrep, then
+ * index).  Each file is a sequence of PSIO-style keyed entries {char key[80],
+ * int64 nbytes, bytes}:
+ *   <prefix>.d2aa / .d2bb / .d2ab: "D2aa" / "D2bb" / "D2ab" records {int i, j,
+ *     k, l; double value} = D2(ij, kl), "length" (int64); TPDM_FULL adds the
+ *     core contributions and "NUMBER ACTIVE ORBITALS", "ACTIVE ORBITALS",
+ *     "NUMBER INACTIVE ORBITALS", "INACTIVE ORBITALS" (with both flags the
+ *     full records replace the active ones, as in hilbert);
+ *   <prefix>.d2: "D2" spin-free records, "length" and the orbital lists
+ *     (hilbert's same-spin sign: -1 as soon as i > j or k > l);
+ *   <prefix>.d1a / .d1b: "D1a" / "D1b" records {int i, j; double value},
+ *     "length";
+ *   <prefix>.aa.iwl / .bb.iwl / .ab.iwl: "IWL Buffers", psi4 libiwl buffers
+ *     {int lastbuf, int inbuf, int16 labels[4 x 2980], double values[2980]},
+ *     values above 1e-14 in magnitude, labels (p, q, r, s) in chemists'
+ *     order, the aa / bb values x 1/4 as hilbert writes them.
+ * counts[9]: D1a, D1b, D2aa, D2bb, D2ab, spin-free D2 records and IWL aa, bb,
+ * ab values; -1 for a file not written. */
+#define V2H_RDM_TPDM_ACTIVE 1     /* TPDM_WRITE */
+#define V2H_RDM_TPDM_FULL 2       /* TPDM_WRITE_FULL */
+#define V2H_RDM_TPDM_SPIN_FREE 4  /* TPDM_WRITE_SPIN_FREE */
+#define V2H_RDM_OPDM_FULL 8       /* OPDM_WRITE_FULL */
+#define V2H_RDM_TPDM_IWL 16       /* DERTYPE FIRST */
+int v2h_batch_write_rdm_files(v2h_batch *b, int inst, int ncore, const int *core, const int *active,
+                              const char *prefix, int which, int64_t *counts);
 
 /* ---- BPSDP with caller-supplied maps ----
  * The generic libsdp contract: BPSDPSolver(n_primal, n_dual, options)
@@ -360,6 +392,9 @@ int v2h_structure_opdm(const v2h_structure *s, const double *x, const char *type
 int v2h_structure_tpdm(const v2h_structure *s, const double *x, double *out);
 /* The TPDM_GUESS HF primal vector (as v2h_batch_set_guess_hf). */
 int v2h_structure_hf_guess(const v2h_structure *s, double *x);
+/* v2h_batch_write_rdm_files for a primal vector x of this structure (host only). */
+int v2h_structure_write_rdm_files(const v2h_structure *s, const double *x, int ncore, const int *core,
+                                  const int *active, const char *prefix, int which, int64_t *counts);
 
 #ifdef __cplusplus
 }
