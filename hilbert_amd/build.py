@@ -18,7 +18,7 @@ SOURCES = ["layout.cpp", "constraints.cpp", "sell.cpp", "rdm.cpp", "projector.cp
            "fcidump.cpp", "writers.cpp", "orbopt_driver.cpp", "abi.cpp",
            "kernels.hip",
            "eig_tri.hip", "refine.hip",
-           "df.hip", "orbopt.hip"]
+           "df.hip", "orbopt.hip", "noalloc.hip"]
 ARCH = os.environ.get("V2H_OFFLOAD_ARCH", "gfx950")
 
 

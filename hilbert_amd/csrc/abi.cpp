@@ -12,6 +12,7 @@
 #include "../../include/v2rdm_hip.h"
 #include "dbuf.hpp"
 #include "kernels.hpp"
+#include "noalloc.hpp"
 #include "rdm.hpp"
 #include "sdp_host.hpp"
 #include "solver.hpp"
@@ -502,6 +503,27 @@ int v2h_batch_write_rdm_files(v2h_batch* b, int inst, int ncore, const int* core
         std::vector<double> x(B(b).n_primal(inst));
         B(b).get_primal(inst, x.data());
         v2h::write_rdm_files(L, x.data(), full_space(L, ncore, core, active), prefix, which, counts);
+    });
+}
+
+int v2h_batch_Au_noalloc(v2h_batch* b, int inst, const double* u, double* out) {
+    return guarded([&] {
+        check_inst(B(b), inst);
+        check_ptr(u);
+        check_ptr(out);
+        v2h::Batch& bt = B(b);
+        const int64_t np = bt.n_primal(inst), nd = bt.n_dual(inst);
+        if (np + nd > v2h::kNoAllocDoubles)
+            throw std::invalid_argument("v2h_batch_Au_noalloc: n_primal + n_dual = " + std::to_string(np + nd) +
+                                        " exceeds the scratch of " + std::to_string(v2h::kNoAllocDoubles) +
+                                        " doubles");
+        V2H_HIP(hipSetDevice(bt.device()));
+        double* s = v2h::noalloc_scratch(bt.device());  // the scratch's device address, never its name
+        hipStream_t st = bt.stream();
+        V2H_HIP(hipMemcpyAsync(s, u, sizeof(double) * (size_t)np, hipMemcpyHostToDevice, st));
+        bt.apply_A_device(inst, s, s + np);
+        V2H_HIP(hipMemcpyAsync(out, s + np, sizeof(double) * (size_t)nd, hipMemcpyDeviceToHost, st));
+        V2H_HIP(hipStreamSynchronize(st));
     });
 }
 

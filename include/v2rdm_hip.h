@@ -249,6 +249,17 @@ int v2h_batch_get_tpdm_sparse(v2h_batch *b, int inst, const char *type, int32_t 
 /* The constraint maps (bpsdp_Au / bpsdp_ATu, v2rdm_solver.cc:2995-3108). */
 int v2h_batch_Au(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_ATu(v2h_batch *b, int inst, const double *y, double *out);
+/* v2h_batch_Au without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * input and output are staged in a 4 MiB scratch array that the library
+ * keeps on each GPU until the process ends (its address is resolved on the
+ * first call on a device): n_primal + n_dual of the instance must be at most
+ * 524288 doubles, else V2H_ERR_INVALID_ARGUMENT.  Every call on one GPU shares
+ * that scratch: calls that overlap on one GPU (from several threads or
+ * handles) race and must be serialised by the caller.  Same result as
+ * v2h_batch_Au, bit for bit. */
+int v2h_batch_Au_noalloc(v2h_batch *b, int inst, const double *u, double *out);
 
 /* HIP-event timing of the kernels launched by v2h_batch_solve, recorded on
  * the library's stream.  ms[16] and counts[16] per kernel kind: 0 iteration
