@@ -34,6 +34,27 @@ class ProblemDesc(C.Structure):
                 ("constrain_t2", C.c_int), ("constrain_d3", C.c_int)]
 
 
+class Tuning(C.Structure):
+    """v2h_tuning: launch configuration that changes no result bit."""
+    _fields_ = [("class_lane", C.c_int * 7), ("fallback_lane", C.c_int * 7), ("refine_lane", C.c_int),
+                ("cg_margin", C.c_int), ("xcd_remap", C.c_int), ("graphs", C.c_int)]
+
+    def as_dict(self):
+        return {"class_lane": list(self.class_lane), "fallback_lane": list(self.fallback_lane),
+                "refine_lane": self.refine_lane, "cg_margin": self.cg_margin, "xcd_remap": self.xcd_remap,
+                "graphs": self.graphs}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k in range(7):
+            t.class_lane[k] = int(d["class_lane"][k])
+            t.fallback_lane[k] = int(d["fallback_lane"][k])
+        t.refine_lane, t.cg_margin, t.xcd_remap = int(d["refine_lane"]), int(d["cg_margin"]), int(d["xcd_remap"])
+        t.graphs = int(d.get("graphs", 0))
+        return t
+
+
 class OrbOptOptions(C.Structure):
     _fields_ = [("g_convergence", C.c_double), ("e_convergence", C.c_double), ("maxiter", C.c_int),
                 ("active_active", C.c_int)]
@@ -112,6 +133,9 @@ SIGNATURES = {
     "v2h_batch_ATu": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_cost_adjoint": (C.c_int, [_vp, C.c_int, _vp, _dp, _dp]),
     "v2h_batch_set_profiling": (C.c_int, [_vp, C.c_int]),
+    "v2h_batch_set_graphs": (C.c_int, [_vp, C.c_int]),
+    "v2h_batch_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+    "v2h_batch_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "v2h_batch_kernel_times": (C.c_int, [_vp, _dp, _lp, _dp, _lp]),
     "v2h_batch_reset_kernel_times": (C.c_int, [_vp]),
     "v2h_batch_launch_stats": (C.c_int, [_vp, _lp]),

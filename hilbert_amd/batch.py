@@ -296,6 +296,10 @@ class Batch:
         back to the tridiagonal eigensolver; PROJECT_TRIDIAGONAL: always the
         tridiagonal eigensolver."""
         N.check(N.lib().v2h_batch_set_projection(self._h, int(mode)))
+        self._projection = int(mode)
+
+    def projection_mode(self) -> int:
+        return getattr(self, "_projection", self.PROJECT_REFINED)
 
     def projection_stats(self) -> dict:
         st = np.zeros(5, dtype=np.int64)
@@ -309,6 +313,25 @@ class Batch:
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))
+
+    def tuning(self) -> dict:
+        """The launch configuration (v2h_tuning) as a dict."""
+        t = N.Tuning()
+        N.check(N.lib().v2h_batch_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, cfg: dict):
+        """Set the launch configuration (changes no result bit)."""
+        t = N.Tuning.from_dict(cfg)
+        N.check(N.lib().v2h_batch_set_tuning(self._h, C.byref(t)))
+
+    GRAPHS_OFF, GRAPHS_CG, GRAPHS_ALL = 0, 1, 2
+
+    def set_graphs(self, mode=GRAPHS_ALL):
+        """hipGraph replay: GRAPHS_CG the first CG chunk of every outer
+        iteration, GRAPHS_ALL also k_post + the projection; GRAPHS_OFF
+        (default) the same launches one by one.  Same bits in every mode."""
+        N.check(N.lib().v2h_batch_set_graphs(self._h, int(mode)))
 
     def kernel_times(self, full=False) -> dict:
         """{kind: (total ms, launches)}; full=True: only launches that

@@ -540,6 +540,46 @@ int v2h_batch_set_profiling(v2h_batch* b, int on) {
     return guarded([&] { B(b).set_profiling(on != 0); });
 }
 
+static_assert(sizeof(v2h_tuning) == sizeof(int) * (2 * 7 + 4), "v2h_tuning layout");
+
+int v2h_batch_get_tuning(const v2h_batch* b, v2h_tuning* t) {
+    return guarded([&] {
+        check_ptr(t);
+        const auto& u = B(const_cast<v2h_batch*>(b)).tuning();
+        for (int k = 0; k < 7; k++) {
+            t->class_lane[k] = u.class_lane[k];
+            t->fallback_lane[k] = u.fallback_lane[k];
+        }
+        t->refine_lane = u.refine_lane;
+        t->cg_margin = u.cg_margin;
+        t->xcd_remap = u.xcd_remap;
+        t->graphs = u.graphs;
+    });
+}
+
+int v2h_batch_set_tuning(v2h_batch* b, const v2h_tuning* t) {
+    return guarded([&] {
+        check_ptr(t);
+        v2h::Batch::Tuning u;
+        for (int k = 0; k < 7; k++) {
+            u.class_lane[k] = t->class_lane[k];
+            u.fallback_lane[k] = t->fallback_lane[k];
+        }
+        u.refine_lane = t->refine_lane;
+        u.cg_margin = t->cg_margin;
+        u.xcd_remap = t->xcd_remap;
+        u.graphs = t->graphs;
+        B(b).set_tuning(u);
+    });
+}
+
+int v2h_batch_set_graphs(v2h_batch* b, int mode) {
+    return guarded([&] {
+        V2H_HIP(hipSetDevice(B(b).device()));
+        B(b).set_graphs(mode);
+    });
+}
+
 int v2h_batch_kernel_times(v2h_batch* b, double* ms, int64_t* counts, double* full_ms, int64_t* full_counts) {
     return guarded([&] { B(b).kernel_times(ms, counts, full_ms, full_counts); });
 }

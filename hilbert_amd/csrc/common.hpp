@@ -133,6 +133,7 @@ struct UnitsDev {
     const int* plan;
     int plan_m;
     int grid;               // workgroups launched
+    int remap;              // 1: blocks dealt to the XCDs in contiguous ranges of logical workgroups
 };
 
 struct ProjItem {

@@ -48,7 +48,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int n) {
 // Logical workgroup of this block under the launch plan (runs of the
 // workgroups of selected instances; identity without a plan).
 __device__ __forceinline__ int unit_wg(const UnitsDev& U) {
-    const int c = xcd_remap(blockIdx.x, U.grid);
+    const int c = U.remap ? xcd_remap(blockIdx.x, U.grid) : (int)blockIdx.x;
     if (U.plan == nullptr) return c;
     int lo = 0, hi = U.plan_m;  // plan[lo] <= c < plan[hi]
     while (hi - lo > 1) {

@@ -107,6 +107,9 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     V2H_HIP(hipEventCreateWithFlags(&ev_ref_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_state_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_seed_, hipEventDisableTiming));
+    V2H_HIP(hipEventCreateWithFlags(&ev_gfork_, hipEventDisableTiming));
+    V2H_HIP(hipEventCreateWithFlags(&ev_joined_, hipEventDisableTiming));
+    V2H_HIP(hipEventCreateWithFlags(&ev_gjoin_, hipEventDisableTiming));
 
     std::map<std::string, int> sid_of;
     for (const auto& d : descs) {
@@ -186,6 +189,8 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
 
 Batch::~Batch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph(g_post_);
+    for (auto& e : g_cg_) drop_graph(e);
     for (auto& t : timers_)
         for (auto& e : t.pool) {
             (void)hipEventDestroy(e.first);
@@ -201,6 +206,9 @@ Batch::~Batch() {
     if (ev_ref_) (void)hipEventDestroy(ev_ref_);
     if (ev_state_) (void)hipEventDestroy(ev_state_);
     if (ev_seed_) (void)hipEventDestroy(ev_seed_);
+    if (ev_gfork_) (void)hipEventDestroy(ev_gfork_);
+    if (ev_joined_) (void)hipEventDestroy(ev_joined_);
+    if (ev_gjoin_) (void)hipEventDestroy(ev_gjoin_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -287,13 +295,13 @@ BatchDev Batch::dev() const {
 UnitsDev Batch::units_a() const {
     const int N = count();
     return UnitsDev{recA_.p, wgA_inst_.p, wgA_begin_.p, nwgA_, N, plan_all_ ? nullptr : plan_dev_.p, planA_m_,
-                    gridA_};
+                    gridA_, tuning_.xcd_remap};
 }
 
 UnitsDev Batch::units_at() const {
     const int N = count();
     return UnitsDev{recAT_.p, wgAT_inst_.p, wgAT_begin_.p, nwgAT_, N,
-                    plan_all_ ? nullptr : plan_dev_.p + 2 * N + 1, planAT_m_, gridAT_};
+                    plan_all_ ? nullptr : plan_dev_.p + 2 * N + 1, planAT_m_, gridAT_, tuning_.xcd_remap};
 }
 
 // Launch only the workgroups of the selected instances.  The selection must
@@ -347,41 +355,157 @@ void Batch::sync_state_to_host() {
 void Batch::tstart(int k, hipStream_t s) {
     if (!profile_) return;
     if (!s) s = stream_;
+    if (capture_) {  // an event-record node of the graph being captured
+        hipEvent_t a, b;
+        V2H_HIP(hipEventCreate(&a));
+        V2H_HIP(hipEventCreate(&b));
+        capture_->timers.push_back({k, a, b});
+        V2H_HIP(hipEventRecordWithFlags(a, s, hipEventRecordExternal));
+        return;
+    }
     KTimer& t = timers_[k];
     if (t.used == t.pool.size()) {
         hipEvent_t a, b;
         V2H_HIP(hipEventCreate(&a));
         V2H_HIP(hipEventCreate(&b));
         t.pool.push_back({a, b});
-        t.full.push_back(0);
     }
-    t.full[t.used] = (k == 1 || k == 2) ? all_iterating_ : all_running_;
     V2H_HIP(hipEventRecord(t.pool[t.used].first, s));
 }
 
 void Batch::tstop(int k, hipStream_t s) {
     if (!profile_) return;
     if (!s) s = stream_;
+    if (capture_) {
+        if (capture_->timers.empty() || capture_->timers.back().kind != k)
+            throw std::logic_error("kernel timer: tstop without its tstart");
+        V2H_HIP(hipEventRecordWithFlags(capture_->timers.back().b, s, hipEventRecordExternal));
+        return;
+    }
     KTimer& t = timers_[k];
     V2H_HIP(hipEventRecord(t.pool[t.used].second, s));
+    t.pending.push_back({t.pool[t.used].first, t.pool[t.used].second, timer_full(k)});
     t.used++;
 }
 
-// Called after a stream synchronisation: all recorded events are complete.
+// Called after a synchronisation of stream_: the events recorded on stream_
+// and on the lanes it joined are complete; the eigenvector formation's lane
+// is not joined, so it is waited for here.
 void Batch::tflush() {
+    if (seed_pending_ && !timers_[kKindSeed].pending.empty())
+        V2H_HIP(hipStreamSynchronize(pstream_[tuning_.refine_lane - 1]));
     for (auto& t : timers_) {
-        for (size_t i = 0; i < t.used; i++) {
+        for (const auto& r : t.pending) {
             float ms = 0.f;
-            V2H_HIP(hipEventElapsedTime(&ms, t.pool[i].first, t.pool[i].second));
+            V2H_HIP(hipEventElapsedTime(&ms, r.a, r.b));
             t.ms += ms;
             t.count++;
-            if (t.full[i]) {
+            if (r.full) {
                 t.full_ms += ms;
                 t.full_count++;
             }
         }
+        t.pending.clear();
         t.used = 0;
     }
+    g_post_.unread = false;
+    for (auto& e : g_cg_) e.unread = false;
+}
+
+// Replay the launches of `body` on stream_ as one graph: captured on first use
+// and whenever `key` changes, then launched with hipGraphLaunch.  `body` may
+// fork onto the side streams by events but must join them back into stream_,
+// and may not synchronise, copy or allocate.
+template <class Body>
+void Batch::run_graph(GraphEntry& e, const std::string& key, Body&& body) {
+    if (!e.exec || e.key != key) {
+        drop_graph(e);
+        const int64_t l0 = launches_;
+        capture_ = &e;
+        V2H_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(stream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            capture_ = nullptr;
+            drop_graph(e);
+            throw;
+        }
+        capture_ = nullptr;
+        V2H_HIP(hipStreamEndCapture(stream_, &g));
+        const hipError_t rc = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (rc != hipSuccess) {
+            e.exec = nullptr;
+            drop_graph(e);
+            V2H_HIP(rc);
+        }
+        e.key = key;
+        e.launches = launches_ - l0;
+        launches_ = l0;
+    }
+    if (e.unread) {  // its timer events still hold the previous replay's times
+        V2H_HIP(hipStreamSynchronize(stream_));
+        tflush();
+    }
+    V2H_HIP(hipGraphLaunch(e.exec, stream_));
+    launches_ += e.launches;
+    for (const auto& t : e.timers) timers_[t.kind].pending.push_back({t.a, t.b, timer_full(t.kind)});
+    e.unread = !e.timers.empty();
+}
+
+void Batch::drop_graph(GraphEntry& e) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    e.exec = nullptr;
+    if (!e.timers.empty()) {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        for (auto& t : timers_)  // pending reads of these events go first
+            if (!t.pending.empty()) {
+                tflush();
+                break;
+            }
+        for (const auto& t : e.timers) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+    }
+    e.timers.clear();
+    e.key.clear();
+    e.launches = 0;
+    e.unread = false;
+}
+
+void Batch::drop_graphs() {
+    drop_graph(g_post_);
+    for (auto& e : g_cg_) drop_graph(e);
+    g_cg_.clear();
+}
+
+void Batch::set_tuning(const Tuning& t) {
+    for (int k = 0; k < kProjClasses; k++)
+        if (t.class_lane[k] < 0 || t.class_lane[k] > kSideStreams || t.fallback_lane[k] < 0 ||
+            t.fallback_lane[k] > kSideStreams)
+            throw std::invalid_argument("tuning: lanes are 0.." + std::to_string(kSideStreams));
+    if (t.refine_lane < 1 || t.refine_lane > kSideStreams)
+        throw std::invalid_argument("tuning: the refinement lane is a side stream (1.." +
+                                    std::to_string(kSideStreams) + ")");
+    if (t.cg_margin < 0 || t.cg_margin > 64) throw std::invalid_argument("tuning: cg_margin is 0..64");
+    if (t.xcd_remap != 0 && t.xcd_remap != 1) throw std::invalid_argument("tuning: xcd_remap is 0 or 1");
+    if (t.graphs < kGraphOff || t.graphs > kGraphAll)
+        throw std::invalid_argument("tuning: graphs is 0 (off), 1 (CG chunk) or 2 (CG chunk and projection)");
+    V2H_HIP(hipSetDevice(device_));
+    join_seed();  // the eigenvector formation in flight ran on the old refinement lane
+    V2H_HIP(hipStreamSynchronize(stream_));
+    tuning_ = t;
+    drop_graphs();  // lanes and the remap flag are baked into the captures
+}
+
+void Batch::set_graphs(int mode) {
+    Tuning t = tuning_;
+    t.graphs = mode;
+    set_tuning(t);
 }
 
 void Batch::kernel_times(double* ms, int64_t* counts, double* full_ms, int64_t* full_counts) {
@@ -631,7 +755,7 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     for (int i = 0; i < count(); i++) seen[i] = state_[i].oiter;
 
     int cg_est = 4;
-    for (const auto& st : state_) cg_est = std::max(cg_est, st.cg_last + 2);
+    for (const auto& st : state_) cg_est = std::max(cg_est, st.cg_last + tuning_.cg_margin);
     auto refresh_flags = [&] {
         all_running_ = all_iterating_ = true;
         for (const auto& st : state_) {
@@ -659,15 +783,38 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         bool any_active = true;
         for (;;) {
             const UnitsDev UA = units_a(), UAT = units_at();
-            for (int k = 0; k < chunk; k++) {
-                tstart(1);
-                launch_cg_at(B, UAT, stream_);
-                tstop(1);
-                tstart(2);
-                launch_cg_a(B, UA, stream_);
-                tstop(2);
+            // the eigenvector formation owed by a projection graph runs as a
+            // branch beside the first chunk, joined at its end
+            const bool with_seed = tuning_.graphs >= kGraphCg && steps == 0 && seed_owed_;
+            auto cg_chunk = [&, chunk, with_seed] {
+                hipStream_t rs = pstream_[tuning_.refine_lane - 1];
+                if (with_seed) {
+                    V2H_HIP(hipEventRecord(ev_gfork_, stream_));
+                    V2H_HIP(hipStreamWaitEvent(rs, ev_gfork_, 0));
+                    tstart(kKindSeed, rs);
+                    proj_.refine_seed(B, rs);
+                    tstop(kKindSeed, rs);
+                    launches_++;
+                    V2H_HIP(hipEventRecord(ev_gjoin_, rs));
+                }
+                for (int k = 0; k < chunk; k++) {
+                    tstart(1);
+                    launch_cg_at(B, UAT, stream_);
+                    tstop(1);
+                    tstart(2);
+                    launch_cg_a(B, UA, stream_);
+                    tstop(2);
+                }
+                launches_ += 2 * chunk;
+                if (with_seed) V2H_HIP(hipStreamWaitEvent(stream_, ev_gjoin_, 0));
+            };
+            if (tuning_.graphs >= kGraphCg && steps == 0) {  // the first chunk covers every running instance
+                run_graph(cg_graph(chunk, with_seed), graph_key("cg" + std::to_string(chunk) + (with_seed ? "s" : "")),
+                          cg_chunk);
+                seed_owed_ = false;
+            } else {
+                cg_chunk();
             }
-            launches_ += 2 * chunk;
             steps += chunk;
             sync_state_to_host();
             refresh_flags();
@@ -684,16 +831,26 @@ int Batch::solve(int maxiter, const Monitor& mon) {
         if (!any_active) break;
         cg_est = 4;
         for (const auto& st : state_)
-            if (st.active) cg_est = std::max(cg_est, st.cg_last + 2);
+            if (st.active) cg_est = std::max(cg_est, st.cg_last + tuning_.cg_margin);
         all_iterating_ = all_running_;  // the next CG solve starts for every running instance
         select(false);
-        tstart(3);
-        launch_post(B, units_at(), stream_);
-        tstop(3);
-        launches_++;
-        // the size classes are independent: largest first, on the streams of
-        // kClassLane, joined back into stream_
-        launch_projection(B);
+        auto post_projection = [&](bool seed) {
+            tstart(3);
+            launch_post(B, units_at(), stream_);
+            tstop(3);
+            launches_++;
+            // the size classes are independent: largest first, on the streams
+            // of tuning_.class_lane, joined back into stream_
+            launch_projection(B, seed);
+        };
+        if (tuning_.graphs == kGraphAll) {
+            // the graph joins every lane it forks, so the eigenvector formation
+            // (off the critical path) goes into the next CG graph
+            run_graph(g_post_, graph_key(proj_.refine_on() ? "post_r" : "post"), [&] { post_projection(false); });
+            seed_owed_ = proj_.refine_on();
+        } else {
+            post_projection(true);
+        }
         V2H_HIP(hipGetLastError());
     }
     select(false);
@@ -701,6 +858,10 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     launch_pre(B, units_a(), stream_);  // book-keeping of the last iteration
     tstop(0);
     launches_++;
+    if (seed_owed_) {  // the last projection graph's eigenvector formation
+        launch_seed(B);
+        seed_owed_ = false;
+    }
     join_seed();
     V2H_HIP(hipGetLastError());
     sync_state_to_host();
@@ -716,16 +877,17 @@ int Batch::solve(int maxiter, const Monitor& mon) {
 // The PSD projection of one outer iteration, after k_post on stream_.
 //
 // Without the refinement every size class starts right after k_post on its
-// lane (kClassLane) and stream_ joins the lanes.  With it, lane kRefLane runs
-// the warm-started refinement of the 65..256 classes: its first kernel sets
-// each block's route, after which the tridiagonal launches of the blocks
-// known to need the fallback start on lanes 2 and 3 (kFallbackLane) beside
-// the refinement; after the refinement the same lanes take the blocks it
-// rejected.  stream_ joins every lane that writes x and z; the eigenvectors
-// of the fallback blocks (needed only by the next iteration's refinement) are
-// formed on kRefLane after the fallback lanes, off the critical path.
-void Batch::launch_projection(const BatchDev& B) {
-    constexpr int rl = kRefLane;
+// lane (tuning_.class_lane) and stream_ joins the lanes.  With it, the lane
+// tuning_.refine_lane runs the warm-started refinement of the 65..256
+// classes: its first kernel sets each block's route, after which the
+// tridiagonal launches of the blocks known to need the fallback start on
+// their lanes (tuning_.fallback_lane) beside the refinement; after the
+// refinement the same lanes take the blocks it rejected.  stream_ joins every
+// lane that writes x and z; the eigenvectors of the fallback blocks (needed
+// only by the next iteration's refinement) are formed on the refinement lane
+// after the fallback lanes, off the critical path.
+void Batch::launch_projection(const BatchDev& B, bool seed) {
+    const int rl = tuning_.refine_lane;
     V2H_HIP(hipEventRecord(ev_post_, stream_));
     bool used[kSideStreams + 1] = {};
     const bool refine = proj_.refine_on();
@@ -744,7 +906,7 @@ void Batch::launch_projection(const BatchDev& B) {
     auto lane_stream = [&](int ln) { return ln == 0 ? stream_ : pstream_[ln - 1]; };
     for (int k = kProjClasses - 1; k >= 0; k--) {
         if (!proj_.has(k) || (refine && Projector::refined_class(k))) continue;
-        const int ln = kClassLane[k];
+        const int ln = tuning_.class_lane[k];
         hipStream_t ps = lane_stream(ln);
         if (ln > 0 && !used[ln]) V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
         used[ln] = true;
@@ -758,7 +920,7 @@ void Batch::launch_projection(const BatchDev& B) {
             bool waited[kSideStreams + 1] = {};
             for (int k = kProjClasses - 1; k >= 0; k--) {
                 if (!proj_.has(k) || !Projector::refined_class(k)) continue;
-                const int ln = kFallbackLane[k];
+                const int ln = tuning_.fallback_lane[k];
                 hipStream_t ps = lane_stream(ln);
                 if (!waited[ln]) V2H_HIP(hipStreamWaitEvent(ps, pass == 0 ? ev_state_ : ev_ref_, 0));
                 waited[ln] = used[ln] = true;
@@ -774,9 +936,10 @@ void Batch::launch_projection(const BatchDev& B) {
             V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
             V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[ln - 1], 0));
         }
-    if (refine) {
-        for (int ln = 1; ln <= kSideStreams; ln++)
-            if (ln != rl && used[ln]) V2H_HIP(hipStreamWaitEvent(rs, ev_proj_[ln - 1], 0));
+    if (refine && seed) {
+        // after every fallback launch, whichever lane (stream_ included) ran it
+        V2H_HIP(hipEventRecord(ev_joined_, stream_));
+        V2H_HIP(hipStreamWaitEvent(rs, ev_joined_, 0));
         tstart(kKindSeed, rs);
         proj_.refine_seed(B, rs);
         tstop(kKindSeed, rs);
@@ -785,12 +948,47 @@ void Batch::launch_projection(const BatchDev& B) {
     }
 }
 
-// The eigenvector formation of the last iteration runs on kRefLane, which
-// stream_ does not wait for; join it before anything else touches the
-// projection state.
+// The eigenvector formation after a replayed projection graph, which has
+// joined every lane into stream_.
+void Batch::launch_seed(const BatchDev& B) {
+    if (!proj_.refine_on()) return;
+    hipStream_t rs = pstream_[tuning_.refine_lane - 1];
+    V2H_HIP(hipEventRecord(ev_post_, stream_));
+    V2H_HIP(hipStreamWaitEvent(rs, ev_post_, 0));
+    tstart(kKindSeed, rs);
+    proj_.refine_seed(B, rs);
+    tstop(kKindSeed, rs);
+    launches_++;
+    seed_pending_ = true;
+}
+
+Batch::GraphEntry& Batch::cg_graph(int steps, bool seed) {
+    const std::string key = graph_key("cg" + std::to_string(steps) + (seed ? "s" : ""));
+    for (auto& e : g_cg_)
+        if (e.key == key) return e;
+    if (g_cg_.size() >= kCgGraphs) {  // the oldest step count goes
+        drop_graph(g_cg_.front());
+        g_cg_.erase(g_cg_.begin());
+    }
+    g_cg_.emplace_back();
+    return g_cg_.back();
+}
+
+// Graph key: what a capture bakes into its launches (the plan's instance
+// selection, hence grid sizes and plan lengths), the sequence's own tag, and
+// whether the timers' event nodes are in it.
+std::string Batch::graph_key(const std::string& tag) const {
+    std::string k(plan_sel_.begin(), plan_sel_.end());
+    for (auto& ch : k) ch = ch ? '1' : '0';
+    return k + "|" + tag + (profile_ ? "|t" : "");
+}
+
+// The eigenvector formation of the last iteration runs on the refinement
+// lane, which stream_ does not wait for; join it before anything else touches
+// the projection state.
 void Batch::join_seed() {
     if (!seed_pending_) return;
-    V2H_HIP(hipEventRecord(ev_seed_, pstream_[kRefLane - 1]));
+    V2H_HIP(hipEventRecord(ev_seed_, pstream_[tuning_.refine_lane - 1]));
     V2H_HIP(hipStreamWaitEvent(stream_, ev_seed_, 0));
     seed_pending_ = false;
 }
@@ -800,6 +998,7 @@ void Batch::set_refine(bool on) {
     if (on && !proj_.refine_on())
         for (int i = 0; i < count(); i++) proj_.forget(i, stream_);  // eigenvectors from before are stale
     proj_.set_refine(on);
+    drop_graphs();
     V2H_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -868,6 +1067,7 @@ void Batch::set_slack(int i, const double* z) {
 void Batch::set_options(const SolverOptions& o) {
     const bool tighter = o.e_convergence < opt_.e_convergence || o.r_convergence < opt_.r_convergence;
     opt_ = o;
+    drop_graphs();  // the options are kernel arguments
     if (!tighter || !constructed_) return;
     sync_state_to_host();
     for (auto& st : state_) st.converged = 0;

@@ -193,14 +193,27 @@ class Batch {
     // The projection classes run concurrently on stream_ (lane 0) and three
     // side streams: with the runtime's default of four hardware queues per
     // process, every stream of a batch keeps a queue of its own, so classes
-    // on different lanes do not serialise behind each other.  Lanes by class
-    // (Jacobi <= 16, tri 64 / 128 / 192 / 224 / 256, Jacobi > 256), chosen
-    // to balance the per-iteration class times of the (16e,16o) workloads.
+    // on different lanes do not serialise behind each other.
     static constexpr int kSideStreams = 3;
-    static constexpr int kClassLane[kProjClasses] = {0, 0, 2, 3, 1, 1, 2};
-    static constexpr int kRefLane = 1;  // the warm-started refinement and the eigenvector formation
-    // lanes of the tridiagonal fallback of the refined classes (2..5)
-    static constexpr int kFallbackLane[kProjClasses] = {0, 0, 2, 3, 3, 2, 2};
+    // Launch configuration, none of which changes a result bit (the same
+    // kernels on the same data, ordered by the same events; the extra CG
+    // steps of a larger first chunk are skipped by the instances that have
+    // stopped): what the autotuner (hilbert_amd/autotune.py) chooses.
+    struct Tuning {
+        // lane (0 = stream_, 1..3 side streams) of each size class (Jacobi
+        // <= 16, tri 64 / 128 / 192 / 224 / 256, Jacobi > 256) when it is not
+        // refined, chosen to balance the per-iteration class times of the
+        // (16e,16o) workloads
+        int class_lane[kProjClasses] = {0, 0, 2, 3, 1, 1, 2};
+        // lanes of the tridiagonal fallback of the refined classes (2..5)
+        int fallback_lane[kProjClasses] = {0, 0, 2, 3, 3, 2, 2};
+        int refine_lane = 1;  // the warm-started refinement and the eigenvector formation (1..3)
+        int cg_margin = 2;    // first CG chunk: the largest previous CG step count + cg_margin
+        int xcd_remap = 1;    // gather workgroups dealt to the XCDs in contiguous ranges
+        int graphs = 0;       // kGraphOff / kGraphCg / kGraphAll (set_graphs)
+    };
+    void set_tuning(const Tuning& t);
+    const Tuning& tuning() const { return tuning_; }
     // Refined projection of the 65..256 blocks on (default) or off (every
     // block takes the tridiagonal eigensolver).
     void set_refine(bool on);
@@ -208,8 +221,24 @@ class Batch {
     // [block projections by the refinement, by the fallback, rejections for
     // divergence, correction size, coupled clusters]
     void projection_stats(int64_t* out) const { proj_.refine_stats(out); }
-    void set_refine_params(const RefParams& p) { proj_.set_refine_params(p); }
+    void set_refine_params(const RefParams& p) {
+        proj_.set_refine_params(p);  // the parameters are kernel arguments: re-capture
+        drop_graphs();
+    }
     void set_profiling(bool on) { profile_ = on; }
+    // hipGraph replay of the fixed launch sequences of an outer iteration:
+    // kGraphCg, the first CG chunk (its k_cg_at / k_cg_a / k_cg_reduce
+    // launches); kGraphAll, also k_post with the whole projection (every size
+    // class on its lane, the refinement and its fallback passes, joined into
+    // stream_), the eigenvector formation then running as a branch of the
+    // next CG graph.  Each graph is captured once per launch plan (the set of
+    // running instances), step count and profiling state, then replayed with
+    // one hipGraphLaunch; the kernel timers are event nodes of the graphs.
+    // kGraphOff (default): the same launches one by one.  The results are the
+    // same bits in every mode.
+    static constexpr int kGraphOff = 0, kGraphCg = 1, kGraphAll = 2;
+    void set_graphs(int mode);
+    int graphs() const { return tuning_.graphs; }
     // ms/counts: all launches; full_ms/full_counts: launches made while every
     // instance of the batch was active (and, for CG kernels, still iterating)
     // at the last host poll, whose work is therefore the whole batch's.
@@ -231,7 +260,8 @@ class Batch {
     void sync_state_to_host();
     void sync_state_to_device();
     void refresh_norms(int i);
-    void launch_projection(const BatchDev& B);
+    void launch_projection(const BatchDev& B, bool seed);
+    void launch_seed(const BatchDev& B);
     void join_seed();
     DfState& df_state(int i);
     double run_df_intake(int i, bool norms = true);  // resident DF integrals -> c, v1, E_fzc (and ||c||)
@@ -272,9 +302,13 @@ class Batch {
     bool seed_pending_ = false;
 
     struct KTimer {
-        std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
-        std::vector<char> full;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;  // events of eager launches, reused
         size_t used = 0;
+        struct Pending {
+            hipEvent_t a, b;
+            char full;
+        };
+        std::vector<Pending> pending;  // recorded, not yet read
         double ms = 0.0, full_ms = 0.0;
         int64_t count = 0, full_count = 0;
     };
@@ -284,6 +318,34 @@ class Batch {
     void tstart(int k, hipStream_t s = nullptr);
     void tstop(int k, hipStream_t s = nullptr);
     void tflush();
+    char timer_full(int k) const { return (k == 1 || k == 2) ? all_iterating_ : all_running_; }
+
+    // Captured launch sequences (set_graphs).
+    struct GraphEntry {
+        std::string key;                 // plan, step count, refinement and profiling state
+        hipGraphExec_t exec = nullptr;
+        struct Timer {
+            int kind;
+            hipEvent_t a, b;
+        };
+        std::vector<Timer> timers;       // events of the graph's event-record nodes
+        int64_t launches = 0;            // kernel launches per replay
+        bool unread = false;             // replayed since the timers were last read
+    };
+    template <class Body>
+    void run_graph(GraphEntry& e, const std::string& key, Body&& body);
+    GraphEntry& cg_graph(int steps, bool seed);
+    std::string graph_key(const std::string& tag) const;
+    static constexpr size_t kCgGraphs = 8;  // step counts kept per plan
+    void drop_graph(GraphEntry& e);
+    void drop_graphs();
+    bool seed_owed_ = false;  // a projection graph ran; its eigenvector formation is not launched yet
+    hipEvent_t ev_gfork_ = nullptr, ev_gjoin_ = nullptr;  // the formation's branch in a CG graph
+    hipEvent_t ev_joined_ = nullptr;  // stream_ after the projection's join, for the formation
+    Tuning tuning_;
+    GraphEntry* capture_ = nullptr;      // the entry being captured
+    std::vector<GraphEntry> g_cg_;       // first CG chunk, one entry per step count
+    GraphEntry g_post_;                  // k_post and the projection
 
     // Launch plans of the gather kernels (UnitsDev::plan): the instances whose
     // workgroups are launched.  Host copy in pinned memory, device copy

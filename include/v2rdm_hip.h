@@ -278,6 +278,42 @@ int v2h_batch_Au_noalloc(v2h_batch *b, int inst, const double *u, double *out);
  * made while every instance was running (for CG kernels: still iterating)
  * at the previous host poll, i.e. launches that processed the whole batch. */
 int v2h_batch_set_profiling(v2h_batch *b, int on);
+/* hipGraph replay inside v2h_batch_solve (the graphs field of v2h_tuning):
+ * mode 1 captures the first CG chunk of every outer iteration (its k_cg_at /
+ * k_cg_a / k_cg_reduce launches), mode 2 also k_post with the whole PSD
+ * projection (every size class on its stream, the refinement and its
+ * fallback passes); each graph is captured once per set of running instances
+ * and step count and replayed with one hipGraphLaunch, the kernel timers
+ * becoming event nodes of the graphs.  Mode 0 (default): every kernel
+ * launched one by one.  The results are the same bits in every mode.  The
+ * libsdp loop it replaces launches nothing (v2rdm_solver.cc:1305-1361). */
+int v2h_batch_set_graphs(v2h_batch *b, int mode);
+
+/* Launch configuration of v2h_batch_solve.  None of it changes a result bit:
+ * the same kernels run on the same data in the same dependency order, and
+ * the extra CG steps of a larger first chunk are skipped by the instances
+ * whose CG has stopped.  hilbert_amd/autotune.py times candidates on a batch
+ * and keeps the fastest per batch signature.  No counterpart in the
+ * reference, whose libsdp loop runs on the host. */
+typedef struct {
+    int class_lane[7];    /* stream of each PSD size class when not refined:
+                           * Jacobi <= 16, tridiagonal <= 64 / 128 / 192 /
+                           * 224 / 256, Jacobi > 256; 0 = the batch's main
+                           * stream, 1..3 its side streams */
+    int fallback_lane[7]; /* stream of the tridiagonal fallback of the refined
+                           * classes (entries 2..5 used) */
+    int refine_lane;      /* side stream (1..3) of the refinement and the
+                           * eigenvector formation */
+    int cg_margin;        /* first CG chunk of an outer iteration: the largest
+                           * CG step count of the previous one + cg_margin
+                           * (0..64); then chunks of 4 */
+    int xcd_remap;        /* 1: gather workgroups dealt to the 8 XCDs in
+                           * contiguous ranges (one instance per L2) */
+    int graphs;           /* hipGraph replay: 0 off, 1 first CG chunk, 2 also
+                           * k_post and the projection (v2h_batch_set_graphs) */
+} v2h_tuning;
+int v2h_batch_get_tuning(const v2h_batch *b, v2h_tuning *t);
+int v2h_batch_set_tuning(v2h_batch *b, const v2h_tuning *t);
 int v2h_batch_kernel_times(v2h_batch *b, double *ms, int64_t *counts, double *full_ms, int64_t *full_counts);
 int v2h_batch_reset_kernel_times(v2h_batch *b);
 /* Launch statistics: [kernel launches so far, A gather workgroups,

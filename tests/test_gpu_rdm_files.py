@@ -69,11 +69,15 @@ def test_files_of_a_solved_instance_equal_the_oracle_records(tmp_path):
         assert got.tobytes() == ref[np.abs(ref["value"]) > W.IWL_CUTOFF].tobytes()
 
 
-@pytest.mark.parametrize("which", ["c1_10", "c2v_8_unsorted"])
+def _unsorted():
+    irr = [1, 0, 3, 0, 2, 1, 3]
+    h, g = synthetic_integrals(7, 21, np.random.default_rng(712), irreps=np.array(irr))
+    return Problem(7, 4, 3, irr, 4), h, g
+
+
+@pytest.mark.parametrize("which", ["c1_10", "c2v_7_unsorted"])
 def test_files_reproduce_the_solver_energy(which, tmp_path):
-    prob, h, g = configs.c1_10()[0] if which == "c1_10" else configs.c2v(8, 1)[0]
-    if which != "c1_10":
-        assert list(prob.irreps) != sorted(prob.irreps)
+    prob, h, g = configs.c1_10()[0] if which == "c1_10" else _unsorted()
     B = Batch([prob], options(**OPTS))
     B.set_integrals(0, h, g)
     B.solve_to_convergence()
