@@ -146,16 +146,16 @@ def roofline(B, kt, counts, workload, dom=None, proj=None):
     traffic = load_traffic(dom, workload)
     if dom == "project_refine":
         # one "launch" = the refinement launches of one outer iteration; its
-        # work = the nominal projection flops of the 65..256 blocks it
-        # accepted (the share of block projections it did, from the counters)
-        share = proj["refined"] / max(1, proj["refined"] + proj["fallback"]) if proj else 1.0
-        work = share * sum(executions(dom, o, it) * project_flops([d for d in B.block_dims(i) if 65 <= d <= 256])
-                           for i, (o, it) in enumerate(counts)) / max(cnt, 1)
+        # work = the flops of the d x d x d products it ran (2 d^3 each,
+        # counted per block on the device: K, G, S and X + XE per step, the
+        # final X diag(l+) X^T), over the timed region
+        work = proj["flops"] / max(cnt, 1)
         achieved = work / (per_launch_ms * 1e-3) / 1e12
         return dom, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "k_ref_* (refinement)",
                      "launches": cnt, "per_launch_ms": per_launch_ms, "work_per_launch": work,
-                     "work": "(16/3) d^3 FP64 flops per accepted 65..256 block per outer iteration"}
+                     "work": "2 d^3 FP64 flops per d x d x d product of the refinement (device counters), "
+                             "per outer iteration"}
     if dom.startswith("project"):
         lo, hi = [(a, b) for k, a, b in PROJ_CLASSES if k == dom][0]
         work = sum(executions(dom, o, it) * project_flops([d for d in B.block_dims(i) if lo <= d <= hi])
@@ -621,6 +621,8 @@ def main():
                          "the 16 (20e,20o)-(24e,24o) D2h SDPs with their adjoint gradients; df: the "
                          "density-fitted integral intake; fcidump: the FCIDUMP writer; orbopt: one v2RDM-CASSCF "
                          "orbital step")
+    ap.add_argument("--refine-steps", type=int, default=3,
+                    help="refinement steps before a block falls back to the tridiagonal eigensolver")
     ap.add_argument("--projection", choices=("refined", "tridiagonal"), default="refined",
                     help="PSD blocks of 65..256 rows: warm-started refinement on the FP64 matrix cores with the "
                          "tridiagonal fallback, or the tridiagonal eigensolver alone")
@@ -669,6 +671,7 @@ def main():
         prob.t1, prob.t2, prob.d3 = t1, t2, bool(args.d3)
     B = make_batch(items, args.maxiter)
     B.set_projection(B.PROJECT_REFINED if args.projection == "refined" else B.PROJECT_TRIDIAGONAL)
+    B.set_refine_params(max_steps=args.refine_steps)
     B.set_profiling(True)
     grads = args.workload == "big16"
     tuned = None
@@ -720,7 +723,9 @@ def main():
     workload = {"sdp": f"c2v{n}x{args.batch}", "d2h64": f"d2h{args.batch}",
                 "big16": f"big{args.batch}"}[args.workload] + ("" if cond == "DQG" else "_" + cond.lower().replace("+", ""))
     proj1 = B.projection_stats()
-    proj = {"mode": args.projection, "refined": proj1["refined"] - proj0["refined"],
+    proj = {"mode": args.projection, "refine_steps": args.refine_steps,
+            "flops": proj1["flops"] - proj0["flops"],
+            "refined": proj1["refined"] - proj0["refined"],
             "fallback": proj1["fallback"] - proj0["fallback"]}
     dom, roof = roofline(B, kt, counts, workload, proj=proj)
     # the HBM-bound gather kernel of the CG step, whichever kernel dominates
@@ -745,6 +750,7 @@ def main():
         for k in range(min(args.single, len(items))):
             S = make_batch(items[k:k + 1], args.maxiter)
             S.set_projection(S.PROJECT_REFINED if args.projection == "refined" else S.PROJECT_TRIDIAGONAL)
+            S.set_refine_params(max_steps=args.refine_steps)
             reset_all(S)
             t0 = time.perf_counter()
             solve_with_progress(S, f"single instance {k}")

@@ -142,7 +142,7 @@ SIGNATURES = {
     "v2h_batch_set_projection": (C.c_int, [_vp, C.c_int]),
     "v2h_batch_Au_noalloc": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "v2h_batch_projection_stats": (C.c_int, [_vp, _lp]),
-    "v2h_batch_set_refine_params": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
+    "v2h_batch_set_refine_params": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_int]),
     "v2h_sdp_create": (C.c_int, [C.c_int64, C.c_int64, C.POINTER(SdpOptions), C.c_int, C.POINTER(_vp)]),
     "v2h_sdp_destroy": (C.c_int, [_vp]),
     "v2h_sdp_set_options": (C.c_int, [_vp, C.POINTER(SdpOptions)]),

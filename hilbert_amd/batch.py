@@ -302,14 +302,14 @@ class Batch:
         return getattr(self, "_projection", self.PROJECT_REFINED)
 
     def projection_stats(self) -> dict:
-        st = np.zeros(5, dtype=np.int64)
+        st = np.zeros(6, dtype=np.int64)
         N.check(N.lib().v2h_batch_projection_stats(self._h, st))
         return dict(zip(("refined", "fallback", "rejected_divergence", "rejected_correction",
-                         "rejected_clusters"), (int(v) for v in st)))
+                         "rejected_clusters", "flops"), (int(v) for v in st)))
 
-    def set_refine_params(self, cluster_rel=1e-8, accept_e=2e-7, accept_cpl=1e-13):
+    def set_refine_params(self, cluster_rel=1e-8, accept_e=2e-7, accept_cpl=1e-13, max_steps=3):
         N.check(N.lib().v2h_batch_set_refine_params(self._h, float(cluster_rel), float(accept_e),
-                                                    float(accept_cpl)))
+                                                    float(accept_cpl), int(max_steps)))
 
     def set_profiling(self, on=True):
         N.check(N.lib().v2h_batch_set_profiling(self._h, int(on)))
@@ -327,10 +327,11 @@ class Batch:
 
     GRAPHS_OFF, GRAPHS_CG, GRAPHS_ALL = 0, 1, 2
 
-    def set_graphs(self, mode=GRAPHS_ALL):
+    def set_graphs(self, mode=GRAPHS_CG):
         """hipGraph replay: GRAPHS_CG the first CG chunk of every outer
-        iteration, GRAPHS_ALL also k_post + the projection; GRAPHS_OFF
-        (default) the same launches one by one.  Same bits in every mode."""
+        iteration; GRAPHS_OFF (default) the same launches one by one, same
+        bits.  GRAPHS_ALL (k_post + the projection too) is switched off and
+        raises."""
         N.check(N.lib().v2h_batch_set_graphs(self._h, int(mode)))
 
     def kernel_times(self, full=False) -> dict:

@@ -610,15 +610,17 @@ int v2h_batch_projection_stats(v2h_batch* b, int64_t* st) {
     });
 }
 
-int v2h_batch_set_refine_params(v2h_batch* b, double cluster_rel, double accept_e, double accept_cpl) {
+int v2h_batch_set_refine_params(v2h_batch* b, double cluster_rel, double accept_e, double accept_cpl,
+                                int max_steps) {
     return guarded([&] {
         if (!(cluster_rel >= 0.0 && cluster_rel < 1e-2) || !(accept_e > 0.0 && accept_e <= 1e-3) ||
-            !(accept_cpl > 0.0 && accept_cpl <= 1e-6))
+            !(accept_cpl > 0.0 && accept_cpl <= 1e-6) || max_steps < 1 || max_steps > v2h::kMaxSteps)
             throw std::invalid_argument("refinement thresholds out of range");
         v2h::RefParams p;
         p.cluster_rel = cluster_rel;
         p.accept_e = accept_e;
         p.accept_cpl = accept_cpl;
+        p.max_steps = max_steps;
         B(b).set_refine_params(p);
     });
 }

@@ -45,6 +45,7 @@ class Projector {
     void refine_stats(int64_t* out) const { refiner_.stats(out); }
     void set_refine_params(const RefParams& p) { refiner_.set_params(p); }
     const RefParams& refine_params() const { return refiner_.params(); }
+    int refine_launches() const { return refiner_.launches(); }
 
   private:
     struct Class {

@@ -345,6 +345,7 @@ __global__ void k_ref_judge(RefDev R, RefParams P, int k, int xout) {
     }
     rs.max_e = me;
     rs.cpl = cpl;
+    const int64_t d3 = 2 * (int64_t)b.d * b.d * b.d;
     bool reject = false;
     if (!(me <= 0.25) || !(cpl == cpl)) {
         reject = true;
@@ -354,13 +355,14 @@ __global__ void k_ref_judge(RefDev R, RefParams P, int k, int xout) {
         rs.cstep = k;
         rs.xfinal = xout;
         rs.backoff = 0;
-    } else if (k >= 3) {
+    } else if (k >= P.max_steps) {
         reject = true;
         if (me > P.accept_e)
             rs.rej_e += 1;
         else
             rs.rej_c += 1;
     }
+    rs.flops += (reject ? 3 : 4) * d3;  // a rejected block skips X + XE
     if (reject) {
         rs.phase = 3;
         const int bo = min(2 * rs.backoff + 2, kMaxBackoff);
@@ -393,6 +395,7 @@ __global__ __launch_bounds__(kNT) void k_ref_close(BatchDev B, RefDev R) {
         B.st[b.inst].pending = 1;
         R.route[b.gblock] = kRouteAccepted;
         rs.refined += 1;
+        rs.flops += 2 * (int64_t)b.d * b.d * b.d;
     }
 }
 
@@ -505,8 +508,9 @@ void launch_ref_sym(const BatchDev& B, const RefDev& R, hipStream_t s) {
 
 void launch_ref_step(const BatchDev& B, const RefDev& R, const RefParams& P, int k, hipStream_t s) {
     if (R.nblk == 0) return;
-    const int xin = k == 1 ? kSlotV : (k == 2 ? kSlotXa : kSlotXb);
-    const int xout = k == 2 ? kSlotXb : kSlotXa;
+    // X alternates between two slots: V -> Xa -> Xb -> Xa -> ...
+    const int xin = k == 1 ? kSlotV : (k % 2 == 0 ? kSlotXa : kSlotXb);
+    const int xout = k % 2 == 1 ? kSlotXa : kSlotXb;
     // K = U X, G = X^T X
     const Prod pk{kSlotU, xin, kSlotK, -1, kAN, kBN, kEpiStore, 0};
     const Prod pg{xin, xin, kSlotG, -1, kAT, kBN, kEpiStore, 0};
@@ -597,7 +601,7 @@ void Refiner::refine(const BatchDev& B, hipStream_t s) const {
     if (blk_.empty()) return;
     const RefDev R = dev();
     launch_ref_sym(B, R, s);
-    for (int k = 1; k <= 3; k++) launch_ref_step(B, R, params_, k, s);
+    for (int k = 1; k <= params_.max_steps; k++) launch_ref_step(B, R, params_, k, s);
     launch_ref_final(B, R, s);
 }
 
@@ -621,6 +625,7 @@ void Refiner::stats(int64_t* out) const {
         out[2] += r.rej_div;
         out[3] += r.rej_e;
         out[4] += r.rej_c;
+        out[5] += r.flops;
     }
 }
 

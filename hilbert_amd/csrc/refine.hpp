@@ -56,6 +56,9 @@ struct RefState {
     // rejections: diverging or non-finite corrections, correction above the
     // acceptance threshold after the last step, clusters left coupled
     int64_t rej_div, rej_e, rej_c;
+    // FP64 flops of the block's d x d x d products so far (2 d^3 each: K, G,
+    // S and X + XE per step, the final X diag(l+) X^T)
+    int64_t flops;
     // After a rejection the block stays on the fallback for `wait` outer
     // iterations (doubling with consecutive rejections, up to kMaxBackoff);
     // its eigenvectors are formed only for the iteration before it tries again.
@@ -88,7 +91,9 @@ struct RefParams {
     double cluster_rel = 1e-8;   // |l_i - l_j| <= cluster_rel max|l|: one cluster
     double accept_e = 2e-7;      // accept a step whose largest correction is below this
     double accept_cpl = 1e-13;   // ... and whose clusters are decoupled to this (relative to max|l|)
+    int max_steps = 3;           // refinement steps before a block is rejected (1..kMaxSteps)
 };
+constexpr int kMaxSteps = 8;
 
 // Start of an outer iteration: phases, backoff and routes of every block.
 void launch_ref_state(const BatchDev& B, const RefDev& R, hipStream_t s);
@@ -116,14 +121,15 @@ class Refiner {
     // the refinement and routes its rejections to kRouteRejected.
     void begin(const BatchDev& B, hipStream_t s) const;
     void refine(const BatchDev& B, hipStream_t s) const;
-    static constexpr int kLaunches = 18;  // kernel launches of begin() + refine()
+    // kernel launches of begin() + refine(): state, sym, five per step, final product and close
+    int launches() const { return 4 + 5 * params_.max_steps; }
     // After the fallback: eigenvectors of the blocks it projected.
     void seed(const BatchDev& B, hipStream_t s) const;
     void forget(int inst, hipStream_t s) const;
     const int* route() const { return route_.p; }
     // [block projections by the refinement, by the fallback, rejections for
     // divergence, for the correction size, for coupled clusters]
-    static constexpr int kStats = 5;
+    static constexpr int kStats = 6;
     void stats(int64_t* out) const;
     void set_work_base(const double* tri_work) { tri_work_ = tri_work; }
     void set_params(const RefParams& p) { params_ = p; }

@@ -99,13 +99,13 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     if (descs.empty()) throw std::invalid_argument("empty batch");
     V2H_HIP(hipSetDevice(device_));
     V2H_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    for (int k = 0; k < kSideStreams; k++) {
-        V2H_HIP(hipStreamCreateWithFlags(&pstream_[k], hipStreamNonBlocking));
-        V2H_HIP(hipEventCreateWithFlags(&ev_proj_[k], hipEventDisableTiming));
+    for (int k = 0; k < kSideStreams; k++) V2H_HIP(hipStreamCreateWithFlags(&pstream_[k], hipStreamNonBlocking));
+    for (ProjEvents* E : {&ev_eager_, &ev_capture_}) {
+        for (int k = 0; k < kSideStreams; k++) V2H_HIP(hipEventCreateWithFlags(&E->proj[k], hipEventDisableTiming));
+        V2H_HIP(hipEventCreateWithFlags(&E->post, hipEventDisableTiming));
+        V2H_HIP(hipEventCreateWithFlags(&E->ref, hipEventDisableTiming));
+        V2H_HIP(hipEventCreateWithFlags(&E->state, hipEventDisableTiming));
     }
-    V2H_HIP(hipEventCreateWithFlags(&ev_post_, hipEventDisableTiming));
-    V2H_HIP(hipEventCreateWithFlags(&ev_ref_, hipEventDisableTiming));
-    V2H_HIP(hipEventCreateWithFlags(&ev_state_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_seed_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_gfork_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_joined_, hipEventDisableTiming));
@@ -198,13 +198,14 @@ Batch::~Batch() {
         }
     if (pinned_) (void)hipHostFree(pinned_);
     if (plan_pinned_) (void)hipHostFree(plan_pinned_);
-    for (int k = 0; k < kSideStreams; k++) {
+    for (int k = 0; k < kSideStreams; k++)
         if (pstream_[k]) (void)hipStreamDestroy(pstream_[k]);
-        if (ev_proj_[k]) (void)hipEventDestroy(ev_proj_[k]);
+    for (ProjEvents* E : {&ev_eager_, &ev_capture_}) {
+        for (int k = 0; k < kSideStreams; k++)
+            if (E->proj[k]) (void)hipEventDestroy(E->proj[k]);
+        for (hipEvent_t e : {E->post, E->ref, E->state})
+            if (e) (void)hipEventDestroy(e);
     }
-    if (ev_post_) (void)hipEventDestroy(ev_post_);
-    if (ev_ref_) (void)hipEventDestroy(ev_ref_);
-    if (ev_state_) (void)hipEventDestroy(ev_state_);
     if (ev_seed_) (void)hipEventDestroy(ev_seed_);
     if (ev_gfork_) (void)hipEventDestroy(ev_gfork_);
     if (ev_joined_) (void)hipEventDestroy(ev_joined_);
@@ -494,7 +495,10 @@ void Batch::set_tuning(const Tuning& t) {
     if (t.cg_margin < 0 || t.cg_margin > 64) throw std::invalid_argument("tuning: cg_margin is 0..64");
     if (t.xcd_remap != 0 && t.xcd_remap != 1) throw std::invalid_argument("tuning: xcd_remap is 0 or 1");
     if (t.graphs < kGraphOff || t.graphs > kGraphAll)
-        throw std::invalid_argument("tuning: graphs is 0 (off), 1 (CG chunk) or 2 (CG chunk and projection)");
+        throw std::invalid_argument("tuning: graphs is 0 (off) or 1 (CG chunk)");
+    if (t.graphs == kGraphAll)  // see kGraphAll
+        throw std::invalid_argument("tuning: graph mode 2 (CG chunk and projection) is switched off: solves split "
+                                    "into short calls did not always reproduce the eager bits");
     V2H_HIP(hipSetDevice(device_));
     join_seed();  // the eigenvector formation in flight ran on the old refinement lane
     V2H_HIP(hipStreamSynchronize(stream_));
@@ -888,27 +892,28 @@ int Batch::solve(int maxiter, const Monitor& mon) {
 // after the fallback lanes, off the critical path.
 void Batch::launch_projection(const BatchDev& B, bool seed) {
     const int rl = tuning_.refine_lane;
-    V2H_HIP(hipEventRecord(ev_post_, stream_));
+    const ProjEvents& E = capture_ ? ev_capture_ : ev_eager_;
+    V2H_HIP(hipEventRecord(E.post, stream_));
     bool used[kSideStreams + 1] = {};
     const bool refine = proj_.refine_on();
     hipStream_t rs = pstream_[rl - 1];
     if (refine) {
-        V2H_HIP(hipStreamWaitEvent(rs, ev_post_, 0));
+        V2H_HIP(hipStreamWaitEvent(rs, E.post, 0));
         used[rl] = true;
         tstart(kKindRefine, rs);
         proj_.refine_begin(B, rs);
-        V2H_HIP(hipEventRecord(ev_state_, rs));
+        V2H_HIP(hipEventRecord(E.state, rs));
         proj_.refine_run(B, rs);
         tstop(kKindRefine, rs);
-        launches_ += Refiner::kLaunches;
-        V2H_HIP(hipEventRecord(ev_ref_, rs));
+        launches_ += proj_.refine_launches();
+        V2H_HIP(hipEventRecord(E.ref, rs));
     }
     auto lane_stream = [&](int ln) { return ln == 0 ? stream_ : pstream_[ln - 1]; };
     for (int k = kProjClasses - 1; k >= 0; k--) {
         if (!proj_.has(k) || (refine && Projector::refined_class(k))) continue;
         const int ln = tuning_.class_lane[k];
         hipStream_t ps = lane_stream(ln);
-        if (ln > 0 && !used[ln]) V2H_HIP(hipStreamWaitEvent(ps, ev_post_, 0));
+        if (ln > 0 && !used[ln]) V2H_HIP(hipStreamWaitEvent(ps, E.post, 0));
         used[ln] = true;
         tstart(4 + k, ps);
         proj_.launch(k, B, ps);
@@ -922,7 +927,7 @@ void Batch::launch_projection(const BatchDev& B, bool seed) {
                 if (!proj_.has(k) || !Projector::refined_class(k)) continue;
                 const int ln = tuning_.fallback_lane[k];
                 hipStream_t ps = lane_stream(ln);
-                if (!waited[ln]) V2H_HIP(hipStreamWaitEvent(ps, pass == 0 ? ev_state_ : ev_ref_, 0));
+                if (!waited[ln]) V2H_HIP(hipStreamWaitEvent(ps, pass == 0 ? E.state : E.ref, 0));
                 waited[ln] = used[ln] = true;
                 tstart(4 + k, ps);
                 proj_.launch(k, B, ps, pass == 0 ? kRouteFallback : kRouteRejected);
@@ -933,8 +938,8 @@ void Batch::launch_projection(const BatchDev& B, bool seed) {
     }
     for (int ln = 1; ln <= kSideStreams; ln++)
         if (used[ln]) {
-            V2H_HIP(hipEventRecord(ev_proj_[ln - 1], pstream_[ln - 1]));
-            V2H_HIP(hipStreamWaitEvent(stream_, ev_proj_[ln - 1], 0));
+            V2H_HIP(hipEventRecord(E.proj[ln - 1], pstream_[ln - 1]));
+            V2H_HIP(hipStreamWaitEvent(stream_, E.proj[ln - 1], 0));
         }
     if (refine && seed) {
         // after every fallback launch, whichever lane (stream_ included) ran it
@@ -953,8 +958,8 @@ void Batch::launch_projection(const BatchDev& B, bool seed) {
 void Batch::launch_seed(const BatchDev& B) {
     if (!proj_.refine_on()) return;
     hipStream_t rs = pstream_[tuning_.refine_lane - 1];
-    V2H_HIP(hipEventRecord(ev_post_, stream_));
-    V2H_HIP(hipStreamWaitEvent(rs, ev_post_, 0));
+    V2H_HIP(hipEventRecord(ev_eager_.post, stream_));
+    V2H_HIP(hipStreamWaitEvent(rs, ev_eager_.post, 0));
     tstart(kKindSeed, rs);
     proj_.refine_seed(B, rs);
     tstop(kKindSeed, rs);

@@ -234,8 +234,11 @@ class Batch {
     // next CG graph.  Each graph is captured once per launch plan (the set of
     // running instances), step count and profiling state, then replayed with
     // one hipGraphLaunch; the kernel timers are event nodes of the graphs.
-    // kGraphOff (default): the same launches one by one.  The results are the
-    // same bits in every mode.
+    // kGraphOff (default): the same launches one by one.  kGraphCg gives the
+    // eager bits in every test.  kGraphAll is switched off (set_tuning
+    // rejects it): it gave the eager bits over whole solves, but a batch
+    // solved in calls of a few outer iterations sometimes stopped at other
+    // iterations, and the cause was not found.
     static constexpr int kGraphOff = 0, kGraphCg = 1, kGraphAll = 2;
     void set_graphs(int mode);
     int graphs() const { return tuning_.graphs; }
@@ -297,8 +300,14 @@ class Batch {
     Projector proj_;
     // side streams of the projection, joined back into stream_ by events
     hipStream_t pstream_[kSideStreams] = {};
-    hipEvent_t ev_post_ = nullptr, ev_proj_[kSideStreams] = {}, ev_ref_ = nullptr, ev_state_ = nullptr,
-               ev_seed_ = nullptr;
+    // the fork / join events of the projection: one set for eager launches and
+    // one used only inside graph captures (an event is never recorded both in
+    // a capture and outside one)
+    struct ProjEvents {
+        hipEvent_t post = nullptr, state = nullptr, ref = nullptr, proj[kSideStreams] = {};
+    };
+    ProjEvents ev_eager_, ev_capture_;
+    hipEvent_t ev_seed_ = nullptr;
     bool seed_pending_ = false;
 
     struct KTimer {

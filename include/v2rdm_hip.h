@@ -280,13 +280,12 @@ int v2h_batch_Au_noalloc(v2h_batch *b, int inst, const double *u, double *out);
 int v2h_batch_set_profiling(v2h_batch *b, int on);
 /* hipGraph replay inside v2h_batch_solve (the graphs field of v2h_tuning):
  * mode 1 captures the first CG chunk of every outer iteration (its k_cg_at /
- * k_cg_a / k_cg_reduce launches), mode 2 also k_post with the whole PSD
- * projection (every size class on its stream, the refinement and its
- * fallback passes); each graph is captured once per set of running instances
- * and step count and replayed with one hipGraphLaunch, the kernel timers
- * becoming event nodes of the graphs.  Mode 0 (default): every kernel
- * launched one by one.  The results are the same bits in every mode.  The
- * libsdp loop it replaces launches nothing (v2rdm_solver.cc:1305-1361). */
+ * k_cg_a / k_cg_reduce launches) once per set of running instances and step
+ * count and replays it with one hipGraphLaunch, the kernel timers becoming
+ * event nodes of the graph; the results are the bits of mode 0 (default,
+ * every kernel launched one by one).  Mode 2 (k_post and the projection in
+ * the graph too) is switched off and rejected with V2H_ERR_INVALID_ARGUMENT.
+ * The libsdp loop it replaces launches nothing (v2rdm_solver.cc:1305-1361). */
 int v2h_batch_set_graphs(v2h_batch *b, int mode);
 
 /* Launch configuration of v2h_batch_solve.  None of it changes a result bit:
@@ -309,8 +308,8 @@ typedef struct {
                            * (0..64); then chunks of 4 */
     int xcd_remap;        /* 1: gather workgroups dealt to the 8 XCDs in
                            * contiguous ranges (one instance per L2) */
-    int graphs;           /* hipGraph replay: 0 off, 1 first CG chunk, 2 also
-                           * k_post and the projection (v2h_batch_set_graphs) */
+    int graphs;           /* hipGraph replay: 0 off, 1 first CG chunk
+                           * (v2h_batch_set_graphs) */
 } v2h_tuning;
 int v2h_batch_get_tuning(const v2h_batch *b, v2h_tuning *t);
 int v2h_batch_set_tuning(v2h_batch *b, const v2h_tuning *t);
@@ -337,13 +336,16 @@ int v2h_batch_set_projection(v2h_batch *b, int mode);
 /* Over the blocks of 65..256 rows since the batch was created: [block
  * projections done by the refinement, by the tridiagonal fallback, refinements
  * rejected for a diverging or non-finite correction, for a correction still
- * above accept_e after three steps, for clusters left coupled]. */
+ * above accept_e after max_steps steps, for clusters left coupled, FP64 flops
+ * of the refinement's d x d x d products (2 d^3 each)]; stats[6]. */
 int v2h_batch_projection_stats(v2h_batch *b, int64_t *stats);
 /* Thresholds of the refinement: eigenvalue estimates closer than cluster_rel
  * x max|l| form a cluster (default 1e-8); a step is accepted when its largest
  * correction is at most accept_e (2e-7) and the couplings left inside
- * clusters at most accept_cpl x max|l| (1e-13). */
-int v2h_batch_set_refine_params(v2h_batch *b, double cluster_rel, double accept_e, double accept_cpl);
+ * clusters at most accept_cpl x max|l| (1e-13); a block not accepted after
+ * max_steps steps (1..8, default 3) takes the tridiagonal fallback. */
+int v2h_batch_set_refine_params(v2h_batch *b, double cluster_rel, double accept_e, double accept_cpl,
+                                int max_steps);
 
 /* Adjoint of the cost packing: dE/dh[nmo*nmo], dE/dg[nmo^4] (caller order)
  * for E = c . xbar; xbar = NULL uses the current primal solution x. */

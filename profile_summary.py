@@ -10,6 +10,13 @@ and over the dispatches with the kernel's largest grid.  The largest grid is
 a launch over the whole batch (instances that converged drop out of later
 launches), which is what bench.py's per-kernel roofline times.
 
+counters -- from a counter run of any command
+
+    rocprofv3 --pmc C1 C2 ... --output-format csv -d <dir> -o run -- python3 bench.py ...
+    python profile_summary.py counters <dir> --counter C1 --counter C2 ...
+
+prints, per kernel, the dispatch count and each counter's mean per dispatch.
+
 traffic -- from two counter runs of the same short bench command
 
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir>/fetch -o run -- python3 bench.py --batch B \
@@ -101,6 +108,20 @@ def stats(d: str):
               f"{len(full):10d} {sum(full) / len(full) / 1e3:12.1f} {gmax:10d}")
 
 
+def counters(args):
+    """Per kernel: dispatches and the mean of each counter per dispatch."""
+    vals = {c: read_counter(args.dir, c) for c in args.counter}
+    kernels = sorted({short_name(k) for v in vals.values() for k in v})
+    print(f"{'kernel':24s} {'dispatches':>10s} " + " ".join(f"{c:>28s}" for c in args.counter))
+    for k in kernels:
+        row, n = [], 0
+        for c in args.counter:
+            v = [x for kk, xs in vals[c].items() if short_name(kk) == k for x in xs]
+            n = max(n, len(v))
+            row.append(f"{(sum(v) / len(v)) if v else float('nan'):28.6g}")
+        print(f"{k:24s} {n:10d} " + " ".join(row))
+
+
 def traffic(args):
     fetch = read_counter(args.fetch, "FETCH_SIZE")
     write = read_counter(args.write, "WRITE_SIZE")
@@ -135,6 +156,9 @@ def main():
     sub = ap.add_subparsers(dest="cmd", required=True)
     a = sub.add_parser("stats")
     a.add_argument("dir")
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--counter", action="append", required=True)
     t = sub.add_parser("traffic")
     t.add_argument("--workload", required=True)
     t.add_argument("--fetch", required=True)
@@ -144,6 +168,8 @@ def main():
     args = ap.parse_args()
     if args.cmd == "stats":
         stats(args.dir)
+    elif args.cmd == "counters":
+        counters(args)
     else:
         traffic(args)
 

@@ -1,29 +1,37 @@
-"""GPU: hipGraph replay of the outer iteration's fixed launch sequences.
+"""GPU: hipGraph replay of the outer iteration's first CG chunk.
 
-Mode GRAPHS_CG captures the first CG chunk of every outer iteration, mode
-GRAPHS_ALL also k_post + the whole projection (the eigenvector formation then
-runs as a branch of the next CG graph); each graph is captured once per
-launch plan and replayed.  Without graphs every kernel is launched one by
-one.  The kernels are the same and deterministic, so all modes must agree
-bit for bit:
+Mode GRAPHS_CG captures the first CG chunk of every outer iteration (its
+k_cg_at / k_cg_a / k_cg_reduce launches) once per launch plan and step count
+and replays it.  Without graphs every kernel is launched one by one.  The
+kernels are the same and deterministic, so the two must agree bit for bit:
 
 * a batch of two (16e,16o) instances and two small ones, 400 outer iterations
   (hundreds of replays, interleaved with eager launches and host polls), with
   the refinement on and off;
 * a mixed batch solved to convergence, whose instances stop at different
-  iterations, so the plan shrinks and the graphs are captured again;
-* with the kernel timers on, the graphs' event nodes give the same launch
-  counts as the eager launches.
+  iterations, so the plan shrinks and the graphs are captured again; with
+  the kernel timers on, the graphs' event nodes give the same launch counts
+  as the eager launches;
+* a batch solved in calls of 11 outer iterations, switching the mode at
+  every call.
+
+Mode GRAPHS_ALL (k_post and the projection in the graph as well) is
+switched off: it gave the eager bits over whole solves, but solves split
+into calls of a few outer iterations sometimes stopped at other iterations
+and the cause was not found.  It is rejected; its equality tests are kept,
+skipped, for when it is fixed.
 """
 import numpy as np
 import pytest
 
 from hilbert_amd import configs
+from hilbert_amd._native import V2hError
 from hilbert_amd.batch import Batch, options
 
 pytestmark = pytest.mark.gpu
 
 NEVER = dict(e_convergence=1e-14, r_convergence=1e-13, maxiter=1000000)
+MODE2_OFF = "graph mode 2 is switched off: short solve calls did not always reproduce the eager bits"
 
 
 def _items():
@@ -48,7 +56,8 @@ def _run(items, graphs, refine, iters, opts, profile=False):
     return out, times
 
 
-@pytest.mark.parametrize("mode", [Batch.GRAPHS_CG, Batch.GRAPHS_ALL])
+@pytest.mark.parametrize("mode", [Batch.GRAPHS_CG,
+                                  pytest.param(Batch.GRAPHS_ALL, marks=pytest.mark.skip(reason=MODE2_OFF))])
 @pytest.mark.parametrize("refine", [True, False])
 def test_graphs_equal_eager_launches(refine, mode):
     items = _items()
@@ -60,10 +69,12 @@ def test_graphs_equal_eager_launches(refine, mode):
             assert np.array_equal(ra[k], rb[k]), (i, k)
 
 
-def test_graphs_follow_a_shrinking_plan():
+@pytest.mark.parametrize("mode", [Batch.GRAPHS_CG,
+                                  pytest.param(Batch.GRAPHS_ALL, marks=pytest.mark.skip(reason=MODE2_OFF))])
+def test_graphs_follow_a_shrinking_plan(mode):
     items = configs.batch64_d2h(5) + configs.c2v(16, 1)
     opts = dict(e_convergence=1e-6, r_convergence=1e-5, maxiter=100000)
-    a, ta = _run(items, Batch.GRAPHS_ALL, True, None, opts, profile=True)
+    a, ta = _run(items, mode, True, None, opts, profile=True)
     b, tb = _run(items, Batch.GRAPHS_OFF, True, None, opts, profile=True)
     oiters = sorted({r[3]["oiter"] for r in a})
     print(f"stop iterations {oiters}", flush=True)
@@ -74,3 +85,31 @@ def test_graphs_follow_a_shrinking_plan():
     for kind in ("pre", "cg_at", "cg_a", "post", "project_refine", "project_seed"):
         assert ta[kind][1] == tb[kind][1] > 0, (kind, ta[kind], tb[kind])
         assert ta[kind][0] > 0.0
+
+
+def test_switching_graphs_between_short_calls():
+    items = configs.c2v(16, 4)
+    opts = dict(e_convergence=1e-7, r_convergence=1e-6, maxiter=100000)
+    ref, _ = _run(items, Batch.GRAPHS_OFF, True, None, opts)
+    B = Batch([p for p, _, _ in items], options(**opts))
+    for i, (_, h, g) in enumerate(items):
+        B.set_integrals(i, h, g)
+    B.solve(300)
+    for k in range(20):
+        B.set_graphs(Batch.GRAPHS_CG if k % 2 == 0 else Batch.GRAPHS_OFF)
+        B.solve(11)
+    B.solve_to_convergence()
+    for i, r in enumerate(ref):
+        st = B.status(i)
+        assert st["oiter"] == r[3]["oiter"] and st["iiter"] == r[3]["iiter"], i
+        assert np.array_equal(B.x(i), r[0]), i
+    B.close()
+
+
+def test_graph_mode_2_is_rejected():
+    prob, h, g = configs.batch64_d2h(1)[0]
+    B = Batch([prob], options())
+    with pytest.raises(V2hError, match="switched off"):
+        B.set_graphs(Batch.GRAPHS_ALL)
+    assert B.tuning()["graphs"] == Batch.GRAPHS_OFF
+    B.close()
