@@ -621,7 +621,7 @@ def main():
                          "the 16 (20e,20o)-(24e,24o) D2h SDPs with their adjoint gradients; df: the "
                          "density-fitted integral intake; fcidump: the FCIDUMP writer; orbopt: one v2RDM-CASSCF "
                          "orbital step")
-    ap.add_argument("--refine-steps", type=int, default=3,
+    ap.add_argument("--refine-steps", type=int, default=5,
                     help="refinement steps before a block falls back to the tridiagonal eigensolver")
     ap.add_argument("--projection", choices=("refined", "tridiagonal"), default="refined",
                     help="PSD blocks of 65..256 rows: warm-started refinement on the FP64 matrix cores with the "

@@ -307,7 +307,7 @@ class Batch:
         return dict(zip(("refined", "fallback", "rejected_divergence", "rejected_correction",
                          "rejected_clusters", "flops"), (int(v) for v in st)))
 
-    def set_refine_params(self, cluster_rel=1e-8, accept_e=2e-7, accept_cpl=1e-13, max_steps=3):
+    def set_refine_params(self, cluster_rel=1e-8, accept_e=2e-7, accept_cpl=1e-13, max_steps=5):
         N.check(N.lib().v2h_batch_set_refine_params(self._h, float(cluster_rel), float(accept_e),
                                                     float(accept_cpl), int(max_steps)))
 

@@ -12,9 +12,9 @@
 //     X <- X + X E,
 // which converges quadratically, restores the orthogonality of X in the same
 // step, and then U+ = X diag(l+) X^T.  A block whose correction does not fall
-// below the acceptance threshold within three steps, and every block without
-// eigenvectors yet (first iteration, after a reset), takes the tridiagonal
-// eigensolver instead; its eigenvectors seed the next iteration.
+// below the acceptance threshold within max_steps steps, and every block
+// without eigenvectors yet (first iteration, after a reset), takes the
+// tridiagonal eigensolver instead; its eigenvectors seed the next iteration.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -91,7 +91,10 @@ struct RefParams {
     double cluster_rel = 1e-8;   // |l_i - l_j| <= cluster_rel max|l|: one cluster
     double accept_e = 2e-7;      // accept a step whose largest correction is below this
     double accept_cpl = 1e-13;   // ... and whose clusters are decoupled to this (relative to max|l|)
-    int max_steps = 3;           // refinement steps before a block is rejected (1..kMaxSteps)
+    // refinement steps before a block is rejected (1..kMaxSteps): 5 keeps
+    // most blocks that converge slowly off the tridiagonal fallback, whose
+    // launch after the refinement is on the critical path of the iteration
+    int max_steps = 5;
 };
 constexpr int kMaxSteps = 8;
 

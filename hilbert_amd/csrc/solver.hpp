@@ -209,7 +209,9 @@ class Batch {
         int fallback_lane[kProjClasses] = {0, 0, 2, 3, 3, 2, 2};
         int refine_lane = 1;  // the warm-started refinement and the eigenvector formation (1..3)
         int cg_margin = 2;    // first CG chunk: the largest previous CG step count + cg_margin
-        int xcd_remap = 1;    // gather workgroups dealt to the XCDs in contiguous ranges
+        // gather workgroups dealt to the XCDs in contiguous ranges (1) or in
+        // launch order (0, default: measured faster on the (16e,16o) batches)
+        int xcd_remap = 0;
         int graphs = 0;       // kGraphOff / kGraphCg / kGraphAll (set_graphs)
     };
     void set_tuning(const Tuning& t);

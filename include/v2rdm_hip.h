@@ -307,7 +307,8 @@ typedef struct {
                            * CG step count of the previous one + cg_margin
                            * (0..64); then chunks of 4 */
     int xcd_remap;        /* 1: gather workgroups dealt to the 8 XCDs in
-                           * contiguous ranges (one instance per L2) */
+                           * contiguous ranges (one instance per L2); 0
+                           * (default): in launch order */
     int graphs;           /* hipGraph replay: 0 off, 1 first CG chunk
                            * (v2h_batch_set_graphs) */
 } v2h_tuning;
@@ -324,8 +325,9 @@ int v2h_batch_launch_stats(v2h_batch *b, int64_t *stats);
  * list v2rdm_solver.cc:3729-3846):
  *   V2H_PROJECT_REFINED (default): the eigenvectors of the previous outer
  *     iteration are refined by FP64 matrix-core products (iterative
- *     refinement of Ogita and Aishima, at most three steps, accepted when the
- *     last correction is below 2e-7); blocks without eigenvectors yet, or
+ *     refinement of Ogita and Aishima, at most five steps by default,
+ *     accepted when the last correction is below 2e-7); blocks without
+ *     eigenvectors yet, or
  *     whose refinement is not accepted, take the tridiagonal eigensolver,
  *     whose eigenvectors seed the next iteration;
  *   V2H_PROJECT_TRIDIAGONAL: every block takes the tridiagonal eigensolver.
@@ -343,7 +345,7 @@ int v2h_batch_projection_stats(v2h_batch *b, int64_t *stats);
  * x max|l| form a cluster (default 1e-8); a step is accepted when its largest
  * correction is at most accept_e (2e-7) and the couplings left inside
  * clusters at most accept_cpl x max|l| (1e-13); a block not accepted after
- * max_steps steps (1..8, default 3) takes the tridiagonal fallback. */
+ * max_steps steps (1..8, default 5) takes the tridiagonal fallback. */
 int v2h_batch_set_refine_params(v2h_batch *b, double cluster_rel, double accept_e, double accept_cpl,
                                 int max_steps);
 
