@@ -1,7 +1,7 @@
 """Benchmark: converged v2RDM SDPs per second at (16e,16o), batched on one MI355X.
 
 A step is one batched solve, from hilbert's random guess to convergence
-(E_CONVERGENCE 1e-7, R_CONVERGENCE 1e-6 by default), of B independent (16e,16o) DQG SDPs
+(E_CONVERGENCE 5e-8, R_CONVERGENCE 5e-7 by default), of B independent (16e,16o) DQG SDPs
 with C2v irrep labels drawn at random and synthetic molecular-like integrals
 (naux = 48), every instance with its own seed.  Integrals and tables are
 resident in HBM before the timed region; the timed region is reset + solve.
@@ -55,10 +55,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 # E_CONVERGENCE, R_CONVERGENCE of the timed solves (--e-convergence,
-# --r-convergence): 1e-7 / 1e-6 stop more than nine instances in ten within
-# 1e-6 Eh of the optimum; hilbert's defaults 1e-6 / 1e-5 stop about half of
-# them further away (the band those thresholds allow is some 1e-5 Eh wide)
-CONV = [1e-7, 1e-6]
+# --r-convergence): 5e-8 / 5e-7 stop nearly every instance within 1e-6 Eh of
+# the optimum; hilbert's defaults 1e-6 / 1e-5 stop about half of them further
+# away (the band those thresholds allow is some 1e-5 Eh wide)
+CONV = [5e-8, 5e-7]
 
 
 def bytes_per_instance(kind, n_p, n_d):
@@ -609,8 +609,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=300,
                     help="sdp workloads: tune the launch configuration after this many outer iterations of the "
                          "first warm-up solve (0: keep the defaults)")
-    ap.add_argument("--e-convergence", type=float, default=1e-7, help="E_CONVERGENCE of the timed solves")
-    ap.add_argument("--r-convergence", type=float, default=1e-6, help="R_CONVERGENCE of the timed solves")
+    ap.add_argument("--e-convergence", type=float, default=5e-8, help="E_CONVERGENCE of the timed solves")
+    ap.add_argument("--r-convergence", type=float, default=5e-7, help="R_CONVERGENCE of the timed solves")
     ap.add_argument("--mu-update-frequency", type=int, default=1000,
                     help="sdp workload: MU_UPDATE_FREQUENCY, outer iterations between mu rebalancings "
                          "(hilbert's default 1000)")

@@ -356,14 +356,6 @@ void Batch::sync_state_to_host() {
 void Batch::tstart(int k, hipStream_t s) {
     if (!profile_) return;
     if (!s) s = stream_;
-    if (capture_) {  // an event-record node of the graph being captured
-        hipEvent_t a, b;
-        V2H_HIP(hipEventCreate(&a));
-        V2H_HIP(hipEventCreate(&b));
-        capture_->timers.push_back({k, a, b});
-        V2H_HIP(hipEventRecordWithFlags(a, s, hipEventRecordExternal));
-        return;
-    }
     KTimer& t = timers_[k];
     if (t.used == t.pool.size()) {
         hipEvent_t a, b;
@@ -377,12 +369,6 @@ void Batch::tstart(int k, hipStream_t s) {
 void Batch::tstop(int k, hipStream_t s) {
     if (!profile_) return;
     if (!s) s = stream_;
-    if (capture_) {
-        if (capture_->timers.empty() || capture_->timers.back().kind != k)
-            throw std::logic_error("kernel timer: tstop without its tstart");
-        V2H_HIP(hipEventRecordWithFlags(capture_->timers.back().b, s, hipEventRecordExternal));
-        return;
-    }
     KTimer& t = timers_[k];
     V2H_HIP(hipEventRecord(t.pool[t.used].second, s));
     t.pending.push_back({t.pool[t.used].first, t.pool[t.used].second, timer_full(k)});
@@ -409,21 +395,24 @@ void Batch::tflush() {
         t.pending.clear();
         t.used = 0;
     }
-    g_post_.unread = false;
-    for (auto& e : g_cg_) e.unread = false;
 }
 
 // Replay the launches of `body` on stream_ as one graph: captured on first use
 // and whenever `key` changes, then launched with hipGraphLaunch.  `body` may
 // fork onto the side streams by events but must join them back into stream_,
-// and may not synchronise, copy or allocate.
+// and may not synchronise, copy or allocate.  With the kernel timers on,
+// `body` runs eagerly instead: its timers are events on the streams.
 template <class Body>
 void Batch::run_graph(GraphEntry& e, const std::string& key, Body&& body) {
+    if (profile_) {  // with the kernel timers on, the launches go one by one
+        body();
+        return;
+    }
     if (!e.exec || e.key != key) {
         drop_graph(e);
         const int64_t l0 = launches_;
-        capture_ = &e;
         V2H_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        capture_ = &e;
         hipGraph_t g = nullptr;
         try {
             body();
@@ -447,35 +436,15 @@ void Batch::run_graph(GraphEntry& e, const std::string& key, Body&& body) {
         e.launches = launches_ - l0;
         launches_ = l0;
     }
-    if (e.unread) {  // its timer events still hold the previous replay's times
-        V2H_HIP(hipStreamSynchronize(stream_));
-        tflush();
-    }
     V2H_HIP(hipGraphLaunch(e.exec, stream_));
     launches_ += e.launches;
-    for (const auto& t : e.timers) timers_[t.kind].pending.push_back({t.a, t.b, timer_full(t.kind)});
-    e.unread = !e.timers.empty();
 }
 
 void Batch::drop_graph(GraphEntry& e) {
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     e.exec = nullptr;
-    if (!e.timers.empty()) {
-        if (stream_) (void)hipStreamSynchronize(stream_);
-        for (auto& t : timers_)  // pending reads of these events go first
-            if (!t.pending.empty()) {
-                tflush();
-                break;
-            }
-        for (const auto& t : e.timers) {
-            (void)hipEventDestroy(t.a);
-            (void)hipEventDestroy(t.b);
-        }
-    }
-    e.timers.clear();
     e.key.clear();
     e.launches = 0;
-    e.unread = false;
 }
 
 void Batch::drop_graphs() {
@@ -980,12 +949,11 @@ Batch::GraphEntry& Batch::cg_graph(int steps, bool seed) {
 }
 
 // Graph key: what a capture bakes into its launches (the plan's instance
-// selection, hence grid sizes and plan lengths), the sequence's own tag, and
-// whether the timers' event nodes are in it.
+// selection, hence grid sizes and plan lengths) and the sequence's own tag.
 std::string Batch::graph_key(const std::string& tag) const {
     std::string k(plan_sel_.begin(), plan_sel_.end());
     for (auto& ch : k) ch = ch ? '1' : '0';
-    return k + "|" + tag + (profile_ ? "|t" : "");
+    return k + "|" + tag;
 }
 
 // The eigenvector formation of the last iteration runs on the refinement

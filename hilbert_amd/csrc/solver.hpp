@@ -234,8 +234,8 @@ class Batch {
     // class on its lane, the refinement and its fallback passes, joined into
     // stream_), the eigenvector formation then running as a branch of the
     // next CG graph.  Each graph is captured once per launch plan (the set of
-    // running instances), step count and profiling state, then replayed with
-    // one hipGraphLaunch; the kernel timers are event nodes of the graphs.
+    // running instances) and step count, then replayed with one
+    // hipGraphLaunch; with the kernel timers on, the launches go one by one.
     // kGraphOff (default): the same launches one by one.  kGraphCg gives the
     // eager bits in every test.  kGraphAll is switched off (set_tuning
     // rejects it): it gave the eager bits over whole solves, but a batch
@@ -333,15 +333,9 @@ class Batch {
 
     // Captured launch sequences (set_graphs).
     struct GraphEntry {
-        std::string key;                 // plan, step count, refinement and profiling state
+        std::string key;                 // plan, step count and refinement state
         hipGraphExec_t exec = nullptr;
-        struct Timer {
-            int kind;
-            hipEvent_t a, b;
-        };
-        std::vector<Timer> timers;       // events of the graph's event-record nodes
         int64_t launches = 0;            // kernel launches per replay
-        bool unread = false;             // replayed since the timers were last read
     };
     template <class Body>
     void run_graph(GraphEntry& e, const std::string& key, Body&& body);
