@@ -632,7 +632,8 @@ def main():
     CONV[0], CONV[1] = args.e_convergence, args.r_convergence
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
-              "communication, and multi-GPU runs are not part of this build.", file=sys.stderr)
+              "communication (one process per GPU, each with its share of the batch); this benchmark "
+              "measures one GPU.", file=sys.stderr)
         sys.exit(2)
 
     from hilbert_amd import _native, configs
@@ -682,8 +683,7 @@ def main():
             # knobs change no result bit), after the first outer iterations
             from hilbert_amd import autotune
             B.solve(args.autotune)
-            tuned = autotune.tune(B, cache=os.path.join(ROOT, "gpurun_out", "tuning.json"),
-                                  log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+            tuned = autotune.tune(B, log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
         solve_with_progress(B, f"warmup {w}")
         if grads:
             gradients(B)

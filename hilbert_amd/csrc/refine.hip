@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kNT) void k_ref_etile(RefDev R, RefParams P) {
 }
 
 // Acceptance of step k from the tile maxima, one thread per block.
-__global__ void k_ref_judge(RefDev R, RefParams P, int k, int xout) {
+__global__ void k_ref_decide(RefDev R, RefParams P, int k, int xout) {
     const int bi = blockIdx.x * blockDim.x + threadIdx.x;
     if (bi >= R.nblk) return;
     RefState& rs = R.rs[bi];
@@ -520,7 +520,7 @@ void launch_ref_step(const BatchDev& B, const RefDev& R, const RefParams& P, int
     const Prod ps{xin, kSlotK, kSlotS, -1, kAT, kBN, kEpiStore, 0};
     hipLaunchKernelGGL(k_ref_gemm, dim3(g, 1), dim3(kNT), 0, s, B, R, ps, ps, (int)kGateRefining, k);
     hipLaunchKernelGGL(k_ref_etile, dim3(R.njobs), dim3(kNT), 0, s, R, P);
-    hipLaunchKernelGGL(k_ref_judge, dim3((R.nblk + 255) / 256), dim3(256), 0, s, R, P, k, xout);
+    hipLaunchKernelGGL(k_ref_decide, dim3((R.nblk + 255) / 256), dim3(256), 0, s, R, P, k, xout);
     // X_out = X + X E (E in slot K)
     const Prod pu{xin, kSlotK, xout, xin, kAN, kBN, kEpiAdd, 0};
     hipLaunchKernelGGL(k_ref_gemm, dim3(g, 1), dim3(kNT), 0, s, B, R, pu, pu, (int)kGateUpdate, k);
