@@ -10,7 +10,7 @@ they agree to about 1e-7 Eh.
 Run from the repository root:  python tests/golden/make_golden.py
 
 With --c2v16, --big and --batch64 it writes the per-instance fixtures of the
-benchmark workloads at 1e-9 / 1e-8: c2v16_oracle.json (the 64 (16e,16o) C2v
+benchmark workloads at 1e-9 / 1e-8: c2v16_oracle.json (the 128 (16e,16o) C2v
 SDPs, 1-RDMs of the first 16), big16_oracle.json (the 16 (20e-24e,20o-24o)
 D2h SDPs with 1-RDMs) and batch64_d2h_all_oracle.json (all 64 instances of
 the mixed D2h batch).
@@ -123,7 +123,7 @@ def main():
         return
     if "--c2v16" in sys.argv:
         # the (16e,16o) C2v benchmark batch: all 64 instances, 1-RDMs of 0-15
-        fixture_main("c2v16x64", configs.c2v(16, 64), lambda k: k < 16, os.path.join(here, "c2v16_oracle.json"))
+        fixture_main("c2v16x128", configs.c2v(16, 128), lambda k: k < 16, os.path.join(here, "c2v16_oracle.json"))
         return
     if "--big" in sys.argv:
         # 16 x (20e-24e, 20o-24o) D2h, naux = 3 n: all instances with 1-RDMs
