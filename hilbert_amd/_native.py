@@ -37,12 +37,12 @@ class ProblemDesc(C.Structure):
 class Tuning(C.Structure):
     """v2h_tuning: launch configuration that changes no result bit."""
     _fields_ = [("class_lane", C.c_int * 7), ("fallback_lane", C.c_int * 7), ("refine_lane", C.c_int),
-                ("cg_margin", C.c_int), ("xcd_remap", C.c_int), ("graphs", C.c_int)]
+                ("cg_margin", C.c_int), ("xcd_remap", C.c_int), ("graphs", C.c_int), ("pipeline", C.c_int)]
 
     def as_dict(self):
         return {"class_lane": list(self.class_lane), "fallback_lane": list(self.fallback_lane),
                 "refine_lane": self.refine_lane, "cg_margin": self.cg_margin, "xcd_remap": self.xcd_remap,
-                "graphs": self.graphs}
+                "graphs": self.graphs, "pipeline": self.pipeline}
 
     @classmethod
     def from_dict(cls, d):
@@ -52,6 +52,7 @@ class Tuning(C.Structure):
             t.fallback_lane[k] = int(d["fallback_lane"][k])
         t.refine_lane, t.cg_margin, t.xcd_remap = int(d["refine_lane"]), int(d["cg_margin"]), int(d["xcd_remap"])
         t.graphs = int(d.get("graphs", 0))
+        t.pipeline = int(d.get("pipeline", 0))
         return t
 
 

@@ -540,7 +540,7 @@ int v2h_batch_set_profiling(v2h_batch* b, int on) {
     return guarded([&] { B(b).set_profiling(on != 0); });
 }
 
-static_assert(sizeof(v2h_tuning) == sizeof(int) * (2 * 7 + 4), "v2h_tuning layout");
+static_assert(sizeof(v2h_tuning) == sizeof(int) * (2 * 7 + 5), "v2h_tuning layout");
 
 int v2h_batch_get_tuning(const v2h_batch* b, v2h_tuning* t) {
     return guarded([&] {
@@ -554,6 +554,7 @@ int v2h_batch_get_tuning(const v2h_batch* b, v2h_tuning* t) {
         t->cg_margin = u.cg_margin;
         t->xcd_remap = u.xcd_remap;
         t->graphs = u.graphs;
+        t->pipeline = u.pipeline;
     });
 }
 
@@ -569,6 +570,7 @@ int v2h_batch_set_tuning(v2h_batch* b, const v2h_tuning* t) {
         u.cg_margin = t->cg_margin;
         u.xcd_remap = t->xcd_remap;
         u.graphs = t->graphs;
+        u.pipeline = t->pipeline;
         B(b).set_tuning(u);
     });
 }

@@ -105,7 +105,16 @@ struct BatchDev {
     double* partials;       // per-workgroup partial dot products
     double* proj_part;      // per-block projection partials (c.x, dual residual)
     SolverOptions opt;
+    // Instances a projection launch covers: every active one (pgroup < 0), or
+    // the active ones with grp[inst] == pgroup (the pipelined solve projects
+    // one group of instances while the other group runs its CG).
+    const int* grp;
+    int pgroup;
 };
+
+__device__ __forceinline__ bool in_pgroup(const BatchDev& B, int inst) {
+    return B.pgroup < 0 || B.grp[inst] == B.pgroup;
+}
 
 // One wavefront's share of a gather launch over one instance, with what the
 // wavefront needs to start loading.  Slices: width x 64 entries from ent,

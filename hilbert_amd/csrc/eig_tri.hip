@@ -604,7 +604,7 @@ __global__ __launch_bounds__(NT, kTriWavesPerEU<NT>) void k_project_tri(BatchDev
 
     const ProjItem it = items[blockIdx.x];
     InstState& st = B.st[it.inst];
-    if (!st.active) return;
+    if (!st.active || !in_pgroup(B, it.inst)) return;
     if (route && route[it.gblock] != want) return;  // not this launch's share of the refined classes
     const InstDev I = B.inst[it.inst];
     const int d = it.d;
@@ -1086,7 +1086,7 @@ __global__ __launch_bounds__(kW) void k_project_wave(BatchDev B, const ProjItem*
 
     const ProjItem it = items[blockIdx.x];
     InstState& st = B.st[it.inst];
-    if (!st.active) return;
+    if (!st.active || !in_pgroup(B, it.inst)) return;
     const InstDev I = B.inst[it.inst];
     const int d = it.d;
     const int j = threadIdx.x;

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(kWG) void k_cg_a(BatchDev B, UnitsDev U) {
 __global__ __launch_bounds__(kWG) void k_cg_reduce(BatchDev B, UnitsDev U) {
     const int inst = blockIdx.x;
     InstState& st = B.st[inst];
-    if (!st.active || st.cg_done) return;
+    if (!st.active || st.cg_done || !in_pgroup(B, inst)) return;
     __shared__ double red[4 * 4];
     const int s = st.cg_iter;
     double tot[4];
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(kWG) void k_pre(BatchDev B, UnitsDev U) {
 __global__ __launch_bounds__(kWG) void k_pre_reduce(BatchDev B, UnitsDev U) {
     const int inst = blockIdx.x;
     InstState& st = B.st[inst];
-    if (!st.active) return;
+    if (!st.active || !in_pgroup(B, inst)) return;
     const InstDev I = B.inst[inst];
     __shared__ double red[4 * 4];
     double tot[4];
@@ -613,7 +613,7 @@ __global__ __launch_bounds__(NT) void k_project(BatchDev B, const ProjItem* item
     extern __shared__ double smem[];
     const ProjItem it = items[blockIdx.x];
     InstState& st = B.st[it.inst];
-    if (!st.active) return;
+    if (!st.active || !in_pgroup(B, it.inst)) return;
     const InstDev I = B.inst[it.inst];
     const int d = it.d, dp = d + (d & 1), m = dp >> 1;
     const int np = dp * (dp + 1) / 2;

@@ -231,6 +231,7 @@ __global__ __launch_bounds__(kNT) void k_ref_gemm(BatchDev B, RefDev R, Prod p0,
     const RefState& rs = R.rs[job.blk];
     if (!gate_ok(rs, gate, k)) return;
     const RefBlock b = R.blk[job.blk];
+    if (!in_pgroup(B, b.inst)) return;  // accepted in the other group's projection
     ref_tile(B, R, blockIdx.y == 0 ? p0 : p1, job, b, rs, As, Bs, red);
 }
 
@@ -240,6 +241,7 @@ __global__ void k_ref_state(BatchDev B, RefDev R) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= R.nblk) return;
     const RefBlock b = R.blk[i];
+    if (!in_pgroup(B, b.inst)) return;
     RefState& rs = R.rs[i];
     const bool active = B.st[b.inst].active != 0;
     const int wait = rs.wait;
@@ -376,6 +378,7 @@ __global__ void k_ref_decide(RefDev R, RefParams P, int k, int xout) {
 __global__ __launch_bounds__(kNT) void k_ref_close(BatchDev B, RefDev R) {
     __shared__ double red[4];
     const RefBlock b = R.blk[blockIdx.x];
+    if (!in_pgroup(B, b.inst)) return;
     RefState& rs = R.rs[blockIdx.x];
     if (rs.phase != 2) {  // projected by the tridiagonal fallback this iteration
         if (threadIdx.x == 0 && B.st[b.inst].active) rs.fallback += 1;
@@ -410,6 +413,7 @@ __global__ __launch_bounds__(kNT) void k_ref_form_v(BatchDev B, RefDev R) {
     __shared__ double Tm[kNB][kNB];
     __shared__ double tau[kNB];
     const RefBlock b = R.blk[blockIdx.x];
+    if (!in_pgroup(B, b.inst)) return;
     RefState& rs = R.rs[blockIdx.x];
     // blocks that took the fallback and try the refinement next iteration
     const int route = R.route[b.gblock];

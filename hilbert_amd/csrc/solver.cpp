@@ -110,6 +110,8 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     V2H_HIP(hipEventCreateWithFlags(&ev_gfork_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_joined_, hipEventDisableTiming));
     V2H_HIP(hipEventCreateWithFlags(&ev_gjoin_, hipEventDisableTiming));
+    for (auto& row : ev_pdone_)
+        for (auto& e : row) V2H_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
     std::map<std::string, int> sid_of;
     for (const auto& d : descs) {
@@ -163,6 +165,8 @@ Batch::Batch(const std::vector<ProblemDesc>& descs, const SolverOptions& opt, in
     for (const auto& s : structs_) sd.push_back(s->dev());
     d_structs_.upload(sd);
     d_inst_.upload(inst_);
+    grp_.assign(N, 0);
+    d_grp_.upload(grp_);
     state_.assign(N, InstState{});
     for (int i = 0; i < N; i++) {
         const auto& b = structs_[inst_sid_[i]]->b;
@@ -210,6 +214,9 @@ Batch::~Batch() {
     if (ev_gfork_) (void)hipEventDestroy(ev_gfork_);
     if (ev_joined_) (void)hipEventDestroy(ev_joined_);
     if (ev_gjoin_) (void)hipEventDestroy(ev_gjoin_);
+    for (auto& row : ev_pdone_)
+        for (auto& e : row)
+            if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -290,6 +297,8 @@ BatchDev Batch::dev() const {
     B.partials = partials_.p;
     B.proj_part = proj_.proj_part();
     B.opt = opt_;
+    B.grp = d_grp_.p;
+    B.pgroup = -1;
     return B;
 }
 
@@ -357,32 +366,40 @@ void Batch::tstart(int k, hipStream_t s) {
     if (!profile_) return;
     if (!s) s = stream_;
     KTimer& t = timers_[k];
-    if (t.used == t.pool.size()) {
+    if (t.free.empty()) {
         hipEvent_t a, b;
         V2H_HIP(hipEventCreate(&a));
         V2H_HIP(hipEventCreate(&b));
         t.pool.push_back({a, b});
+        t.free.push_back({a, b});
     }
-    V2H_HIP(hipEventRecord(t.pool[t.used].first, s));
+    t.open = t.free.back();
+    t.free.pop_back();
+    V2H_HIP(hipEventRecord(t.open.first, s));
 }
 
 void Batch::tstop(int k, hipStream_t s) {
     if (!profile_) return;
     if (!s) s = stream_;
     KTimer& t = timers_[k];
-    V2H_HIP(hipEventRecord(t.pool[t.used].second, s));
-    t.pending.push_back({t.pool[t.used].first, t.pool[t.used].second, timer_full(k)});
-    t.used++;
+    V2H_HIP(hipEventRecord(t.open.second, s));
+    t.pending.push_back({t.open.first, t.open.second, timer_full(k)});
 }
 
-// Called after a synchronisation of stream_: the events recorded on stream_
-// and on the lanes it joined are complete; the eigenvector formation's lane
-// is not joined, so it is waited for here.
+// Reads the timers whose launches have finished; the others (on side
+// streams that stream_ has not joined yet: the eigenvector formation, the
+// projection of a pipelined group) stay pending until a later call.
 void Batch::tflush() {
-    if (seed_pending_ && !timers_[kKindSeed].pending.empty())
-        V2H_HIP(hipStreamSynchronize(pstream_[tuning_.refine_lane - 1]));
     for (auto& t : timers_) {
-        for (const auto& r : t.pending) {
+        size_t keep = 0;
+        for (size_t j = 0; j < t.pending.size(); j++) {
+            const auto r = t.pending[j];
+            const hipError_t q = hipEventQuery(r.b);
+            if (q == hipErrorNotReady) {
+                t.pending[keep++] = r;
+                continue;
+            }
+            V2H_HIP(q);
             float ms = 0.f;
             V2H_HIP(hipEventElapsedTime(&ms, r.a, r.b));
             t.ms += ms;
@@ -391,9 +408,9 @@ void Batch::tflush() {
                 t.full_ms += ms;
                 t.full_count++;
             }
+            t.free.push_back({r.a, r.b});
         }
-        t.pending.clear();
-        t.used = 0;
+        t.pending.resize(keep);
     }
 }
 
@@ -463,6 +480,7 @@ void Batch::set_tuning(const Tuning& t) {
                                     std::to_string(kSideStreams) + ")");
     if (t.cg_margin < 0 || t.cg_margin > 64) throw std::invalid_argument("tuning: cg_margin is 0..64");
     if (t.xcd_remap != 0 && t.xcd_remap != 1) throw std::invalid_argument("tuning: xcd_remap is 0 or 1");
+    if (t.pipeline < 0) throw std::invalid_argument("tuning: pipeline is 0 (lock-step) or an instance count");
     if (t.graphs < kGraphOff || t.graphs > kGraphAll)
         throw std::invalid_argument("tuning: graphs is 0 (off) or 1 (CG chunk)");
     if (t.graphs == kGraphAll)  // see kGraphAll
@@ -483,6 +501,7 @@ void Batch::set_graphs(int mode) {
 
 void Batch::kernel_times(double* ms, int64_t* counts, double* full_ms, int64_t* full_counts) {
     V2H_HIP(hipStreamSynchronize(stream_));
+    for (hipStream_t p : pstream_) V2H_HIP(hipStreamSynchronize(p));  // timers of unjoined side work
     tflush();
     for (int k = 0; k < kKinds; k++) {
         if (ms) ms[k] = timers_[k].ms;
@@ -494,6 +513,7 @@ void Batch::kernel_times(double* ms, int64_t* counts, double* full_ms, int64_t* 
 
 void Batch::reset_kernel_times() {
     V2H_HIP(hipStreamSynchronize(stream_));
+    for (hipStream_t p : pstream_) V2H_HIP(hipStreamSynchronize(p));  // timers of unjoined side work
     tflush();
     for (auto& t : timers_) {
         t.ms = t.full_ms = 0.0;
@@ -707,6 +727,14 @@ void Batch::reset_state(int i) {
     V2H_HIP(hipStreamSynchronize(stream_));
 }
 
+// The outer iterations of one call.  Lock-step: every running instance goes
+// through pre, its CG (chunks of steps between host polls), post and the
+// projection together.  Pipelined (tuning_.pipeline): the running instances
+// form two groups that take turns on stream_; while one group runs pre, its
+// CG and post, the other group's projection runs on the side streams, so the
+// latency-bound projection hides behind the bandwidth-bound CG.  Every
+// instance goes through the same kernels on the same data in the same order
+// either way, so both schedules give the same bits.
 int Batch::solve(int maxiter, const Monitor& mon) {
     auto t0 = std::chrono::steady_clock::now();
     V2H_HIP(hipSetDevice(device_));
@@ -718,30 +746,50 @@ int Batch::solve(int maxiter, const Monitor& mon) {
     }
     sync_state_to_device();
     const BatchDev B = dev();
+    BatchDev Bg[2] = {B, B};
+    Bg[0].pgroup = 0;
+    Bg[1].pgroup = 1;
     std::vector<char> sel(count());
-    auto select = [&](bool iterating) {
+    // running instances of group g (g < 0: of the batch), as far as the host knows
+    auto in_group = [&](int i, int g) { return state_[i].active != 0 && (g < 0 || grp_[i] == g); };
+    auto running = [&](int g) {
+        int n = 0;
+        for (int i = 0; i < count(); i++) n += in_group(i, g) ? 1 : 0;
+        return n;
+    };
+    auto select = [&](int g, bool iterating) {
         for (int i = 0; i < count(); i++)
-            sel[i] = state_[i].active && (!iterating || !state_[i].cg_done) ? 1 : 0;
+            sel[i] = in_group(i, g) && (!iterating || !state_[i].cg_done) ? 1 : 0;
         set_plan(sel);
     };
     std::vector<int64_t> seen(count());
     for (int i = 0; i < count(); i++) seen[i] = state_[i].oiter;
 
-    int cg_est = 4;
-    for (const auto& st : state_) cg_est = std::max(cg_est, st.cg_last + tuning_.cg_margin);
-    auto refresh_flags = [&] {
-        all_running_ = all_iterating_ = true;
+    int cg_est[3] = {4, 4, 4};  // first CG chunk of group 0, group 1, the whole batch
+    for (const auto& st : state_) cg_est[2] = std::max(cg_est[2], st.cg_last + tuning_.cg_margin);
+    cg_est[0] = cg_est[1] = cg_est[2];
+    // timer attribution: a launch covers the whole batch only in lock-step
+    auto refresh_flags = [&](int g) {
+        all_running_ = all_iterating_ = g < 0;
         for (const auto& st : state_) {
             all_running_ &= st.active != 0;
             all_iterating_ &= st.active && !st.cg_done;
         }
     };
-    refresh_flags();
+    refresh_flags(-1);
     all_iterating_ = all_running_;
-    for (int it = 0; it < maxiter; it++) {
-        select(false);  // the instances running, as far as the host knows
+
+    // One outer iteration of group g (g < 0: the whole batch) on stream_:
+    // pre, the CG in chunks with host polls, post and the projection.
+    // Returns false when none of its instances runs any more.
+    auto iterate = [&](int g) -> bool {
+        const BatchDev& Bx = g < 0 ? B : Bg[g];
+        int& est = cg_est[g < 0 ? 2 : g];
+        refresh_flags(g);
+        all_iterating_ = all_running_;
+        select(g, false);  // the instances running, as far as the host knows
         tstart(0);
-        launch_pre(B, units_a(), stream_);
+        launch_pre(Bx, units_a(), stream_);
         tstop(0);
         launches_++;
         if (mon) {
@@ -752,36 +800,37 @@ int Batch::solve(int maxiter, const Monitor& mon) {
                     mon(i, status(i));
                 }
         }
-        int steps = 0, chunk = cg_est;
+        int steps = 0, chunk = est;
         bool any_active = true;
+        const bool graphs = g < 0 && tuning_.graphs >= kGraphCg;
         for (;;) {
             const UnitsDev UA = units_a(), UAT = units_at();
             // the eigenvector formation owed by a projection graph runs as a
             // branch beside the first chunk, joined at its end
-            const bool with_seed = tuning_.graphs >= kGraphCg && steps == 0 && seed_owed_;
+            const bool with_seed = graphs && steps == 0 && seed_owed_;
             auto cg_chunk = [&, chunk, with_seed] {
                 hipStream_t rs = pstream_[tuning_.refine_lane - 1];
                 if (with_seed) {
                     V2H_HIP(hipEventRecord(ev_gfork_, stream_));
                     V2H_HIP(hipStreamWaitEvent(rs, ev_gfork_, 0));
                     tstart(kKindSeed, rs);
-                    proj_.refine_seed(B, rs);
+                    proj_.refine_seed(Bx, rs);
                     tstop(kKindSeed, rs);
                     launches_++;
                     V2H_HIP(hipEventRecord(ev_gjoin_, rs));
                 }
                 for (int k = 0; k < chunk; k++) {
                     tstart(1);
-                    launch_cg_at(B, UAT, stream_);
+                    launch_cg_at(Bx, UAT, stream_);
                     tstop(1);
                     tstart(2);
-                    launch_cg_a(B, UA, stream_);
+                    launch_cg_a(Bx, UA, stream_);
                     tstop(2);
                 }
                 launches_ += 2 * chunk;
                 if (with_seed) V2H_HIP(hipStreamWaitEvent(stream_, ev_gjoin_, 0));
             };
-            if (tuning_.graphs >= kGraphCg && steps == 0) {  // the first chunk covers every running instance
+            if (graphs && steps == 0) {  // the first chunk covers every running instance
                 run_graph(cg_graph(chunk, with_seed), graph_key("cg" + std::to_string(chunk) + (with_seed ? "s" : "")),
                           cg_chunk);
                 seed_owed_ = false;
@@ -790,33 +839,33 @@ int Batch::solve(int maxiter, const Monitor& mon) {
             }
             steps += chunk;
             sync_state_to_host();
-            refresh_flags();
+            refresh_flags(g);
             bool pending = false;
             any_active = false;
-            for (const auto& st : state_) {
-                any_active |= st.active != 0;
-                pending |= st.active && !st.cg_done;
+            for (int i = 0; i < count(); i++) {
+                any_active |= in_group(i, g);
+                pending |= in_group(i, g) && !state_[i].cg_done;
             }
             if (!pending || steps > opt_.cg_maxiter) break;
-            select(true);  // the instances whose CG goes on
+            select(g, true);  // the instances whose CG goes on
             chunk = 4;
         }
-        if (!any_active) break;
-        cg_est = 4;
-        for (const auto& st : state_)
-            if (st.active) cg_est = std::max(cg_est, st.cg_last + tuning_.cg_margin);
+        if (!any_active) return false;
+        est = 4;
+        for (int i = 0; i < count(); i++)
+            if (in_group(i, g)) est = std::max(est, state_[i].cg_last + tuning_.cg_margin);
         all_iterating_ = all_running_;  // the next CG solve starts for every running instance
-        select(false);
+        select(g, false);
         auto post_projection = [&](bool seed) {
             tstart(3);
-            launch_post(B, units_at(), stream_);
+            launch_post(Bx, units_at(), stream_);
             tstop(3);
             launches_++;
             // the size classes are independent: largest first, on the streams
-            // of tuning_.class_lane, joined back into stream_
-            launch_projection(B, seed);
+            // of tuning_.class_lane, joined back into stream_ (lock-step)
+            launch_projection(Bx, seed, g);
         };
-        if (tuning_.graphs == kGraphAll) {
+        if (g < 0 && tuning_.graphs == kGraphAll) {
             // the graph joins every lane it forks, so the eigenvector formation
             // (off the critical path) goes into the next CG graph
             run_graph(g_post_, graph_key(proj_.refine_on() ? "post_r" : "post"), [&] { post_projection(false); });
@@ -825,8 +874,56 @@ int Batch::solve(int maxiter, const Monitor& mon) {
             post_projection(true);
         }
         V2H_HIP(hipGetLastError());
+        return true;
+    };
+
+    // two halves of the running instances, in batch order (no projection may
+    // be in flight: set_groups waits for every stream)
+    auto split = [&] {
+        const int n = running(-1);
+        std::vector<int> grp(count(), 0);
+        int k = 0;
+        for (int i = 0; i < count(); i++)
+            if (state_[i].active) grp[i] = 2 * k++ >= n ? 1 : 0;
+        set_groups(grp);
+    };
+    const bool pipe = tuning_.pipeline > 0 && tuning_.graphs == kGraphOff;
+    bool piped = pipe && running(-1) >= tuning_.pipeline;
+    if (piped) split();
+    int left[2] = {maxiter, maxiter};  // outer iterations left to each group, or (left[0]) to the batch
+    for (int turn = 0;;) {
+        if (!piped) {
+            if (left[0] <= 0 || !iterate(-1)) break;
+            left[0]--;
+            continue;
+        }
+        const int n0 = running(0), n1 = running(1);
+        const bool live0 = n0 > 0 && left[0] > 0, live1 = n1 > 0 && left[1] > 0;
+        if (!live0 && !live1) break;
+        if (n0 + n1 < tuning_.pipeline) {  // too few instances to overlap: lock-step from here
+            wait_projection(0);
+            wait_projection(1);
+            piped = false;
+            left[0] = std::max(left[0], left[1]);
+            continue;
+        }
+        if (std::max(n0, n1) - std::min(n0, n1) > 1 && 3 * std::min(n0, n1) < 2 * std::max(n0, n1)) {
+            // unbalanced: split the running instances again
+            wait_projection(0);
+            wait_projection(1);
+            split();
+            left[0] = left[1] = std::max(left[0], left[1]);
+            continue;
+        }
+        const int g = turn;
+        turn ^= 1;
+        if (!(g == 0 ? live0 : live1)) continue;
+        wait_projection(g);  // the group's last projection
+        if (iterate(g)) left[g]--;
     }
-    select(false);
+    wait_projection(0);
+    wait_projection(1);
+    select(-1, false);
     tstart(0);
     launch_pre(B, units_a(), stream_);  // book-keeping of the last iteration
     tstop(0);
@@ -859,7 +956,7 @@ int Batch::solve(int maxiter, const Monitor& mon) {
 // lane that writes x and z; the eigenvectors of the fallback blocks (needed
 // only by the next iteration's refinement) are formed on the refinement lane
 // after the fallback lanes, off the critical path.
-void Batch::launch_projection(const BatchDev& B, bool seed) {
+void Batch::launch_projection(const BatchDev& B, bool seed, int g) {
     const int rl = tuning_.refine_lane;
     const ProjEvents& E = capture_ ? ev_capture_ : ev_eager_;
     V2H_HIP(hipEventRecord(E.post, stream_));
@@ -877,10 +974,15 @@ void Batch::launch_projection(const BatchDev& B, bool seed) {
         launches_ += proj_.refine_launches();
         V2H_HIP(hipEventRecord(E.ref, rs));
     }
+    // a pipelined group's projection stays off stream_, which runs the other
+    // group's CG: the classes of lane 0 go to a side stream other than the
+    // refinement's
+    const int lane0 = g < 0 ? 0 : (rl == 2 ? 3 : 2);
+    auto lane_of = [&](int ln) { return ln == 0 ? lane0 : ln; };
     auto lane_stream = [&](int ln) { return ln == 0 ? stream_ : pstream_[ln - 1]; };
     for (int k = kProjClasses - 1; k >= 0; k--) {
         if (!proj_.has(k) || (refine && Projector::refined_class(k))) continue;
-        const int ln = tuning_.class_lane[k];
+        const int ln = lane_of(tuning_.class_lane[k]);
         hipStream_t ps = lane_stream(ln);
         if (ln > 0 && !used[ln]) V2H_HIP(hipStreamWaitEvent(ps, E.post, 0));
         used[ln] = true;
@@ -894,7 +996,7 @@ void Batch::launch_projection(const BatchDev& B, bool seed) {
             bool waited[kSideStreams + 1] = {};
             for (int k = kProjClasses - 1; k >= 0; k--) {
                 if (!proj_.has(k) || !Projector::refined_class(k)) continue;
-                const int ln = tuning_.fallback_lane[k];
+                const int ln = lane_of(tuning_.fallback_lane[k]);
                 hipStream_t ps = lane_stream(ln);
                 if (!waited[ln]) V2H_HIP(hipStreamWaitEvent(ps, pass == 0 ? E.state : E.ref, 0));
                 waited[ln] = used[ln] = true;
@@ -904,6 +1006,24 @@ void Batch::launch_projection(const BatchDev& B, bool seed) {
                 launches_++;
             }
         }
+    }
+    if (g >= 0) {
+        // the group's next pre waits for these events (wait_projection); the
+        // eigenvector formation, after every fallback launch of the group
+        for (int ln = 1; ln <= kSideStreams; ln++) {
+            pdone_used_[g][ln - 1] = used[ln];
+            if (used[ln]) V2H_HIP(hipEventRecord(ev_pdone_[g][ln - 1], pstream_[ln - 1]));
+        }
+        if (refine && seed) {
+            for (int ln = 1; ln <= kSideStreams; ln++)
+                if (used[ln] && ln != rl) V2H_HIP(hipStreamWaitEvent(rs, ev_pdone_[g][ln - 1], 0));
+            tstart(kKindSeed, rs);
+            proj_.refine_seed(B, rs);
+            tstop(kKindSeed, rs);
+            launches_++;
+            seed_pending_ = true;
+        }
+        return;
     }
     for (int ln = 1; ln <= kSideStreams; ln++)
         if (used[ln]) {
@@ -920,6 +1040,23 @@ void Batch::launch_projection(const BatchDev& B, bool seed) {
         launches_++;
         seed_pending_ = true;
     }
+}
+
+void Batch::wait_projection(int g) {
+    for (int k = 0; k < kSideStreams; k++)
+        if (pdone_used_[g][k]) {
+            V2H_HIP(hipStreamWaitEvent(stream_, ev_pdone_[g][k], 0));
+            pdone_used_[g][k] = false;
+        }
+}
+
+void Batch::set_groups(const std::vector<int>& grp) {
+    if (grp == grp_) return;
+    // the projection kernels and the eigenvector formation read the groups
+    V2H_HIP(hipStreamSynchronize(stream_));
+    for (hipStream_t p : pstream_) V2H_HIP(hipStreamSynchronize(p));
+    grp_ = grp;
+    V2H_HIP(hipMemcpy(d_grp_.p, grp_.data(), sizeof(int) * grp_.size(), hipMemcpyHostToDevice));
 }
 
 // The eigenvector formation after a replayed projection graph, which has

@@ -213,6 +213,12 @@ class Batch {
         // launch order (0, default: measured faster on the (16e,16o) batches)
         int xcd_remap = 0;
         int graphs = 0;       // kGraphOff / kGraphCg / kGraphAll (set_graphs)
+        // Pipelined solve: while at least this many instances run, they form
+        // two groups that take turns on stream_ (pre, CG, post), the
+        // projection of one group running on the side streams beside the CG
+        // of the other; below it, and with 0 or with graphs, the whole batch
+        // iterates in lock-step.
+        int pipeline = 0;
     };
     void set_tuning(const Tuning& t);
     const Tuning& tuning() const { return tuning_; }
@@ -265,7 +271,12 @@ class Batch {
     void sync_state_to_host();
     void sync_state_to_device();
     void refresh_norms(int i);
-    void launch_projection(const BatchDev& B, bool seed);
+    // g < 0: the lock-step projection, joined into stream_; g = 0, 1: the
+    // projection of group g on the side streams only, its completion recorded
+    // in ev_pdone_[g] (the next pre of the group waits for it).
+    void launch_projection(const BatchDev& B, bool seed, int g = -1);
+    void wait_projection(int g);    // stream_ waits for group g's last projection
+    void set_groups(const std::vector<int>& grp);  // upload the groups (projection idle)
     void launch_seed(const BatchDev& B);
     void join_seed();
     DfState& df_state(int i);
@@ -309,17 +320,24 @@ class Batch {
         hipEvent_t post = nullptr, state = nullptr, ref = nullptr, proj[kSideStreams] = {};
     };
     ProjEvents ev_eager_, ev_capture_;
+    // pipelined solve: group of every instance (host and device copies), the
+    // completion of each group's projection per side stream
+    std::vector<int> grp_;
+    DBuf<int> d_grp_;
+    hipEvent_t ev_pdone_[2][kSideStreams] = {};
+    bool pdone_used_[2][kSideStreams] = {};
     hipEvent_t ev_seed_ = nullptr;
     bool seed_pending_ = false;
 
     struct KTimer {
-        std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;  // events of eager launches, reused
-        size_t used = 0;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;  // every event pair of this kind
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> free;  // pairs not in flight
+        std::pair<hipEvent_t, hipEvent_t> open{};            // between tstart and tstop
         struct Pending {
             hipEvent_t a, b;
             char full;
         };
-        std::vector<Pending> pending;  // recorded, not yet read
+        std::vector<Pending> pending;  // recorded, not yet read (may still be in flight)
         double ms = 0.0, full_ms = 0.0;
         int64_t count = 0, full_count = 0;
     };

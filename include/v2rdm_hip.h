@@ -311,6 +311,12 @@ typedef struct {
                            * (default): in launch order */
     int graphs;           /* hipGraph replay: 0 off, 1 first CG chunk
                            * (v2h_batch_set_graphs) */
+    int pipeline;         /* > 0: while at least this many instances run,
+                           * two groups of them take turns on the main stream
+                           * (pre, CG, post) and each group's projection runs
+                           * on the side streams beside the other group's CG;
+                           * 0: lock-step (every instance together); ignored
+                           * with graphs */
 } v2h_tuning;
 int v2h_batch_get_tuning(const v2h_batch *b, v2h_tuning *t);
 int v2h_batch_set_tuning(v2h_batch *b, const v2h_tuning *t);

@@ -8,7 +8,7 @@ import numpy as np
 from hilbert_amd import autotune
 
 BASE = {"class_lane": [0, 0, 2, 3, 1, 1, 2], "fallback_lane": [0, 0, 2, 3, 3, 2, 2], "refine_lane": 1,
-        "cg_margin": 2, "xcd_remap": 0, "graphs": 0}
+        "cg_margin": 2, "xcd_remap": 0, "graphs": 0, "pipeline": 0}
 
 
 class _FakeBatch:
