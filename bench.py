@@ -1,7 +1,9 @@
 """Benchmark: converged v2RDM SDPs per second at (16e,16o), batched on one MI355X.
 
 A step is one batched solve, from hilbert's random guess to convergence
-(E_CONVERGENCE 5e-8, R_CONVERGENCE 5e-7 by default), of B independent (16e,16o) DQG SDPs
+(E_CONVERGENCE 5e-8, R_CONVERGENCE 5e-7 by default; each outer iteration's CG
+stops at a relative residual of 0.007 x max(primal, dual error), --cg-factor),
+of B independent (16e,16o) DQG SDPs
 (B = 128 by default: a batch of 128 keeps the projection's latency-bound
 launches busier than 64 and solves about 12 % more SDPs per second; 192 adds
 3 % more) with C2v irrep labels drawn at random and synthetic molecular-like
@@ -86,12 +88,14 @@ def project_flops(dims):
 
 
 MU_FREQ = [1000]   # MU_UPDATE_FREQUENCY (hilbert's default 1000)
+CG_FACTOR = [0.007]  # CG stopping rule: relative residual below this x max(eps_p, eps_d)
 
 
 def make_batch(items, maxiter):
     from hilbert_amd.batch import Batch, options
     B = Batch([p for p, _, _ in items],
               options(e_convergence=CONV[0], r_convergence=CONV[1], maxiter=maxiter, mu_update_frequency=MU_FREQ[0]))
+    B.set_cg_tolerance_factor(CG_FACTOR[0])
     for i, (_, h, g) in enumerate(items):
         B.set_integrals(i, h, g)
     return B
@@ -614,6 +618,9 @@ def main():
                          "first warm-up solve (0: keep the defaults)")
     ap.add_argument("--e-convergence", type=float, default=5e-8, help="E_CONVERGENCE of the timed solves")
     ap.add_argument("--r-convergence", type=float, default=5e-7, help="R_CONVERGENCE of the timed solves")
+    ap.add_argument("--cg-factor", type=float, default=0.007,
+                    help="CG stopping rule of the batched and single solves: relative residual below this x the "
+                         "larger of the primal and dual errors (the library's default, the oracle's, is 0.003)")
     ap.add_argument("--mu-update-frequency", type=int, default=1000,
                     help="sdp workload: MU_UPDATE_FREQUENCY, outer iterations between mu rebalancings "
                          "(hilbert's default 1000)")
@@ -632,6 +639,7 @@ def main():
     args = ap.parse_args()
     PROGRESS_CHUNK[0] = max(1, args.chunk)
     MU_FREQ[0] = args.mu_update_frequency
+    CG_FACTOR[0] = args.cg_factor
     CONV[0], CONV[1] = args.e_convergence, args.r_convergence
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: batches of independent SDPs shard across GPUs without "
@@ -795,7 +803,7 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": wl, "batch": args.batch, **cfg, "projection": args.projection,
                    "e_convergence": CONV[0], "r_convergence": CONV[1], "positivity": args.positivity,
-                   "mu_update_frequency": MU_FREQ[0],
+                   "mu_update_frequency": MU_FREQ[0], "cg_tolerance_factor": CG_FACTOR[0],
                    "constrain_d3": bool(args.d3), "constrain_spin": True, "constrain_sz": True},
         "converged": sum(conv), "attempted": args.batch * args.steps,
         "converged_check": {"stopping_test": sum(conv_solver), "within_1e-6_of_oracle": sum(conv),

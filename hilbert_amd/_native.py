@@ -96,6 +96,7 @@ SIGNATURES = {
     "v2h_batch_destroy": (C.c_int, [_vp]),
     "v2h_batch_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "v2h_batch_set_options": (C.c_int, [_vp, C.POINTER(SdpOptions)]),
+    "v2h_batch_set_cg_tolerance_factor": (C.c_int, [_vp, C.c_double]),
     "v2h_batch_sizes": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int)]),
     "v2h_batch_block_dims": (C.c_int, [_vp, C.c_int, _ip]),

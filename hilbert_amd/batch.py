@@ -78,6 +78,14 @@ class Batch:
         self.opts = opts
         N.check(N.lib().v2h_batch_set_options(self._h, C.byref(opts)))
 
+    CG_TOLERANCE_FACTOR = 0.003  # the default, the oracle's
+
+    def set_cg_tolerance_factor(self, factor: float):
+        """CG stopping rule of each outer iteration: relative residual below
+        factor x max(primal error, dual error), capped at 1e-2, at least
+        CG_CONVERGENCE.  Changes the trajectory, not the optimum."""
+        N.check(N.lib().v2h_batch_set_cg_tolerance_factor(self._h, float(factor)))
+
     def set_integrals(self, i, h, g):
         n = self.problems[i].nmo
         h = np.ascontiguousarray(h, dtype=np.float64).reshape(n * n)

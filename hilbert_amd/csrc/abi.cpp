@@ -181,6 +181,10 @@ int v2h_batch_set_options(v2h_batch* b, const v2h_sdp_options* opt) {
     });
 }
 
+int v2h_batch_set_cg_tolerance_factor(v2h_batch* b, double factor) {
+    return guarded([&] { B(b).set_cg_tolerance_factor(factor); });
+}
+
 int v2h_batch_sizes(const v2h_batch* b, int inst, int64_t* np, int64_t* nd, int* nb) {
     return guarded([&] {
         check_inst(B(b), inst);

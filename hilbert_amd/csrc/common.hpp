@@ -83,6 +83,7 @@ constexpr double kCgTolCap = 1e-2;
 struct SolverOptions {
     double e_convergence, r_convergence, cg_convergence, mu0;
     int cg_maxiter, maxiter, mu_update_frequency;
+    double cg_tol_factor = kCgTolFactor;  // the factor of the CG stopping rule above
 };
 
 // CG state of one dual row, stored together so that a gather of the row
@@ -105,15 +106,17 @@ struct BatchDev {
     double* partials;       // per-workgroup partial dot products
     double* proj_part;      // per-block projection partials (c.x, dual residual)
     SolverOptions opt;
-    // Instances a projection launch covers: every active one (pgroup < 0), or
-    // the active ones with grp[inst] == pgroup (the pipelined solve projects
-    // one group of instances while the other group runs its CG).
-    const int* grp;
-    int pgroup;
+    // Instances a projection launch covers: every active one (pgroup < 0, the
+    // default of a value-initialised BatchDev), or the active ones with
+    // grp[inst] == pgroup (the pipelined solve projects one group of
+    // instances while the other group runs its CG).
+    const int* grp = nullptr;
+    int pgroup = -1;
 };
+static_assert(BatchDev{}.pgroup < 0 && BatchDev{}.grp == nullptr, "a value-initialised BatchDev covers every instance");
 
 __device__ __forceinline__ bool in_pgroup(const BatchDev& B, int inst) {
-    return B.pgroup < 0 || B.grp[inst] == B.pgroup;
+    return B.pgroup < 0 || (B.grp != nullptr && B.grp[inst] == B.pgroup);
 }
 
 // One wavefront's share of a gather launch over one instance, with what the

@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kWG) void k_pre_reduce(BatchDev B, UnitsDev U) {
         }
         st.pending = 0;
         double e = st.oiter == 0 ? 1.0 : fmax(st.eps_p, st.eps_d);
-        double tol = kCgTolFactor * e;
+        double tol = o.cg_tol_factor * e;
         if (tol > kCgTolCap) tol = kCgTolCap;
         if (tol < o.cg_convergence) tol = o.cg_convergence;
         double mu = st.mu;

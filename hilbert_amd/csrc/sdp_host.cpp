@@ -62,6 +62,8 @@ void CallbackSdp::project(double* x) {
     B.v1 = d_v1_.p;
     B.c = d_c_.p;
     B.proj_part = proj_.proj_part();
+    B.grp = nullptr;
+    B.pgroup = -1;  // one instance: every projection launch covers it
     // the warm-started refinement first; the tridiagonal classes take the
     // blocks it did not accept, whose eigenvectors then seed the next call
     const bool refine = proj_.refine_on();

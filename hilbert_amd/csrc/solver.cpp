@@ -297,6 +297,7 @@ BatchDev Batch::dev() const {
     B.partials = partials_.p;
     B.proj_part = proj_.proj_part();
     B.opt = opt_;
+    B.opt.cg_tol_factor = cg_factor_;
     B.grp = d_grp_.p;
     B.pgroup = -1;
     return B;
@@ -1174,6 +1175,12 @@ void Batch::set_slack(int i, const double* z) {
     for (int64_t k = 0; k < n_primal(i); k++) c[k] -= z[k];
     h2d(v1_.p + I.poff, c.data(), n_primal(i), stream_);
 }
+void Batch::set_cg_tolerance_factor(double f) {
+    if (!(f > 0.0 && f <= 1.0)) throw std::invalid_argument("CG tolerance factor must be in (0, 1]");
+    cg_factor_ = f;
+    drop_graphs();  // the options are kernel arguments
+}
+
 void Batch::set_options(const SolverOptions& o) {
     const bool tighter = o.e_convergence < opt_.e_convergence || o.r_convergence < opt_.r_convergence;
     opt_ = o;

@@ -117,6 +117,11 @@ class Batch {
     // instances that converged under the old ones, so that the next solve
     // continues them from their state to the new thresholds.
     void set_options(const SolverOptions& o);
+    // Factor f of the CG stopping rule: relative residual below f x the
+    // larger of the last primal and dual errors (capped at 1e-2, at least
+    // CG_CONVERGENCE); default kCgTolFactor.
+    void set_cg_tolerance_factor(double f);
+    double cg_tolerance_factor() const { return cg_factor_; }
 
     // Hamiltonian in the user's orbital order: h[n*n], g[n^4] chemists' (pq|rs).
     void set_integrals(int i, const double* h, const double* g);
@@ -290,6 +295,7 @@ class Batch {
     BatchDev dev() const;
 
     SolverOptions opt_;
+    double cg_factor_ = kCgTolFactor;
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     std::vector<std::shared_ptr<Structure>> structs_;

@@ -102,6 +102,14 @@ int v2h_batch_count(const v2h_batch *b, int *count);
  * instances that had converged, so that the next v2h_batch_solve continues
  * them from their current x, y, z and mu to the new thresholds. */
 int v2h_batch_set_options(v2h_batch *b, const v2h_sdp_options *opt);
+/* Stopping rule of the CG solve inside each outer iteration: relative
+ * residual ||r|| / ||rhs|| below factor x max(primal error, dual error) of the
+ * previous outer iteration, capped at 1e-2 and at least CG_CONVERGENCE
+ * (default factor 0.003, the oracle's).  libsdp's own schedule for this solve
+ * (BPSDPSolver, not vendored; called at v2rdm_solver.cc:1340) is not known;
+ * a larger factor solves each CG less tightly and changes the trajectory,
+ * not the optimum.  factor in (0, 1]. */
+int v2h_batch_set_cg_tolerance_factor(v2h_batch *b, double factor);
 
 /* Sizes (determine_n_primal / determine_n_dual) and the PSD block list
  * (dimensions_, v2rdm_solver.cc:3729-3846) of one instance. */
