@@ -153,16 +153,16 @@ def roofline(B, kt, counts, workload, dom=None, proj=None):
     traffic = load_traffic(dom, workload)
     if dom == "project_refine":
         # one "launch" = the refinement launches of one outer iteration; its
-        # work = the flops of the d x d x d products it ran (counted per
-        # block on the device: 2 d^3 for K and X + XE per step and the final
-        # X diag(l+) X^T, d^3 for the symmetric G and S), over the timed region
+        # work = the flops of the d x d x d products it ran (2 d^3 each,
+        # counted per block on the device: K, G, S and X + XE per step, the
+        # final X diag(l+) X^T), over the timed region
         work = proj["flops"] / max(cnt, 1)
         achieved = work / (per_launch_ms * 1e-3) / 1e12
         return dom, {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "k_ref_* (refinement)",
                      "launches": cnt, "per_launch_ms": per_launch_ms, "work_per_launch": work,
-                     "work": "FP64 flops of the refinement's d x d x d products (device counters: 2 d^3 "
-                             "per product, d^3 for the symmetric G and S), per outer iteration"}
+                     "work": "2 d^3 FP64 flops per d x d x d product of the refinement (device counters), "
+                             "per outer iteration"}
     if dom.startswith("project"):
         lo, hi = [(a, b) for k, a, b in PROJ_CLASSES if k == dom][0]
         work = sum(executions(dom, o, it) * project_flops([d for d in B.block_dims(i) if lo <= d <= hi])

@@ -30,11 +30,9 @@ enum Epi : int { kEpiStore = 0, kEpiAdd = 1, kEpiFinal = 2 };
 enum Gate : int { kGateRefining = 0, kGateUpdate = 1, kGateFinal = 2 };
 
 // One product C = [D +] op(A) op(B) of every block, operands named by slot.
-// sym: C is symmetric (G = X^T X, S = X^T U X): only the tiles on and above
-// the diagonal are computed, each stored with its mirror image.
 struct Prod {
     int a, b, c, dadd;  // slots; a < 0: the accepted X of the block
-    int am, bm, epi, sym;
+    int am, bm, epi, pad;
 };
 
 // NaN-propagating maximum (a NaN correction must reject the block).
@@ -169,14 +167,12 @@ __device__ void ref_tile(const BatchDev& B, const RefDev& R, const Prod& p, cons
     if (p.epi != kEpiFinal) {
         double* C = slot(R, b, p.c);
         const double* D = p.epi == kEpiAdd ? slot(R, b, p.dadd >= 0 ? p.dadd : rs.xfinal) : nullptr;
-        const bool mirror = p.sym && job.ti != job.tj;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int m = m0 + wm + 4 * r + lk, n = n0 + wn + li;
             if (m < d && n < d) {
                 const int64_t e = (int64_t)m * d + n;
                 C[e] = D ? D[e] + acc[r] : acc[r];
-                if (mirror) C[(int64_t)n * d + m] = acc[r];
             }
         }
         return;
@@ -236,9 +232,7 @@ __global__ __launch_bounds__(kNT) void k_ref_gemm(BatchDev B, RefDev R, Prod p0,
     if (!gate_ok(rs, gate, k)) return;
     const RefBlock b = R.blk[job.blk];
     if (!in_pgroup(B, b.inst)) return;  // accepted in the other group's projection
-    const Prod& p = blockIdx.y == 0 ? p0 : p1;
-    if (p.sym && job.ti > job.tj) return;  // the mirror of a tile above the diagonal
-    ref_tile(B, R, p, job, b, rs, As, Bs, red);
+    ref_tile(B, R, blockIdx.y == 0 ? p0 : p1, job, b, rs, As, Bs, red);
 }
 
 // ---------------------------------------------------------------- per block
@@ -370,7 +364,7 @@ __global__ void k_ref_decide(RefDev R, RefParams P, int k, int xout) {
         else
             rs.rej_c += 1;
     }
-    rs.flops += (reject ? 2 : 3) * d3;  // K, the symmetric G and S (half each), X + XE unless rejected
+    rs.flops += (reject ? 3 : 4) * d3;  // a rejected block skips X + XE
     if (reject) {
         rs.phase = 3;
         const int bo = min(2 * rs.backoff + 2, kMaxBackoff);
@@ -523,11 +517,11 @@ void launch_ref_step(const BatchDev& B, const RefDev& R, const RefParams& P, int
     const int xout = k % 2 == 1 ? kSlotXa : kSlotXb;
     // K = U X, G = X^T X
     const Prod pk{kSlotU, xin, kSlotK, -1, kAN, kBN, kEpiStore, 0};
-    const Prod pg{xin, xin, kSlotG, -1, kAT, kBN, kEpiStore, 1};
+    const Prod pg{xin, xin, kSlotG, -1, kAT, kBN, kEpiStore, 0};
     const int g = gemm_grid(R);
     hipLaunchKernelGGL(k_ref_gemm, dim3(g, 2), dim3(kNT), 0, s, B, R, pk, pg, (int)kGateRefining, k);
     // S = X^T K
-    const Prod ps{xin, kSlotK, kSlotS, -1, kAT, kBN, kEpiStore, 1};
+    const Prod ps{xin, kSlotK, kSlotS, -1, kAT, kBN, kEpiStore, 0};
     hipLaunchKernelGGL(k_ref_gemm, dim3(g, 1), dim3(kNT), 0, s, B, R, ps, ps, (int)kGateRefining, k);
     hipLaunchKernelGGL(k_ref_etile, dim3(R.njobs), dim3(kNT), 0, s, R, P);
     hipLaunchKernelGGL(k_ref_decide, dim3((R.nblk + 255) / 256), dim3(256), 0, s, R, P, k, xout);

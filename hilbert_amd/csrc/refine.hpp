@@ -56,9 +56,8 @@ struct RefState {
     // rejections: diverging or non-finite corrections, correction above the
     // acceptance threshold after the last step, clusters left coupled
     int64_t rej_div, rej_e, rej_c;
-    // FP64 flops of the block's d x d x d products so far: 2 d^3 for K and
-    // X + XE of each step and the final X diag(l+) X^T, d^3 for the
-    // symmetric G and S (their tiles above the diagonal, mirrored)
+    // FP64 flops of the block's d x d x d products so far (2 d^3 each: K, G,
+    // S and X + XE per step, the final X diag(l+) X^T)
     int64_t flops;
     // After a rejection the block stays on the fallback for `wait` outer
     // iterations (doubling with consecutive rejections, up to kMaxBackoff);

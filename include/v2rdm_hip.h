@@ -353,9 +353,7 @@ int v2h_batch_set_projection(v2h_batch *b, int mode);
  * projections done by the refinement, by the tridiagonal fallback, refinements
  * rejected for a diverging or non-finite correction, for a correction still
  * above accept_e after max_steps steps, for clusters left coupled, FP64 flops
- * of the refinement's d x d x d products (2 d^3 each; d^3 for the symmetric
- * G = X^T X and S = X^T U X, of which only one triangle of tiles is
- * formed)]; stats[6]. */
+ * of the refinement's d x d x d products (2 d^3 each)]; stats[6]. */
 int v2h_batch_projection_stats(v2h_batch *b, int64_t *stats);
 /* Thresholds of the refinement: eigenvalue estimates closer than cluster_rel
  * x max|l| form a cluster (default 1e-8); a step is accepted when its largest
