@@ -2,7 +2,8 @@
 
 A step is one batched solve, from hilbert's random guess to convergence
 (E_CONVERGENCE 5e-8, R_CONVERGENCE 5e-7 by default; each outer iteration's CG
-stops at a relative residual of 0.007 x max(primal, dual error), --cg-factor),
+stops at a relative residual of 0.007 x max(primal, dual error), --cg-factor;
+mu is rebalanced every 250 outer iterations, --mu-update-frequency),
 of B independent (16e,16o) DQG SDPs
 (B = 128 by default: a batch of 128 keeps the projection's latency-bound
 launches busier than 64 and solves about 12 % more SDPs per second; 192 adds
@@ -87,7 +88,7 @@ def project_flops(dims):
     return sum(16.0 / 3.0 * float(d) ** 3 for d in dims if d > 0)
 
 
-MU_FREQ = [1000]   # MU_UPDATE_FREQUENCY (hilbert's default 1000)
+MU_FREQ = [250]    # MU_UPDATE_FREQUENCY (hilbert's default 1000)
 CG_FACTOR = [0.007]  # CG stopping rule: relative residual below this x max(eps_p, eps_d)
 
 
@@ -621,9 +622,9 @@ def main():
     ap.add_argument("--cg-factor", type=float, default=0.007,
                     help="CG stopping rule of the batched and single solves: relative residual below this x the "
                          "larger of the primal and dual errors (the library's default, the oracle's, is 0.003)")
-    ap.add_argument("--mu-update-frequency", type=int, default=1000,
-                    help="sdp workload: MU_UPDATE_FREQUENCY, outer iterations between mu rebalancings "
-                         "(hilbert's default 1000)")
+    ap.add_argument("--mu-update-frequency", type=int, default=250,
+                    help="MU_UPDATE_FREQUENCY of the batched and single solves: outer iterations between mu "
+                         "rebalancings (hilbert's default 1000)")
     ap.add_argument("--casscf", type=int, default=1,
                     help="orbopt workload: also run the whole v2RDM-CASSCF loop once (0 = skip)")
     ap.add_argument("--workload", choices=("sdp", "d2h64", "big16", "df", "fcidump", "orbopt"), default="sdp",

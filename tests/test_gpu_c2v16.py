@@ -6,7 +6,7 @@ and naux 48, against the oracle's per-instance optimum
   within 1e-6 Eh of the oracle's energy and its spin-summed 1-RDM within 1e-5
   of the oracle's;
 * the same 16 at the benchmark's settings (thresholds 5e-8 / 5e-7, CG
-  tolerance factor 0.007): every instance stops inside the band those
+  tolerance factor 0.007, MU_UPDATE_FREQUENCY 250): every instance stops inside the band those
   thresholds allow (5e-6 Eh), and most within 1e-6 Eh (the benchmark counts
   only those);
 * the CG tolerance factor is validated.
@@ -25,8 +25,9 @@ pytestmark = pytest.mark.gpu
 FIX = os.path.join(os.path.dirname(__file__), "golden", "c2v16_oracle.json")
 
 
-def _solve(items, e, r, cg_factor=None):
-    B = Batch([p for p, _, _ in items], options(e_convergence=e, r_convergence=r, maxiter=100000))
+def _solve(items, e, r, cg_factor=None, mu_update_frequency=1000):
+    B = Batch([p for p, _, _ in items], options(e_convergence=e, r_convergence=r, maxiter=100000,
+                                                mu_update_frequency=mu_update_frequency))
     if cg_factor is not None:
         B.set_cg_tolerance_factor(cg_factor)
     for i, (_, h, g) in enumerate(items):
@@ -55,9 +56,9 @@ def test_c2v16_batch_reaches_the_oracle_optimum():
 def test_c2v16_batch_at_the_benchmark_thresholds():
     items = configs.c2v(16, 16)
     ref = {o["index"]: o for o in json.load(open(FIX))["instances"]}
-    res = _solve(items, 5e-8, 5e-7, cg_factor=0.007)
+    res = _solve(items, 5e-8, 5e-7, cg_factor=0.007, mu_update_frequency=250)
     dE = np.array([abs(st["energy_primal"] - ref[i]["energy_primal"]) for i, (st, _) in enumerate(res)])
-    print(f"16 x (16e,16o) at 5e-8 / 5e-7, CG factor 0.007: {int((dE <= 1e-6).sum())} of 16 within 1e-6 Eh, "
+    print(f"16 x (16e,16o) at 5e-8 / 5e-7, CG factor 0.007, mu every 250: {int((dE <= 1e-6).sum())} of 16 within 1e-6 Eh, "
           f"max |dE| {dE.max():.2e}", flush=True)
     assert all(st["converged"] for st, _ in res)
     assert dE.max() <= 5e-6
