@@ -9,8 +9,8 @@ Knobs, searched one at a time (coordinate descent, in this order), each
 candidate timed over `window` outer iterations in `rounds` interleaved
 rounds (median):
 
-* ``graphs``: hipGraph replay of the first CG chunk of each outer iteration,
-  or eager launches;
+* ``graphs``: eager launches, hipGraph replay of the first CG chunk of each
+  outer iteration, or of that chunk and of k_post with the whole projection;
 * ``xcd_remap``: gather workgroups dealt to the 8 XCDs in contiguous ranges
   of one instance (L2 locality) or in launch order;
 * ``cg_margin``: the first CG chunk of an outer iteration is the largest CG
@@ -92,7 +92,7 @@ def tune(B, window: int = 20, rounds: int = 3, cache: str | None = None, log=Non
     Stops early, keeping the best so far, if the batch converges."""
     best = B.tuning()
     report = {}
-    for knob, values in (("graphs", (0, 1)), ("xcd_remap", (1, 0)), ("cg_margin", CG_MARGINS),
+    for knob, values in (("graphs", (0, 1, 2)), ("xcd_remap", (1, 0)), ("cg_margin", CG_MARGINS),
                          ("lanes", tuple(LANE_PRESETS))):
         cands = [_with(best, knob, v) for v in values]
         times = [[] for _ in cands]

@@ -337,9 +337,9 @@ class Batch:
 
     def set_graphs(self, mode=GRAPHS_CG):
         """hipGraph replay: GRAPHS_CG the first CG chunk of every outer
-        iteration; GRAPHS_OFF (default) the same launches one by one, same
-        bits.  GRAPHS_ALL (k_post + the projection too) is switched off and
-        raises."""
+        iteration; GRAPHS_ALL also k_post and the whole projection;
+        GRAPHS_OFF (default) the same launches one by one.  All three give
+        the same bits."""
         N.check(N.lib().v2h_batch_set_graphs(self._h, int(mode)))
 
     def kernel_times(self, full=False) -> dict:

@@ -247,6 +247,7 @@ __global__ void k_ref_state(BatchDev B, RefDev R) {
     const int wait = rs.wait;
     const bool run = active && rs.vvalid && wait == 0;
     rs.phase = run ? 1 : 0;
+    rs.live = active ? 1 : 0;
     if (active && wait > 0) rs.wait = wait - 1;
     R.route[b.gblock] = run ? kRouteRefining : kRouteFallback;
 }
@@ -417,7 +418,7 @@ __global__ __launch_bounds__(kNT) void k_ref_form_v(BatchDev B, RefDev R) {
     RefState& rs = R.rs[blockIdx.x];
     // blocks that took the fallback and try the refinement next iteration
     const int route = R.route[b.gblock];
-    if (!B.st[b.inst].active || (route != kRouteFallback && route != kRouteRejected) || rs.wait > 0) return;
+    if (!rs.live || (route != kRouteFallback && route != kRouteRejected) || rs.wait > 0) return;
     const int d = b.d, t = threadIdx.x;
     const int64_t dd = (int64_t)d * d;
     const double* hv = R.tri_work + b.tri;  // row k: reflector k (v[k + 1] = 1), tau_k at [k][k]

@@ -63,6 +63,11 @@ struct RefState {
     // iterations (doubling with consecutive rejections, up to kMaxBackoff);
     // its eigenvectors are formed only for the iteration before it tries again.
     int wait, backoff;
+    // The instance was running when this iteration's projection began (set
+    // by k_ref_state).  The eigenvector formation reads it instead of the
+    // instance state, which the next k_pre_reduce on the main stream may
+    // change while the formation runs on its side stream.
+    int live, pad;
 };
 constexpr int kMaxBackoff = 64;
 

@@ -483,10 +483,7 @@ void Batch::set_tuning(const Tuning& t) {
     if (t.xcd_remap != 0 && t.xcd_remap != 1) throw std::invalid_argument("tuning: xcd_remap is 0 or 1");
     if (t.pipeline < 0) throw std::invalid_argument("tuning: pipeline is 0 (lock-step) or an instance count");
     if (t.graphs < kGraphOff || t.graphs > kGraphAll)
-        throw std::invalid_argument("tuning: graphs is 0 (off) or 1 (CG chunk)");
-    if (t.graphs == kGraphAll)  // see kGraphAll
-        throw std::invalid_argument("tuning: graph mode 2 (CG chunk and projection) is switched off: solves split "
-                                    "into short calls did not always reproduce the eager bits");
+        throw std::invalid_argument("tuning: graphs is 0 (off), 1 (CG chunk) or 2 (CG chunk, post and projection)");
     V2H_HIP(hipSetDevice(device_));
     join_seed();  // the eigenvector formation in flight ran on the old refinement lane
     V2H_HIP(hipStreamSynchronize(stream_));

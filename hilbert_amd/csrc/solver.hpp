@@ -247,11 +247,9 @@ class Batch {
     // next CG graph.  Each graph is captured once per launch plan (the set of
     // running instances) and step count, then replayed with one
     // hipGraphLaunch; with the kernel timers on, the launches go one by one.
-    // kGraphOff (default): the same launches one by one.  kGraphCg gives the
-    // eager bits in every test.  kGraphAll is switched off (set_tuning
-    // rejects it): it gave the eager bits over whole solves, but a batch
-    // solved in calls of a few outer iterations sometimes stopped at other
-    // iterations, and the cause was not found.
+    // kGraphOff (default): the same launches one by one.  Both graph modes
+    // give the eager bits, also over solves split into short calls with the
+    // mode switched between them.
     static constexpr int kGraphOff = 0, kGraphCg = 1, kGraphAll = 2;
     void set_graphs(int mode);
     int graphs() const { return tuning_.graphs; }

@@ -291,8 +291,10 @@ int v2h_batch_set_profiling(v2h_batch *b, int on);
  * k_cg_a / k_cg_reduce launches) once per set of running instances and step
  * count and replays it with one hipGraphLaunch (with the kernel timers on,
  * v2h_batch_set_profiling, the launches go one by one); the results are the
- * bits of mode 0 (default, every kernel launched one by one).  Mode 2 (k_post and the projection in
- * the graph too) is switched off and rejected with V2H_ERR_INVALID_ARGUMENT.
+ * bits of mode 0 (default, every kernel launched one by one).  Mode 2 also
+ * captures k_post and the whole projection (forked onto the side streams and
+ * joined back), with the eigenvector formation as a branch of the next CG
+ * graph; the same bits again.
  * The libsdp loop it replaces launches nothing (v2rdm_solver.cc:1305-1361). */
 int v2h_batch_set_graphs(v2h_batch *b, int mode);
 

@@ -3,8 +3,10 @@
 Mode GRAPHS_CG captures the first CG chunk of every outer iteration (its
 k_cg_at / k_cg_a / k_cg_reduce launches) once per launch plan and step count
 and replays it (with the kernel timers off; with them on the launches go one
-by one).  Without graphs every kernel is launched one by one.  The
-kernels are the same and deterministic, so the two must agree bit for bit:
+by one); GRAPHS_ALL also captures k_post and the whole projection, with the
+eigenvector formation as a branch of the next CG graph.  Without graphs
+every kernel is launched one by one.  The kernels are the same and
+deterministic, so all three must agree bit for bit:
 
 * a batch of two (16e,16o) instances and two small ones, 400 outer iterations
   (hundreds of replays, interleaved with eager launches and host polls), with
@@ -12,13 +14,9 @@ kernels are the same and deterministic, so the two must agree bit for bit:
 * a mixed batch solved to convergence, whose instances stop at different
   iterations, so the plan shrinks and the graphs are captured again;
 * a batch solved in calls of 11 outer iterations, switching the mode at
-  every call.
-
-Mode GRAPHS_ALL (k_post and the projection in the graph as well) is
-switched off: it gave the eager bits over whole solves, but solves split
-into calls of a few outer iterations sometimes stopped at other iterations
-and the cause was not found.  It is rejected; its equality tests are kept,
-skipped, for when it is fixed.
+  every call (GRAPHS_ALL three times over: its eigenvector formation runs
+  after the last iteration's bookkeeping at the end of a call, and must
+  form the same eigenvectors as the eager launch order).
 """
 import numpy as np
 import pytest
@@ -30,7 +28,6 @@ from hilbert_amd.batch import Batch, options
 pytestmark = pytest.mark.gpu
 
 NEVER = dict(e_convergence=1e-14, r_convergence=1e-13, maxiter=1000000)
-MODE2_OFF = "graph mode 2 is switched off: short solve calls did not always reproduce the eager bits"
 
 
 def _items():
@@ -55,8 +52,7 @@ def _run(items, graphs, refine, iters, opts, profile=False):
     return out, times
 
 
-@pytest.mark.parametrize("mode", [Batch.GRAPHS_CG,
-                                  pytest.param(Batch.GRAPHS_ALL, marks=pytest.mark.skip(reason=MODE2_OFF))])
+@pytest.mark.parametrize("mode", [Batch.GRAPHS_CG, Batch.GRAPHS_ALL])
 @pytest.mark.parametrize("refine", [True, False])
 def test_graphs_equal_eager_launches(refine, mode):
     items = _items()
@@ -68,8 +64,7 @@ def test_graphs_equal_eager_launches(refine, mode):
             assert np.array_equal(ra[k], rb[k]), (i, k)
 
 
-@pytest.mark.parametrize("mode", [Batch.GRAPHS_CG,
-                                  pytest.param(Batch.GRAPHS_ALL, marks=pytest.mark.skip(reason=MODE2_OFF))])
+@pytest.mark.parametrize("mode", [Batch.GRAPHS_CG, Batch.GRAPHS_ALL])
 def test_graphs_follow_a_shrinking_plan(mode):
     items = configs.batch64_d2h(5) + configs.c2v(16, 1)
     opts = dict(e_convergence=1e-6, r_convergence=1e-5, maxiter=100000)
@@ -83,30 +78,33 @@ def test_graphs_follow_a_shrinking_plan(mode):
         assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1]), i
 
 
-def test_switching_graphs_between_short_calls():
+@pytest.mark.parametrize("mode,repeats", [(Batch.GRAPHS_CG, 1), (Batch.GRAPHS_ALL, 3)])
+def test_switching_graphs_between_short_calls(mode, repeats):
     items = configs.c2v(16, 4)
     opts = dict(e_convergence=1e-7, r_convergence=1e-6, maxiter=100000)
     ref, _ = _run(items, Batch.GRAPHS_OFF, True, None, opts)
-    B = Batch([p for p, _, _ in items], options(**opts))
-    for i, (_, h, g) in enumerate(items):
-        B.set_integrals(i, h, g)
-    B.solve(300)
-    for k in range(20):
-        B.set_graphs(Batch.GRAPHS_CG if k % 2 == 0 else Batch.GRAPHS_OFF)
-        B.solve(11)
-    B.solve_to_convergence()
-    for i, r in enumerate(ref):
-        st = B.status(i)
-        assert st["oiter"] == r[3]["oiter"] and st["iiter"] == r[3]["iiter"], i
-        assert np.array_equal(B.x(i), r[0]), i
-    B.close()
+    for rep in range(repeats):
+        B = Batch([p for p, _, _ in items], options(**opts))
+        for i, (_, h, g) in enumerate(items):
+            B.set_integrals(i, h, g)
+        B.solve(300)
+        for k in range(20):
+            B.set_graphs(mode if k % 2 == 0 else Batch.GRAPHS_OFF)
+            B.solve(11)
+        B.set_graphs(mode)
+        B.solve_to_convergence()
+        for i, r in enumerate(ref):
+            st = B.status(i)
+            assert st["oiter"] == r[3]["oiter"] and st["iiter"] == r[3]["iiter"], (rep, i)
+            assert np.array_equal(B.x(i), r[0]), (rep, i)
+        B.close()
 
 
-def test_graph_mode_2_is_rejected():
+def test_unknown_graph_mode_is_rejected():
     prob, h, g = configs.batch64_d2h(1)[0]
     B = Batch([prob], options())
-    with pytest.raises(V2hError, match="switched off"):
-        B.set_graphs(Batch.GRAPHS_ALL)
+    with pytest.raises(V2hError):
+        B.set_graphs(3)
     assert B.tuning()["graphs"] == Batch.GRAPHS_OFF
     B.close()
 
