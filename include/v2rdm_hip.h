@@ -319,7 +319,8 @@ typedef struct {
     int xcd_remap;        /* 1: gather workgroups dealt to the 8 XCDs in
                            * contiguous ranges (one instance per L2); 0
                            * (default): in launch order */
-    int graphs;           /* hipGraph replay: 0 off, 1 first CG chunk
+    int graphs;           /* hipGraph replay: 0 off, 1 first CG chunk, 2
+                           * also k_post and the projection
                            * (v2h_batch_set_graphs) */
     int pipeline;         /* > 0: while at least this many instances run,
                            * two groups of them take turns on the main stream
