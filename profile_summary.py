@@ -26,7 +26,8 @@ traffic -- from two counter runs of the same short bench command
     python profile_summary.py traffic --workload c2v16xB --copy-mib M --fetch <dir>/fetch --write <dir>/write
 
 (bench.py with --calibration-copy-mib M) averages FETCH_SIZE and WRITE_SIZE
-(KiB per dispatch) per kernel.  It scales both by the calibration factor of
+(KiB per dispatch) per kernel (summed over the counter's instances within a
+dispatch).  It scales both by the calibration factor of
 the largest k_copy dispatch, bench's plain copy of M MiB, which moves M MiB
 each way: factor = M MiB / measured bytes.  The bytes per launch of every kernel (fetch + write,
 scaled) go into profiles/traffic.json under the workload name, and a text
@@ -134,7 +135,7 @@ def traffic(args):
     if cfk <= 0.0 or cwk <= 0.0:
         raise SystemExit("no k_copy dispatch in the counter runs (bench.py --calibration-copy-mib)")
     cf, cw = copy_bytes / cfk, copy_bytes / cwk
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    path = getattr(args, "out", None) or os.path.join(ROOT, "profiles", "traffic.json")
     os.makedirs(os.path.dirname(path), exist_ok=True)
     data = json.load(open(path)) if os.path.exists(path) else {}
     rec = {}
@@ -165,6 +166,7 @@ def main():
     t.add_argument("--write", required=True)
     t.add_argument("--copy-mib", type=int, required=True,
                    help="size of bench.py's calibration copy (--calibration-copy-mib)")
+    t.add_argument("--out", default=None, help="JSON file to update (default profiles/traffic.json)")
     args = ap.parse_args()
     if args.cmd == "stats":
         stats(args.dir)
