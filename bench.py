@@ -693,10 +693,14 @@ def main():
         reset_all(B)
         if w == 0 and args.autotune:
             # the tuner times its candidates on the live warm-up solve (its
-            # knobs change no result bit), after the first outer iterations
+            # knobs change no result bit), after the first outer iterations;
+            # with the kernel timers off, since with them on the graph modes
+            # launch one by one
             from hilbert_amd import autotune
             B.solve(args.autotune)
+            B.set_profiling(False)
             tuned = autotune.tune(B, log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+            B.set_profiling(True)
         solve_with_progress(B, f"warmup {w}")
         if grads:
             gradients(B)

@@ -86,7 +86,9 @@ def _running(B) -> bool:
 
 def tune(B, window: int = 20, rounds: int = 3, cache: str | None = None, log=None) -> dict:
     """Time the candidates on the live solve of batch B (best after its first
-    few hundred outer iterations, when the per-iteration work has settled),
+    few hundred outer iterations, when the per-iteration work has settled,
+    and with the kernel timers off: with them on, the graph modes launch one
+    by one),
     set the fastest configuration, store it under B's signature and return
     {"tuning": cfg, "ms_per_iteration": {knob: {value: ms}}, "signature"}.
     Stops early, keeping the best so far, if the batch converges."""
